@@ -1,0 +1,52 @@
+"""ctypes declarations of include/cbm_hip.h."""
+import ctypes
+
+c_i32, c_i64, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
+
+# enum cbm_mode / cbm_dtype
+MODE_AX, MODE_ADX, MODE_DADX = 0, 1, 2
+F32, BF16 = 0, 1
+
+
+class CbmDesc(ctypes.Structure):
+    """struct cbm_desc (include/cbm_hip.h)."""
+    _fields_ = [
+        ("n_rows", c_i32),
+        ("n_levels", c_i32),
+        ("nnz", c_i64),
+        ("n_edges", c_i64),
+        ("mode", c_i32),
+        ("lvl_ptr_host", c_vp),
+        ("lvl_node", c_vp),
+        ("lvl_parent", c_vp),
+        ("fwd_ptr", c_vp),
+        ("fwd_col", c_vp),
+        ("bwd_ptr", c_vp),
+        ("bwd_row", c_vp),
+        ("ch_ptr", c_vp),
+        ("ch_node", c_vp),
+        ("d", c_vp),
+    ]
+
+
+def declare(lib, sig):
+    P_vp = ctypes.POINTER(c_vp)
+    sig(lib, "cbm_last_error", ctypes.c_char_p)
+    sig(lib, "cbm_create", c_i32, ctypes.POINTER(CbmDesc), c_vp, P_vp)
+    sig(lib, "cbm_destroy", c_i32, c_vp)
+    sig(lib, "cbm_matmul", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
+    sig(lib, "cbm_matmul_t", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
+    sig(lib, "cbm_delta_spmm", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
+    sig(lib, "cbm_propagate", c_i32, c_vp, c_vp, c_i64, c_i32, c_vp)
+    sig(lib, "cbm_last_launches", c_i32, c_vp)
+    sig(lib, "csr_last_error", ctypes.c_char_p)
+    sig(lib, "csr_create", c_i32, c_i32, c_i32, c_i64, c_vp, c_vp, c_vp, P_vp)
+    sig(lib, "csr_destroy", c_i32, c_vp)
+    sig(lib, "csr_set_algorithm", c_i32, c_vp, c_i32)
+    sig(lib, "csr_matmul", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
+
+
+# Every symbol include/cbm_hip.h declares (checked by the CPU test suite).
+EXPORTED = ("cbm_last_error", "cbm_create", "cbm_destroy", "cbm_matmul", "cbm_matmul_t", "cbm_delta_spmm",
+            "cbm_propagate", "cbm_last_launches", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
+            "csr_last_error")

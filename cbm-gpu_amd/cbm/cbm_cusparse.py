@@ -1,0 +1,247 @@
+"""MI355X counterpart of the reference extension module ``cbm_cusparse``
+(cbm/cbm.cu:302-310): class :class:`cusparseCBM` with the same constructor
+and ``matmul_AX_ADX`` / ``matmul_DAX_DADX`` methods, backed by the HIP kernels
+of libcbm_hip.so instead of cuSPARSE + the ``update`` kernels.
+
+Additions that leave the reference surface intact: BF16 right-hand sides,
+the adjoint :meth:`cusparseCBM.matmul_t`, and the two halves of the forward
+product (:meth:`delta_spmm`, :meth:`propagate`) for separate timing.
+"""
+import ctypes
+
+import torch
+
+from . import _lib, structure
+from ._hip_sigs import BF16, F32, MODE_AX, MODE_ADX, MODE_DADX, CbmDesc
+
+_MODE_NAMES = {MODE_AX: "A", MODE_ADX: "A D^-1/2", MODE_DADX: "D^-1/2 A D^-1/2"}
+
+
+def _require_cuda(t):
+    # Same message as the reference's TORCH_CHECK (cbm/cbm.cu:101-108,151-152).
+    if not t.is_cuda:
+        raise RuntimeError("tensor is not a CUDA tensor")
+
+
+def current_stream_handle(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _dtype_code(t):
+    if t.dtype == torch.float32:
+        return F32
+    if t.dtype == torch.bfloat16:
+        return BF16
+    raise TypeError(f"right-hand side must be float32 or bfloat16, got {t.dtype}")
+
+
+def infer_scaling(n, crow, col, val, multipliers):
+    """Reads the product a reference-style ``deltas`` tensor encodes.
+
+    * ``multipliers`` non-empty: values are +-D[row]*D[col]   -> D A D  (cbm/cbm4dad.py:25-34)
+    * all values +-1:                                          -> A      (cbm/cbm4mm.py)
+    * values +-c[col], one magnitude per column:               -> A D    (cbm/cbm4ad.py:25)
+    Anything else is not a CBM product and raises ``ValueError``.
+    """
+    mag = val.abs()
+    if multipliers is not None and multipliers.numel():
+        d = multipliers.reshape(-1).to(torch.float32)
+        if d.numel() != n:
+            raise ValueError(f"multipliers has {d.numel()} entries, expected {n}")
+        rows = torch.repeat_interleave(torch.arange(n, device=crow.device), crow.diff().long())
+        expect = d[rows] * d[col.long()]
+        if mag.numel() and not torch.allclose(mag, expect, rtol=1e-5, atol=0.0):
+            raise ValueError("deltas values are not +-multipliers[row]*multipliers[col]")
+        return MODE_DADX, d
+    if mag.numel() == 0 or bool((mag == 1).all()):
+        return MODE_AX, None
+    colv = col.long()
+    hi = torch.zeros(n, dtype=torch.float32, device=mag.device).scatter_reduce(0, colv, mag, "amax",
+                                                                               include_self=False)
+    lo = torch.full((n,), float("inf"), dtype=torch.float32, device=mag.device).scatter_reduce(
+        0, colv, mag, "amin", include_self=False)
+    used = torch.bincount(colv, minlength=n) > 0
+    if not torch.equal(hi[used], lo[used]):
+        raise ValueError("deltas values are neither +-1, +-scale[col] nor +-multipliers[row]*multipliers[col]")
+    return MODE_ADX, hi
+
+
+class cusparseCBM:
+    """CBM matrix on the GPU.  Same constructor as cbm/cbm.cu:95-142:
+    ``cusparseCBM(deltas, mca_branches, mca_src_node_idx, mca_dst_node_idx, multipliers)``
+    where ``deltas`` is a CUDA sparse CSR tensor (int32 indices, float32 values)
+    and ``multipliers`` is D^-1/2 for the D A D product or an empty tensor.
+
+    Keyword-only ``mode`` (0: A, 1: A D, 2: D A D) and ``scale`` (D^-1/2) skip
+    the inference of the product from the values.  The handle owns device
+    copies of its layout; nothing is allocated per call.
+    """
+
+    def __init__(self, deltas, mca_branches, mca_src_node_idx, mca_dst_node_idx, multipliers, *, mode=None,
+                 scale=None, n_threads=0):
+        for t in (deltas, mca_branches, mca_src_node_idx, mca_dst_node_idx):
+            _require_cuda(t)
+        if multipliers is not None and multipliers.numel():
+            _require_cuda(multipliers)
+        if deltas.layout != torch.sparse_csr:
+            raise TypeError("deltas must be a sparse CSR tensor")
+        n = int(deltas.size(0))
+        if deltas.dim() != 2 or int(deltas.size(1)) != n:
+            raise ValueError(f"deltas must be square, got {tuple(deltas.shape)}")
+        crow, col, val = deltas.crow_indices(), deltas.col_indices(), deltas.values()
+        for t, dt, name in ((crow, torch.int32, "crow_indices"), (col, torch.int32, "col_indices"),
+                            (val, torch.float32, "values"), (mca_src_node_idx, torch.int32, "mca_src_node_idx"),
+                            (mca_dst_node_idx, torch.int32, "mca_dst_node_idx")):
+            if t.dtype != dt:
+                raise TypeError(f'"{name}" is not a tensor of type "{dt}"')
+        if mode is None:
+            mode, scale = infer_scaling(n, crow, col, val, multipliers)
+        elif mode != MODE_AX and scale is None:
+            raise ValueError(f"mode {mode} needs the D^-1/2 vector as `scale`")
+        self.device = deltas.device
+        plan = structure.plan(n, crow.cpu(), col.cpu(), val.cpu(), mca_src_node_idx.cpu(), mca_dst_node_idx.cpu(),
+                              n_threads=n_threads)
+        self.num_rows = n
+        self.depth = plan.depth
+        self.nnz = int(col.numel())
+        self.n_edges = int(mca_src_node_idx.numel())
+        self.level_sizes = plan.lvl_ptr.diff().tolist()
+        self._lvl_ptr_host = plan.lvl_ptr.contiguous()
+        self._dev = {k: v.to(self.device) for k, v in plan.arrays.items() if k != "lvl_ptr"}
+        self._lib = _lib.hip()
+        self._bind(mode, scale)
+
+    def _bind(self, mode, scale):
+        """Creates the native handle for product `mode` over this layout."""
+        self.mode = int(mode)
+        self._d = None if self.mode == MODE_AX else scale.to(self.device, torch.float32).contiguous()
+        dp = {k: v.data_ptr() for k, v in self._dev.items()}
+        desc = CbmDesc(n_rows=self.num_rows, n_levels=self.depth, nnz=self.nnz, n_edges=self.n_edges,
+                       mode=self.mode, lvl_ptr_host=self._lvl_ptr_host.data_ptr(), lvl_node=dp["lvl_node"],
+                       lvl_parent=dp["lvl_parent"], fwd_ptr=dp["fwd_ptr"], fwd_col=dp["fwd_col"] or None,
+                       bwd_ptr=dp["bwd_ptr"], bwd_row=dp["bwd_row"] or None, ch_ptr=dp["ch_ptr"],
+                       ch_node=dp["ch_node"] or None, d=self._d.data_ptr() if self._d is not None else None)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            rc = self._lib.cbm_create(ctypes.byref(desc), current_stream_handle(self.device), ctypes.byref(h))
+        _lib.check_hip(rc, "cusparseCBM")
+        self._h = h
+
+    def with_scaling(self, mode, scale=None):
+        """A handle for another product (A, A D or D A D) over the same device
+        layout; the layout arrays are shared, not copied."""
+        other = object.__new__(type(self))
+        for k in ("device", "num_rows", "depth", "nnz", "n_edges", "level_sizes", "_lvl_ptr_host", "_dev", "_lib"):
+            setattr(other, k, getattr(self, k))
+        if mode != MODE_AX and scale is None:
+            raise ValueError(f"mode {mode} needs the D^-1/2 vector as `scale`")
+        other._bind(mode, scale)
+        return other
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.cbm_destroy(h)
+            self._h = None
+
+    @property
+    def D(self):
+        """D^-1/2 (FP32, zero for degree-0 nodes) or None for the plain product."""
+        return self._d
+
+    def describe(self):
+        return {"n": self.num_rows, "nnz_delta": self.nnz, "tree_edges": self.n_edges, "depth": self.depth,
+                "product": _MODE_NAMES[self.mode]}
+
+    # ---------------------------------------------------------------- checks
+    def _check_in(self, x):
+        _require_cuda(x)
+        if x.dim() != 2 or int(x.size(0)) != self.num_rows:
+            raise ValueError(f"operand must be {self.num_rows} x F, got {tuple(x.shape)}")
+        if x.device != self.device:
+            raise ValueError(f"operand is on {x.device}, the matrix on {self.device}")
+        code = _dtype_code(x)
+        if x.stride(1) != 1 or x.stride(0) < x.size(1):
+            x = x.contiguous()
+        return x, code
+
+    def _check_out(self, y, n_cols, name="res"):
+        _require_cuda(y)
+        if y.dtype != torch.float32:
+            raise TypeError(f'"{name}" is not a tensor of type "torch.float32"')
+        if y.dim() != 2 or tuple(y.shape) != (self.num_rows, n_cols):
+            raise ValueError(f"{name} must be {self.num_rows} x {n_cols}, got {tuple(y.shape)}")
+        if y.device != self.device:
+            raise ValueError(f"{name} is on {y.device}, the matrix on {self.device}")
+        if y.stride(1) != 1 or y.stride(0) < n_cols:
+            raise ValueError(f"{name} must have unit column stride")
+
+    # --------------------------------------------------------------- products
+    def forward(self, rhs, res):
+        """res = M rhs for this handle's product M (A, A D or D A D)."""
+        x, code = self._check_in(rhs)
+        f = int(x.size(1))
+        self._check_out(res, f)
+        if f == 0:
+            return 0
+        rc = self._lib.cbm_matmul(self._h, code, x.data_ptr(), x.stride(0), res.data_ptr(), res.stride(0), f,
+                                  current_stream_handle(self.device))
+        _lib.check_hip(rc, "matmul")
+        return 0
+
+    def matmul_AX_ADX(self, rhs, res):
+        """res = A rhs or A D^-1/2 rhs (cbm/cbm.cu:150-207)."""
+        _require_cuda(rhs)
+        _require_cuda(res)
+        if self.mode == MODE_DADX:
+            raise ValueError("this matrix carries D^-1/2 A D^-1/2 scalings; use matmul_DAX_DADX")
+        return self.forward(rhs, res)
+
+    def matmul_DAX_DADX(self, rhs, res):
+        """res = D^-1/2 A D^-1/2 rhs (cbm/cbm.cu:210-268)."""
+        _require_cuda(rhs)
+        _require_cuda(res)
+        if self.mode != MODE_DADX:
+            raise ValueError("this matrix was built without multipliers; use matmul_AX_ADX")
+        return self.forward(rhs, res)
+
+    def matmul_t(self, grad, out, workspace=None):
+        """out = M^T grad.  ``workspace`` (FP32, N x F) is allocated if not given."""
+        g, code = self._check_in(grad)
+        f = int(g.size(1))
+        self._check_out(out, f, "out")
+        if f == 0:
+            return 0
+        if workspace is None:
+            workspace = torch.empty((self.num_rows, f), dtype=torch.float32, device=self.device)
+        else:
+            self._check_out(workspace, f, "workspace")
+        rc = self._lib.cbm_matmul_t(self._h, code, g.data_ptr(), g.stride(0), out.data_ptr(), out.stride(0),
+                                    workspace.data_ptr(), workspace.stride(0), f, current_stream_handle(self.device))
+        _lib.check_hip(rc, "matmul_t")
+        return 0
+
+    def delta_spmm(self, rhs, res):
+        """res = Delta' rhs without the tree propagation."""
+        x, code = self._check_in(rhs)
+        f = int(x.size(1))
+        self._check_out(res, f)
+        if f == 0:
+            return 0
+        rc = self._lib.cbm_delta_spmm(self._h, code, x.data_ptr(), x.stride(0), res.data_ptr(), res.stride(0), f,
+                                      current_stream_handle(self.device))
+        _lib.check_hip(rc, "delta_spmm")
+        return 0
+
+    def propagate(self, res):
+        """In place, level by level: res_v += s_v res_parent(v) (cbm/cbm.cu:25-91)."""
+        f = int(res.size(1)) if res.dim() == 2 else -1
+        self._check_out(res, f)
+        if f == 0:
+            return 0
+        rc = self._lib.cbm_propagate(self._h, res.data_ptr(), res.stride(0), f, current_stream_handle(self.device))
+        _lib.check_hip(rc, "propagate")
+        return 0
+
+    def last_launches(self):
+        return int(self._lib.cbm_last_launches(self._h))

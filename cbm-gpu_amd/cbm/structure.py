@@ -1,0 +1,135 @@
+"""Host-side CBM structure: synthetic graphs, the ``init`` build and the
+device plan, as CPU torch tensors.  Thin wrappers over libcbm_host.so."""
+import ctypes
+from dataclasses import dataclass, field
+
+import torch
+
+from . import _lib
+
+# Array and statistic slots of include/cbm_host.h.
+(_B_DELTA_ROW, _B_DELTA_COL, _B_DELTA_VAL, _B_BRANCHES, _B_TREE_SRC, _B_TREE_DST, _B_PARENT, _B_CSR_PTR, _B_CSR_COL,
+ _B_CSR_VAL) = range(10)
+_STAT_NAMES = ("n", "nnz_a", "nnz_delta", "candidates", "msa_weight", "tree_edges", "branches", "depth")
+_PLAN_NAMES = ("lvl_ptr", "lvl_node", "lvl_parent", "fwd_ptr", "fwd_col", "bwd_ptr", "bwd_row", "ch_ptr",
+               "ch_node")
+
+
+def _copy_out(ptr, n, dtype):
+    t = torch.empty(int(n), dtype=dtype)
+    if n > 0:
+        ctypes.memmove(t.data_ptr(), ptr, int(n) * t.element_size())
+    return t
+
+
+def _cpu_contig(t, dtype, name):
+    if not isinstance(t, torch.Tensor):
+        t = torch.as_tensor(t)
+    if t.dtype != dtype:
+        raise TypeError(f'"{name}" is not a tensor of type "{dtype}"')
+    return t.detach().to("cpu").contiguous()
+
+
+def copying_model(n_nodes, n_blocks, p_copy, lam, seed_deg=0, seed=0):
+    """Seeded community copying model; returns ``edge_index`` int32 ``[2, nnz]``
+    (CPU), symmetric, coalesced, sorted by (row, col), without self-loops."""
+    lib = _lib.host()
+    g = ctypes.c_void_p()
+    rc = lib.cbm_graph_copying_model(int(n_nodes), int(n_blocks), float(p_copy), float(lam), int(seed_deg),
+                                     int(seed) & (2**64 - 1), ctypes.byref(g))
+    _lib.check_host(rc, "copying_model")
+    try:
+        nnz = lib.cbm_graph_nnz(g)
+        ei = torch.empty((2, nnz), dtype=torch.int32)
+        if nnz:
+            ctypes.memmove(ei[0].data_ptr(), lib.cbm_graph_row(g), nnz * 4)
+            ctypes.memmove(ei[1].data_ptr(), lib.cbm_graph_col(g), nnz * 4)
+        return ei
+    finally:
+        lib.cbm_graph_free(g)
+
+
+@dataclass
+class Build:
+    """Result of the host build (the six ``init`` arrays plus extras)."""
+    delta_row: torch.Tensor
+    delta_col: torch.Tensor
+    delta_val: torch.Tensor
+    branches: torch.Tensor
+    tree_src: torch.Tensor
+    tree_dst: torch.Tensor
+    parent: torch.Tensor
+    csr_ptr: torch.Tensor
+    csr_col: torch.Tensor
+    csr_val: torch.Tensor
+    stats: dict = field(default_factory=dict)
+
+    def init_tuple(self):
+        """The list ``cbm_mkl_cpp.init`` returns (cbm/cbm_extensions.cpp:300-344)."""
+        return [self.delta_row, self.delta_col, self.delta_val, self.branches, self.tree_src, self.tree_dst]
+
+
+def build(row, col, val, n_rows, alpha, n_threads=0):
+    """Builds the CBM structure of the binary matrix given as COO."""
+    row = _cpu_contig(row, torch.int32, "src_row_idx")
+    col = _cpu_contig(col, torch.int32, "src_col_idx")
+    val = _cpu_contig(val, torch.float32, "src_values")
+    nnz = row.numel()
+    if col.numel() != nnz or val.numel() != nnz:
+        raise ValueError(f"row/col/val lengths differ: {row.numel()}, {col.numel()}, {val.numel()}")
+    lib = _lib.host()
+    b = ctypes.c_void_p()
+    rc = lib.cbm_build_create(row.data_ptr(), col.data_ptr(), val.data_ptr(), nnz, int(n_rows), int(alpha),
+                              int(n_threads), ctypes.byref(b))
+    _lib.check_host(rc, "init")
+    try:
+        def arr(which, dtype):
+            return _copy_out(lib.cbm_build_array(b, which), lib.cbm_build_size(b, which), dtype)
+        out = Build(arr(_B_DELTA_ROW, torch.int32), arr(_B_DELTA_COL, torch.int32), arr(_B_DELTA_VAL, torch.float32),
+                    arr(_B_BRANCHES, torch.int32), arr(_B_TREE_SRC, torch.int32), arr(_B_TREE_DST, torch.int32),
+                    arr(_B_PARENT, torch.int32), arr(_B_CSR_PTR, torch.int32), arr(_B_CSR_COL, torch.int32),
+                    arr(_B_CSR_VAL, torch.float32))
+        out.stats = {name: int(lib.cbm_build_stat(b, i)) for i, name in enumerate(_STAT_NAMES)}
+        return out
+    finally:
+        lib.cbm_build_free(b)
+
+
+@dataclass
+class Plan:
+    """Device layout of a CBM structure (see csrc/host/plan.cpp)."""
+    n: int
+    depth: int
+    arrays: dict
+
+    def __getattr__(self, name):
+        arrays = self.__dict__.get("arrays")
+        if arrays is not None and name in arrays:
+            return arrays[name]
+        raise AttributeError(name)
+
+
+def plan(n_rows, crow, col, val, tree_src, tree_dst, n_threads=0):
+    """Lays out Delta (CSR with +-scale values) and the tree for the kernels."""
+    crow = _cpu_contig(crow, torch.int32, "crow_indices")
+    col = _cpu_contig(col, torch.int32, "col_indices")
+    val = _cpu_contig(val, torch.float32, "values")
+    tree_src = _cpu_contig(tree_src, torch.int32, "mca_src_idx")
+    tree_dst = _cpu_contig(tree_dst, torch.int32, "mca_dst_idx")
+    n_rows = int(n_rows)
+    if crow.numel() != n_rows + 1:
+        raise ValueError(f"crow_indices has {crow.numel()} entries, expected {n_rows + 1}")
+    if tree_src.numel() != tree_dst.numel():
+        raise ValueError("mca_src_idx and mca_dst_idx differ in length")
+    lib = _lib.host()
+    p = ctypes.c_void_p()
+    rc = lib.cbm_plan_create(n_rows, crow.data_ptr(), col.data_ptr(), val.data_ptr(), col.numel(),
+                             tree_src.data_ptr(), tree_dst.data_ptr(), tree_src.numel(), int(n_threads),
+                             ctypes.byref(p))
+    _lib.check_host(rc, "plan")
+    try:
+        arrays = {name: _copy_out(lib.cbm_plan_array(p, i), lib.cbm_plan_size(p, i), torch.int32)
+                  for i, name in enumerate(_PLAN_NAMES)}
+        return Plan(n=n_rows, depth=int(lib.cbm_plan_depth(p)), arrays=arrays)
+    finally:
+        lib.cbm_plan_free(p)

@@ -1,0 +1,344 @@
+// HIP kernels of the CBM product for gfx950 (CDNA4, wave64).
+//
+// All work is one kind of pass over rows: gather a short list of rows of a
+// dense N x F matrix with +-1 (times an optional per-row scale), optionally
+// add one more scaled row, store an FP32 row.  The forward product runs it
+// once per tree level (Delta row + parent row, parents finished by the
+// previous level's launch); the adjoint runs it once per level from the
+// deepest up over child lists, then once over Delta^T.
+//
+// Mapping: a group of LPR lanes (a power of two <= 64) owns one output row;
+// each lane owns VE consecutive columns, loaded as one 16-byte vector (4 FP32
+// or 8 BF16 values), so a row of F columns is covered by F/VE lanes and a
+// wavefront holds 64/LPR rows.  The group's lanes fetch LPR packed indices in
+// one coalesced load and broadcast them with cross-lane moves
+// (ds_bpermute, or v_readlane when a row owns the whole wavefront); four
+// gathers are kept in flight per lane.  Accumulation is FP32 with fma, so a
+// +-1 entry is added exactly as an add or a subtract.
+#include "cbm_kernels.hpp"
+
+namespace cbmk {
+namespace {
+
+using bf16_t = uint16_t;  // storage type of bfloat16 values
+
+constexpr uint32_t kSign = 0x80000000u;
+constexpr int32_t kIndexMask = 0x7fffffff;
+
+__device__ inline float bf16_to_f32(uint32_t bits16) { return __uint_as_float(bits16 << 16); }
+
+// Loads VE consecutive values starting at p as FP32.
+template <typename T, int VE>
+struct Load;
+
+template <>
+struct Load<float, 1> {
+    __device__ static void run(const float *p, float (&v)[1]) { v[0] = *p; }
+};
+template <>
+struct Load<float, 4> {
+    __device__ static void run(const float *p, float (&v)[4]) {
+        const float4 t = *reinterpret_cast<const float4 *>(p);
+        v[0] = t.x;
+        v[1] = t.y;
+        v[2] = t.z;
+        v[3] = t.w;
+    }
+};
+template <>
+struct Load<float, 8> {
+    __device__ static void run(const float *p, float (&v)[8]) {
+        const float4 a = reinterpret_cast<const float4 *>(p)[0];
+        const float4 b = reinterpret_cast<const float4 *>(p)[1];
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = a.w;
+        v[4] = b.x;
+        v[5] = b.y;
+        v[6] = b.z;
+        v[7] = b.w;
+    }
+};
+template <>
+struct Load<bf16_t, 1> {
+    __device__ static void run(const bf16_t *p, float (&v)[1]) { v[0] = bf16_to_f32(*p); }
+};
+template <>
+struct Load<bf16_t, 4> {
+    __device__ static void run(const bf16_t *p, float (&v)[4]) {
+        const uint2 t = *reinterpret_cast<const uint2 *>(p);
+        v[0] = bf16_to_f32(t.x & 0xffffu);
+        v[1] = bf16_to_f32(t.x >> 16);
+        v[2] = bf16_to_f32(t.y & 0xffffu);
+        v[3] = bf16_to_f32(t.y >> 16);
+    }
+};
+template <>
+struct Load<bf16_t, 8> {
+    __device__ static void run(const bf16_t *p, float (&v)[8]) {
+        const uint4 t = *reinterpret_cast<const uint4 *>(p);
+        v[0] = bf16_to_f32(t.x & 0xffffu);
+        v[1] = bf16_to_f32(t.x >> 16);
+        v[2] = bf16_to_f32(t.y & 0xffffu);
+        v[3] = bf16_to_f32(t.y >> 16);
+        v[4] = bf16_to_f32(t.z & 0xffffu);
+        v[5] = bf16_to_f32(t.z >> 16);
+        v[6] = bf16_to_f32(t.w & 0xffffu);
+        v[7] = bf16_to_f32(t.w >> 16);
+    }
+};
+
+template <int VE>
+__device__ inline void store_row(float *p, const float (&v)[VE]) {
+    if constexpr (VE == 1) {
+        *p = v[0];
+    } else {
+#pragma unroll
+        for (int q = 0; q < VE; q += 4)
+            *reinterpret_cast<float4 *>(p + q) = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
+    }
+}
+
+// Value of lane k of the calling lane's group of LPR lanes.
+template <int LPR>
+__device__ inline int32_t group_bcast(int32_t v, int32_t k) {
+    if constexpr (LPR == 1) {
+        return v;
+    } else if constexpr (LPR == 64) {
+        return __builtin_amdgcn_readlane(v, k);
+    } else {
+        return __shfl(v, k, LPR);
+    }
+}
+
+template <int LPR>
+__device__ inline float group_bcast(float v, int32_t k) {
+    return __int_as_float(group_bcast<LPR>(__float_as_int(v), k));
+}
+
+// +-scale: the packed index's bit 31 flips the sign of the scale.
+__device__ inline float signed_scale(float scale, int32_t packed) {
+    return __uint_as_float(__float_as_uint(scale) ^ (uint32_t(packed) & kSign));
+}
+
+template <typename Tg, typename Te, int VE, int LPR, int EXTRA>
+__global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
+    constexpr int kGroupsPerWave = 64 / LPR;
+    const int lane = int(threadIdx.x & 63u);
+    const int l = lane & (LPR - 1);
+    const int64_t group =
+        (int64_t(blockIdx.x) * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kGroupsPerWave + lane / LPR;
+    const int64_t p64 = int64_t(a.p_begin) + group;
+    if (p64 >= int64_t(a.p_end)) return;  // whole groups leave together
+    const int32_t p = int32_t(p64);
+    const int32_t row = a.node_of ? a.node_of[p] : p;
+    const int32_t kb = a.ptr[p], ke = a.ptr[p + 1];
+    const Tg *__restrict__ g = static_cast<const Tg *>(a.g);
+    const float rs = a.rowscale ? a.rowscale[p] : 1.0f;
+    constexpr int32_t kSpan = LPR * VE;
+    const int32_t passes = (a.ncols + kSpan - 1) / kSpan;
+
+    for (int32_t pass = 0; pass < passes; ++pass) {
+        const int32_t c0 = pass * kSpan + l * VE;
+        const bool active = c0 < a.ncols;
+        float acc[VE];
+#pragma unroll
+        for (int q = 0; q < VE; ++q) acc[q] = 0.0f;
+
+        for (int32_t base = kb; base < ke; base += LPR) {
+            int32_t mine = 0;
+            float mine_scale = 0.0f;
+            if (base + l < ke) {
+                mine = a.idx[base + l];
+                const float cs = a.colscale ? a.colscale[mine & kIndexMask] : 1.0f;
+                mine_scale = signed_scale(cs, mine);
+            }
+            const int32_t cnt = min(LPR, ke - base);
+            int32_t k = 0;
+            for (; k + 4 <= cnt; k += 4) {
+                int32_t j[4];
+                float sc[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    j[u] = group_bcast<LPR>(mine, k + u) & kIndexMask;
+                    sc[u] = group_bcast<LPR>(mine_scale, k + u);
+                }
+                if (active) {
+                    float v[4][VE];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) Load<Tg, VE>::run(g + int64_t(j[u]) * a.ldg + c0, v[u]);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[u][q], sc[u], acc[q]);
+                }
+            }
+            for (; k < cnt; ++k) {
+                const int32_t j = group_bcast<LPR>(mine, k) & kIndexMask;
+                const float sc = group_bcast<LPR>(mine_scale, k);
+                if (active) {
+                    float v[VE];
+                    Load<Tg, VE>::run(g + int64_t(j) * a.ldg + c0, v);
+#pragma unroll
+                    for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], sc, acc[q]);
+                }
+            }
+        }
+        if (!active) continue;
+#pragma unroll
+        for (int q = 0; q < VE; ++q) acc[q] *= rs;
+        if constexpr (EXTRA == kExtraParent) {
+            const int32_t par = a.parent_of[p];
+            if (par >= 0) {
+                const float es = a.escale ? a.escale[p] : 1.0f;
+                float v[VE];
+                Load<Te, VE>::run(static_cast<const Te *>(a.e) + int64_t(par) * a.lde + c0, v);
+#pragma unroll
+                for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], es, acc[q]);
+            }
+        } else if constexpr (EXTRA == kExtraSelf) {
+            const float es = a.escale ? a.escale[row] : 1.0f;
+            float v[VE];
+            Load<Te, VE>::run(static_cast<const Te *>(a.e) + int64_t(row) * a.lde + c0, v);
+#pragma unroll
+            for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], es, acc[q]);
+        }
+        store_row<VE>(a.y + int64_t(row) * a.ldy + c0, acc);
+    }
+}
+
+template <int VE, int LPR>
+__global__ __launch_bounds__(256) void k_propagate(int32_t p_begin, int32_t p_end, const int32_t *__restrict__ node_of,
+                                                   const int32_t *__restrict__ parent_of,
+                                                   const float *__restrict__ parscale, float *y, int64_t ldy,
+                                                   int32_t ncols) {
+    constexpr int kGroupsPerWave = 64 / LPR;
+    const int lane = int(threadIdx.x & 63u);
+    const int l = lane & (LPR - 1);
+    const int64_t group =
+        (int64_t(blockIdx.x) * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kGroupsPerWave + lane / LPR;
+    const int64_t p64 = int64_t(p_begin) + group;
+    if (p64 >= int64_t(p_end)) return;
+    const int32_t p = int32_t(p64);
+    const int32_t par = parent_of[p];
+    if (par < 0) return;
+    const int32_t row = node_of[p];
+    const float ps = parscale ? parscale[p] : 1.0f;
+    for (int32_t c0 = l * VE; c0 < ncols; c0 += LPR * VE) {
+        float a[VE], b[VE];
+        Load<float, VE>::run(y + int64_t(par) * ldy + c0, a);
+        Load<float, VE>::run(y + int64_t(row) * ldy + c0, b);
+#pragma unroll
+        for (int q = 0; q < VE; ++q) b[q] = fmaf(a[q], ps, b[q]);
+        store_row<VE>(y + int64_t(row) * ldy + c0, b);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dad_scales(int32_t n, const int32_t *__restrict__ node_of,
+                                                    const int32_t *__restrict__ parent_of, const float *__restrict__ d,
+                                                    float *__restrict__ rs, float *__restrict__ ps) {
+    const int32_t p = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= n) return;
+    const float dv = d[node_of[p]];
+    const int32_t par = parent_of[p];
+    rs[p] = dv;
+    const float dp = par >= 0 ? d[par] : 0.0f;
+    ps[p] = dp != 0.0f ? dv / dp : 0.0f;
+}
+
+int lanes_per_row(int32_t ncols, int ve) {
+    const int32_t need = (ncols + ve - 1) / ve;
+    int lpr = 1;
+    while (lpr < need && lpr < 64) lpr <<= 1;
+    return lpr;
+}
+
+bool aligned16(const void *p, int64_t ld, int elem) {
+    return (reinterpret_cast<uintptr_t>(p) & 15u) == 0 && ((ld * elem) & 15) == 0;
+}
+
+template <typename Tg, typename Te, int VE, int EXTRA>
+hipError_t dispatch_lpr(const RowPass &rp, int lpr, hipStream_t stream) {
+    const int64_t n = int64_t(rp.p_end) - rp.p_begin;
+    if (n <= 0) return hipSuccess;
+    const int64_t rows_per_block = 4 * (64 / lpr);
+    const dim3 grid(unsigned((n + rows_per_block - 1) / rows_per_block)), block(256);
+    switch (lpr) {
+#define CBM_LPR_CASE(L) \
+    case L: k_row_pass<Tg, Te, VE, L, EXTRA><<<grid, block, 0, stream>>>(rp); break;
+        CBM_LPR_CASE(1)
+        CBM_LPR_CASE(2)
+        CBM_LPR_CASE(4)
+        CBM_LPR_CASE(8)
+        CBM_LPR_CASE(16)
+        CBM_LPR_CASE(32)
+        CBM_LPR_CASE(64)
+#undef CBM_LPR_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename Tg, typename Te, int EXTRA>
+hipError_t dispatch_ve(const RowPass &rp, hipStream_t stream) {
+    constexpr int kVe = 16 / int(sizeof(Tg));
+    bool vec = (rp.ncols % kVe) == 0 && aligned16(rp.g, rp.ldg, int(sizeof(Tg))) && aligned16(rp.y, rp.ldy, 4);
+    if (EXTRA != kExtraNone) vec = vec && aligned16(rp.e, rp.lde, int(sizeof(Te)));
+    if (vec) return dispatch_lpr<Tg, Te, kVe, EXTRA>(rp, lanes_per_row(rp.ncols, kVe), stream);
+    return dispatch_lpr<Tg, Te, 1, EXTRA>(rp, lanes_per_row(rp.ncols, 1), stream);
+}
+
+}  // namespace
+
+hipError_t launch_row_pass(const RowPass &rp, hipStream_t stream) {
+    if (!rp.g_bf16 && (rp.extra == kExtraNone || !rp.e_bf16)) {
+        switch (rp.extra) {
+            case kExtraNone: return dispatch_ve<float, float, kExtraNone>(rp, stream);
+            case kExtraParent: return dispatch_ve<float, float, kExtraParent>(rp, stream);
+            case kExtraSelf: return dispatch_ve<float, float, kExtraSelf>(rp, stream);
+        }
+    } else if (rp.g_bf16 && (rp.extra == kExtraNone || !rp.e_bf16)) {
+        switch (rp.extra) {
+            case kExtraNone: return dispatch_ve<bf16_t, float, kExtraNone>(rp, stream);
+            case kExtraParent: return dispatch_ve<bf16_t, float, kExtraParent>(rp, stream);
+            default: break;
+        }
+    } else if (!rp.g_bf16 && rp.e_bf16 && rp.extra == kExtraSelf) {
+        return dispatch_ve<float, bf16_t, kExtraSelf>(rp, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_propagate(int32_t p_begin, int32_t p_end, const int32_t *node_of, const int32_t *parent_of,
+                            const float *parscale, float *y, int64_t ldy, int32_t ncols, hipStream_t stream) {
+    const int64_t n = int64_t(p_end) - p_begin;
+    if (n <= 0) return hipSuccess;
+    const bool vec = (ncols % 4) == 0 && aligned16(y, ldy, 4);
+    const int ve = vec ? 4 : 1;
+    const int lpr = lanes_per_row(ncols, ve);
+    const int64_t rows_per_block = 4 * (64 / lpr);
+    const dim3 grid(unsigned((n + rows_per_block - 1) / rows_per_block)), block(256);
+#define CBM_PROP_CASE(V, L)                                                                                  \
+    if (ve == V && lpr == L) {                                                                               \
+        k_propagate<V, L><<<grid, block, 0, stream>>>(p_begin, p_end, node_of, parent_of, parscale, y, ldy, \
+                                                      ncols);                                                \
+        return hipGetLastError();                                                                            \
+    }
+    CBM_PROP_CASE(4, 1) CBM_PROP_CASE(4, 2) CBM_PROP_CASE(4, 4) CBM_PROP_CASE(4, 8) CBM_PROP_CASE(4, 16)
+    CBM_PROP_CASE(4, 32) CBM_PROP_CASE(4, 64)
+    CBM_PROP_CASE(1, 1) CBM_PROP_CASE(1, 2) CBM_PROP_CASE(1, 4) CBM_PROP_CASE(1, 8) CBM_PROP_CASE(1, 16)
+    CBM_PROP_CASE(1, 32) CBM_PROP_CASE(1, 64)
+#undef CBM_PROP_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_dad_scales(int32_t n, const int32_t *node_of, const int32_t *parent_of, const float *d, float *rs,
+                             float *ps, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    k_dad_scales<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream>>>(n, node_of, parent_of, d, rs, ps);
+    return hipGetLastError();
+}
+
+}  // namespace cbmk

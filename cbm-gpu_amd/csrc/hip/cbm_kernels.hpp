@@ -1,0 +1,55 @@
+// Kernel launchers of the CBM product (gfx950).  See cbm_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cbmk {
+
+// Where the optional per-row extra term of a row pass comes from.
+enum Extra : int32_t {
+    kExtraNone = 0,    // y_row = rowscale * sum
+    kExtraParent = 1,  // y_row = rowscale * sum + escale[p] * e[parent_of[p]]   (forward level pass)
+    kExtraSelf = 2     // y_row = sum + escale[row] * e[row]                      (adjoint reverse pass)
+};
+
+// One row pass: for every position p in [p_begin, p_end), with row = node_of[p]
+// (or p when node_of is null),
+//   sum = sum_k  sign_k * colscale[j_k] * g[j_k, :]   over k in [ptr[p], ptr[p+1]),
+// where idx[k] = j_k | sign_k << 31, then the extra term per `extra`, and the
+// result is stored to y[row, :] in FP32.
+struct RowPass {
+    int32_t p_begin = 0, p_end = 0;
+    const int32_t *node_of = nullptr;
+    const int32_t *parent_of = nullptr;
+    const int32_t *ptr = nullptr;
+    const int32_t *idx = nullptr;
+    const float *colscale = nullptr;  // indexed by gathered row; null = 1
+    const float *rowscale = nullptr;  // indexed by position;     null = 1
+    const float *escale = nullptr;    // by position (parent) or by row (self); null = 1
+    const void *g = nullptr;          // gathered matrix
+    int64_t ldg = 0;
+    bool g_bf16 = false;
+    const void *e = nullptr;          // extra-term matrix
+    int64_t lde = 0;
+    bool e_bf16 = false;
+    Extra extra = kExtraNone;
+    float *y = nullptr;
+    int64_t ldy = 0;
+    int32_t ncols = 0;
+};
+
+// Launches one row pass; returns the launch status.
+hipError_t launch_row_pass(const RowPass &rp, hipStream_t stream);
+
+// In-place propagation of one level: y[node_of[p]] += parscale[p] * y[parent_of[p]].
+hipError_t launch_propagate(int32_t p_begin, int32_t p_end, const int32_t *node_of, const int32_t *parent_of,
+                            const float *parscale, float *y, int64_t ldy, int32_t ncols, hipStream_t stream);
+
+// Per-position scales of the D A D product: rs[p] = d[node[p]],
+// ps[p] = d[node[p]] / d[parent[p]] (0 for roots and zero-degree parents).
+hipError_t launch_dad_scales(int32_t n, const int32_t *node_of, const int32_t *parent_of, const float *d, float *rs,
+                             float *ps, hipStream_t stream);
+
+}  // namespace cbmk

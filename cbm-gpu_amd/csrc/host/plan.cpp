@@ -1,0 +1,196 @@
+// Device plan: lays a CBM structure out the way the HIP kernels read it.
+//
+//   level order  nodes sorted by (depth, id); depth 0 = children of the
+//                virtual root.  A node's parent always sits in an earlier
+//                level, so one pass per level (or a dependency wait on the
+//                parent) gives a race-free propagation.
+//   forward      Delta rows permuted into level order; each column index
+//                carries the entry's sign in bit 31 (no value array).
+//   backward     Delta^T (the CSC of Delta) in natural row order, signs packed
+//                the same way, for the adjoint's Delta^T product.
+//   children     per level position, the node ids of the children, for the
+//                adjoint's reverse (leaves-to-roots) gather.
+#include "common.hpp"
+#include "../../../include/cbm_host.h"
+
+#include <algorithm>
+#include <memory>
+#include <new>
+#include <numeric>
+
+#include <omp.h>
+
+struct cbm_plan {
+    std::vector<int32_t> arr[CBM_P_N_ARRAYS];
+    int32_t depth = 0;
+};
+
+using namespace cbmh;
+
+static constexpr uint32_t kSignBit = 0x80000000u;
+
+extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *col, const float *val, int64_t nnz,
+                               const int32_t *tree_src, const int32_t *tree_dst, int64_t n_edges, int32_t n_threads,
+                               cbm_plan **out) {
+    if (!out || n < 1 || !crow || nnz < 0 || (nnz > 0 && (!col || !val)) || n_edges < 0 ||
+        (n_edges > 0 && (!tree_src || !tree_dst)) || nnz > INT32_MAX) {
+        set_error("plan: invalid arguments (n=%d nnz=%lld edges=%lld)", n, (long long)nnz, (long long)n_edges);
+        return CBM_ERR_ARG;
+    }
+    *out = nullptr;
+    if (crow[0] != 0 || int64_t(crow[n]) != nnz) {
+        set_error("plan: row offsets run from %d to %d, expected 0 to %lld", crow[0], crow[n], (long long)nnz);
+        return CBM_ERR_ARG;
+    }
+    for (int32_t r = 0; r < n; ++r)
+        if (crow[r + 1] < crow[r]) {
+            set_error("plan: row offsets decrease at row %d", r);
+            return CBM_ERR_ARG;
+        }
+    const int saved_threads = omp_get_max_threads();
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+    struct Restore {
+        int t;
+        ~Restore() { omp_set_num_threads(t); }
+    } restore{saved_threads};
+    try {
+        auto plan = std::make_unique<cbm_plan>();
+        // Parent array from the tree edges; a node may have at most one parent.
+        std::vector<int32_t> parent(size_t(n), -1);
+        for (int64_t e = 0; e < n_edges; ++e) {
+            const int32_t u = tree_src[e], v = tree_dst[e];
+            if (u < 0 || u >= n || v < 0 || v >= n) {
+                set_error("plan: tree edge %lld = (%d -> %d) is outside 0..%d", (long long)e, u, v, n - 1);
+                return CBM_ERR_ARG;
+            }
+            if (parent[size_t(v)] != -1 || u == v) {
+                set_error("plan: node %d has more than one parent or is its own parent", v);
+                return CBM_ERR_VALUE;
+            }
+            parent[size_t(v)] = u;
+        }
+        // Children by parent, ascending id.
+        std::vector<int32_t> kptr(size_t(n) + 1, 0);
+        std::vector<int32_t> kid(static_cast<size_t>(n_edges));
+        for (int32_t v = 0; v < n; ++v)
+            if (parent[size_t(v)] >= 0) ++kptr[size_t(parent[size_t(v)]) + 1];
+        std::partial_sum(kptr.begin(), kptr.end(), kptr.begin());
+        {
+            std::vector<int32_t> fill(kptr.begin(), kptr.end() - 1);
+            for (int32_t v = 0; v < n; ++v)
+                if (parent[size_t(v)] >= 0) kid[size_t(fill[size_t(parent[size_t(v)])]++)] = v;
+        }
+        // Depth by BFS from the roots; nodes never reached sit on a cycle.
+        std::vector<int32_t> depth(size_t(n), -1), queue;
+        queue.reserve(size_t(n));
+        for (int32_t v = 0; v < n; ++v)
+            if (parent[size_t(v)] < 0) {
+                depth[size_t(v)] = 0;
+                queue.push_back(v);
+            }
+        for (size_t h = 0; h < queue.size(); ++h) {
+            const int32_t u = queue[h];
+            for (int32_t k = kptr[size_t(u)]; k < kptr[size_t(u) + 1]; ++k) {
+                depth[size_t(kid[size_t(k)])] = depth[size_t(u)] + 1;
+                queue.push_back(kid[size_t(k)]);
+            }
+        }
+        if (int64_t(queue.size()) != n) {
+            set_error("plan: the tree edges contain a cycle (%lld of %d nodes reachable from a root)",
+                      (long long)queue.size(), n);
+            return CBM_ERR_VALUE;
+        }
+        int32_t levels = 0;
+        for (int32_t v = 0; v < n; ++v) levels = std::max(levels, depth[size_t(v)] + 1);
+        plan->depth = levels;
+
+        // Level order by counting sort on depth (stable in node id).
+        auto &lvl_ptr = plan->arr[CBM_P_LVL_PTR];
+        auto &lvl_node = plan->arr[CBM_P_LVL_NODE];
+        auto &lvl_parent = plan->arr[CBM_P_LVL_PARENT];
+        lvl_ptr.assign(size_t(levels) + 1, 0);
+        for (int32_t v = 0; v < n; ++v) ++lvl_ptr[size_t(depth[size_t(v)]) + 1];
+        std::partial_sum(lvl_ptr.begin(), lvl_ptr.end(), lvl_ptr.begin());
+        lvl_node.resize(size_t(n));
+        lvl_parent.resize(size_t(n));
+        {
+            std::vector<int32_t> fill(lvl_ptr.begin(), lvl_ptr.end() - 1);
+            for (int32_t v = 0; v < n; ++v) lvl_node[size_t(fill[size_t(depth[size_t(v)])]++)] = v;
+        }
+        for (int32_t p = 0; p < n; ++p) lvl_parent[size_t(p)] = parent[size_t(lvl_node[size_t(p)])];
+
+        // Forward Delta in level order with packed signs.
+        auto &fptr = plan->arr[CBM_P_FWD_PTR];
+        auto &fcol = plan->arr[CBM_P_FWD_COL];
+        fptr.assign(size_t(n) + 1, 0);
+        for (int32_t p = 0; p < n; ++p) {
+            const int32_t v = lvl_node[size_t(p)];
+            fptr[size_t(p) + 1] = fptr[size_t(p)] + (crow[v + 1] - crow[v]);
+        }
+        fcol.resize(size_t(nnz));
+        int bad = -1;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(max : bad)
+        for (int32_t p = 0; p < n; ++p) {
+            const int32_t v = lvl_node[size_t(p)];
+            int32_t o = fptr[size_t(p)];
+            for (int32_t k = crow[v]; k < crow[v + 1]; ++k, ++o) {
+                const int32_t c = col[k];
+                const float x = val[k];
+                if (c < 0 || c >= n || !(x != 0.0f) || x != x) bad = std::max(bad, k);
+                fcol[size_t(o)] = int32_t(uint32_t(c) | (x < 0.0f ? kSignBit : 0u));
+            }
+        }
+        if (bad >= 0) {
+            set_error("plan: Delta entry %d has an invalid column (%d) or a zero/NaN value", bad, col[bad]);
+            return CBM_ERR_VALUE;
+        }
+
+        // Backward: CSC of Delta (rows ascending within each column).
+        auto &bptr = plan->arr[CBM_P_BWD_PTR];
+        auto &brow = plan->arr[CBM_P_BWD_ROW];
+        bptr.assign(size_t(n) + 1, 0);
+        for (int64_t k = 0; k < nnz; ++k) ++bptr[size_t(col[k]) + 1];
+        std::partial_sum(bptr.begin(), bptr.end(), bptr.begin());
+        brow.resize(size_t(nnz));
+        {
+            std::vector<int32_t> fill(bptr.begin(), bptr.end() - 1);
+            for (int32_t r = 0; r < n; ++r)
+                for (int32_t k = crow[r]; k < crow[r + 1]; ++k)
+                    brow[size_t(fill[size_t(col[k])]++)] = int32_t(uint32_t(r) | (val[k] < 0.0f ? kSignBit : 0u));
+        }
+
+        // Children per level position.
+        auto &cptr = plan->arr[CBM_P_CH_PTR];
+        auto &cnode = plan->arr[CBM_P_CH_NODE];
+        cptr.assign(size_t(n) + 1, 0);
+        for (int32_t p = 0; p < n; ++p) {
+            const int32_t u = lvl_node[size_t(p)];
+            cptr[size_t(p) + 1] = cptr[size_t(p)] + (kptr[size_t(u) + 1] - kptr[size_t(u)]);
+        }
+        cnode.resize(size_t(n_edges));
+#pragma omp parallel for schedule(static)
+        for (int32_t p = 0; p < n; ++p) {
+            const int32_t u = lvl_node[size_t(p)];
+            std::copy(kid.begin() + kptr[size_t(u)], kid.begin() + kptr[size_t(u) + 1], cnode.begin() + cptr[size_t(p)]);
+        }
+        *out = plan.release();
+        return CBM_OK;
+    } catch (const std::bad_alloc &) {
+        set_error("plan: out of host memory");
+        return CBM_ERR_NOMEM;
+    }
+}
+
+extern "C" int64_t cbm_plan_size(const cbm_plan *p, int32_t which) {
+    if (!p || which < 0 || which >= CBM_P_N_ARRAYS) return -1;
+    return int64_t(p->arr[which].size());
+}
+
+extern "C" const int32_t *cbm_plan_array(const cbm_plan *p, int32_t which) {
+    if (!p || which < 0 || which >= CBM_P_N_ARRAYS) return nullptr;
+    return p->arr[which].data();
+}
+
+extern "C" int32_t cbm_plan_depth(const cbm_plan *p) { return p ? p->depth : -1; }
+
+extern "C" void cbm_plan_free(cbm_plan *p) { delete p; }

@@ -1,0 +1,153 @@
+/*
+ * cbm_hip.h -- C ABI of the CBM product on AMD Instinct MI355X (gfx950),
+ * libcbm_hip.so.
+ *
+ * A handle binds one CBM matrix, laid out by cbm_plan_create (cbm_host.h)
+ * and copied to device memory by the caller, to the HIP kernels:
+ *
+ *     forward   Y  = M X         M = A, A D, or D A D   (D = D^-1/2, diagonal)
+ *     adjoint   dX = M^T G
+ *
+ * with A = (I - P)^-1 Delta.  X and G are row-major N x F in FP32 or BF16;
+ * Y, dX and all accumulation are FP32.  Every call is asynchronous on the
+ * given stream, allocates nothing and may be captured in a HIP graph.
+ * Calls return CBM_OK (0) or a negative status; cbm_last_error() describes
+ * the failure for the calling thread.
+ *
+ * Replaces class cusparseCBM (cbm/cbm.cu:93-300): its constructor
+ * (cbm/cbm.cu:95-142) becomes cbm_create, matmul_AX_ADX (:150-207) and
+ * matmul_DAX_DADX (:210-268) become cbm_matmul, and the adjoint is new.
+ */
+#ifndef CBM_HIP_H
+#define CBM_HIP_H
+
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (shared with cbm_host.h). */
+#ifndef CBM_OK
+#define CBM_OK 0
+#define CBM_ERR_ARG -1
+#define CBM_ERR_VALUE -2
+#define CBM_ERR_NOMEM -3
+#define CBM_ERR_INTERNAL -4
+#endif
+#define CBM_ERR_DEVICE -5 /* a HIP or rocSPARSE call failed */
+
+/* Which product a handle computes. */
+enum cbm_mode {
+    CBM_MODE_AX = 0,   /* M = A            (cbm4mm,  cbm/cbm4mm.py)   */
+    CBM_MODE_ADX = 1,  /* M = A D^-1/2     (cbm4ad,  cbm/cbm4ad.py)   */
+    CBM_MODE_DADX = 2  /* M = D^-1/2 A D^-1/2 (cbm4dad, cbm/cbm4dad.py) */
+};
+
+/* Storage type of X and G. */
+enum cbm_dtype { CBM_F32 = 0, CBM_BF16 = 1 };
+
+/*
+ * Device layout of one CBM matrix.  All device pointers are borrowed and must
+ * stay valid until cbm_destroy.  Arrays are those of cbm_plan_create:
+ *   lvl_node[N], lvl_parent[N], fwd_ptr[N+1], fwd_col[nnz], bwd_ptr[N+1],
+ *   bwd_row[nnz], ch_ptr[N+1], ch_node[E]; lvl_ptr is the HOST copy [L+1].
+ * d is the device vector D^-1/2 [N] (0 for nodes of degree 0) for ADX/DADX,
+ * NULL for AX.
+ */
+typedef struct cbm_desc {
+    int32_t n_rows;
+    int32_t n_levels;
+    int64_t nnz;
+    int64_t n_edges;
+    int32_t mode;
+    const int32_t *lvl_ptr_host;
+    const int32_t *lvl_node;
+    const int32_t *lvl_parent;
+    const int32_t *fwd_ptr;
+    const int32_t *fwd_col;
+    const int32_t *bwd_ptr;
+    const int32_t *bwd_row;
+    const int32_t *ch_ptr;
+    const int32_t *ch_node;
+    const float *d;
+} cbm_desc;
+
+typedef struct cbm_handle cbm_handle;
+
+/* Message describing the last failure on the calling thread ("" if none). */
+const char *cbm_last_error(void);
+
+/*
+ * Creates a handle.  Derived per-position scale vectors (N floats each) are
+ * allocated here, once, and computed on `stream`; the call synchronises the
+ * stream before returning.
+ */
+int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle **out);
+int cbm_destroy(cbm_handle *h);
+
+/*
+ * Forward product Y = M X.  x: N x n_cols, leading dimension ldx (elements);
+ * y: N x n_cols FP32, leading dimension ldy; x and y must not overlap.
+ * Fused: each level launch computes Delta rows and adds the parent's
+ * finished row.  Replaces cusparseCBM::matmul_AX_ADX / matmul_DAX_DADX.
+ */
+int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy, int32_t n_cols,
+               hipStream_t stream);
+
+/*
+ * Adjoint dX = M^T G.  z is caller-provided FP32 workspace N x n_cols with
+ * leading dimension ldz (it receives (I-P)^-T applied to the scaled G).
+ * No reference counterpart on the GPU; the CPU op s_spmm_csr_t_int32
+ * (cbm/cbm_extensions.cpp:742-782) is the closest.
+ */
+int cbm_matmul_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, float *dx, int64_t lddx, float *z,
+                 int64_t ldz, int32_t n_cols, hipStream_t stream);
+
+/*
+ * The two halves of the forward product, for separate timing and for the
+ * operator surface:  cbm_delta_spmm writes Y = Delta' X (Delta' = Delta,
+ * Delta D or D Delta D by mode) without propagation; cbm_propagate then adds
+ * parents in level order in place, Y_v += s_v Y_parent(v) with s_v = 1 or
+ * D_v / D_parent (cbm/cbm.cu:25-91 `update` / `fused_update`).
+ */
+int cbm_delta_spmm(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy, int32_t n_cols,
+                   hipStream_t stream);
+int cbm_propagate(cbm_handle *h, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream);
+
+/* Per-kernel launch counts of the last call on this handle (for profiling). */
+int cbm_last_launches(const cbm_handle *h);
+
+/* ------------------------------------------------------------------------- */
+/* CSR comparator on the same GPU: rocSPARSE SpMM, FP32.                      */
+/* Replaces class cusparseCSR (cbm/csr.cu:24-126).                            */
+/* ------------------------------------------------------------------------- */
+
+typedef struct csr_handle csr_handle;
+
+/* Borrowed device CSR (int32 indices, FP32 values) of an n_rows x n_cols matrix. */
+int csr_create(int32_t n_rows, int32_t n_cols, int64_t nnz, const int32_t *crow, const int32_t *col,
+               const float *val, csr_handle **out);
+int csr_destroy(csr_handle *h);
+
+/*
+ * Y = A X (trans = 0) or Y = A^T X (trans = 1) with rocsparse_spmm.  The
+ * rocSPARSE work buffer is allocated on the first call for a given shape and
+ * kept by the handle; later calls of the same or smaller size allocate nothing.
+ */
+int csr_matmul(csr_handle *h, int32_t trans, const float *x, int64_t ldx, float *y, int64_t ldy, int32_t n_cols,
+               hipStream_t stream);
+
+/* Selects the rocsparse_spmm_alg used by csr_matmul (default: rocsparse_spmm_alg_default). */
+int csr_set_algorithm(csr_handle *h, int32_t alg);
+
+/* Message describing the last csr_* failure on the calling thread. */
+const char *csr_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CBM_HIP_H */

@@ -1,0 +1,154 @@
+/*
+ * cbm_host.h -- C ABI of the host-side CBM structure builder (libcbm_host.so).
+ *
+ * The CBM (compressed binary matrix) form of a binary adjacency matrix A is
+ *     A = (I - P)^-1 * Delta
+ * where P[v, parent(v)] = 1 for every tree edge of a minimum spanning
+ * arborescence over row Hamming distances, and Delta holds +-1 entries.
+ *
+ * Everything here runs on the CPU.  Nothing in this header touches a GPU.
+ * All functions return CBM_OK (0) on success and a negative status on
+ * failure; cbm_host_last_error() then returns a message for this thread.
+ */
+#ifndef CBM_HOST_H
+#define CBM_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CBM_OK = 0,
+    CBM_ERR_ARG = -1,      /* invalid argument (null pointer, negative size, index out of range) */
+    CBM_ERR_VALUE = -2,    /* input violates a precondition (non-binary value, ...)             */
+    CBM_ERR_NOMEM = -3,    /* host allocation failed                                             */
+    CBM_ERR_INTERNAL = -4  /* internal invariant broken                                          */
+};
+
+/* Message describing the last failure on the calling thread ("" if none). */
+const char *cbm_host_last_error(void);
+
+/* ------------------------------------------------------------------------- */
+/* Synthetic graphs: the seeded community copying model.                     */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * Builds an undirected binary graph without self-loops and returns it as a
+ * coalesced COO list sorted by (row, col), both directions present.
+ *   n_nodes   number of nodes; nodes are numbered contiguously by block
+ *   n_blocks  number of equal blocks (the last one takes the remainder)
+ *   p_copy    probability of copying each out-link of the prototype
+ *   lam       mean of the Poisson number of uniform random out-links per node
+ *   seed_deg  out-links drawn inside its block for the first node of a block
+ *   seed      64-bit seed; equal arguments give identical graphs everywhere
+ * The result is owned by the library: read it with cbm_graph_{size,row,col}
+ * and release it with cbm_graph_free.
+ * No reference counterpart: it replaces the network dataset loaders of
+ * benchmark/utilities.py:62-198.
+ */
+typedef struct cbm_graph cbm_graph;
+int cbm_graph_copying_model(int64_t n_nodes, int32_t n_blocks, double p_copy, double lam,
+                            int32_t seed_deg, uint64_t seed, cbm_graph **out);
+int64_t cbm_graph_nnz(const cbm_graph *g);
+const int32_t *cbm_graph_row(const cbm_graph *g);
+const int32_t *cbm_graph_col(const cbm_graph *g);
+void cbm_graph_free(cbm_graph *g);
+
+/* ------------------------------------------------------------------------- */
+/* Structure build: replaces cbm_init_ (cbm/cbm_extensions.cpp:60-345).      */
+/* ------------------------------------------------------------------------- */
+
+/* Arrays held by a cbm_build result, addressed by cbm_build_array(). */
+enum {
+    CBM_B_DELTA_ROW = 0,   /* int32 [nnz_delta]  COO rows of Delta, construction order  (ref :301-305)    */
+    CBM_B_DELTA_COL = 1,   /* int32 [nnz_delta]  COO cols of Delta                      (ref :307-311)    */
+    CBM_B_DELTA_VAL = 2,   /* float [nnz_delta]  +1 / -1                                (ref :313-317)    */
+    CBM_B_BRANCHES = 3,    /* int32 [nb + 1]     prefix offsets of the non-trivial branches (ref :319-323) */
+    CBM_B_TREE_SRC = 4,    /* int32 [E]          parent of each tree edge, BFS order per branch (ref :325) */
+    CBM_B_TREE_DST = 5,    /* int32 [E]          child of each tree edge                 (ref :331-335)   */
+    CBM_B_PARENT = 6,      /* int32 [N]          parent(v), or -1 when v hangs off the virtual root       */
+    CBM_B_CSR_PTR = 7,     /* int32 [N+1]        Delta as CSR: row offsets (rows ascending)               */
+    CBM_B_CSR_COL = 8,     /* int32 [nnz_delta]  Delta as CSR: columns, ascending within a row            */
+    CBM_B_CSR_VAL = 9,     /* float [nnz_delta]  Delta as CSR: +1 / -1                                    */
+    CBM_B_N_ARRAYS = 10
+};
+
+/* Scalar statistics of a cbm_build result, addressed by cbm_build_stat(). */
+enum {
+    CBM_S_N = 0,            /* rows                                                         */
+    CBM_S_NNZ_A = 1,        /* nnz(A) after coalescing                                      */
+    CBM_S_NNZ_DELTA = 2,    /* nnz(Delta)                                                   */
+    CBM_S_CANDIDATES = 3,   /* candidate edges handed to the arborescence (root edges incl.) */
+    CBM_S_MSA_WEIGHT = 4,   /* total weight of the minimum arborescence (== nnz(Delta))     */
+    CBM_S_TREE_EDGES = 5,   /* E, edges whose parent is a real node                          */
+    CBM_S_BRANCHES = 6,     /* nb, non-trivial branches                                     */
+    CBM_S_DEPTH = 7,        /* levels of the forest (a forest of singletons has depth 1)    */
+    CBM_S_N_STATS = 8
+};
+
+typedef struct cbm_build cbm_build;
+
+/*
+ * Builds the CBM structure of the binary matrix given as COO (row, col, val).
+ * Mirrors cbm_mkl_cpp.init(row, col, val, n_rows, alpha):
+ *   - candidate edges are created in the reference order (cbm_extensions.cpp:158-182):
+ *     for each row r ascending, the root edge N->r of weight deg(r), then for each
+ *     c > r sharing a column with r, ascending: c->r if h < deg(r)-alpha and h != deg(r),
+ *     and r->c if h < deg(c)-alpha and h != deg(c), where h = |A_r xor A_c|;
+ *   - the minimum spanning arborescence rooted at the virtual node N is found by a
+ *     Tarjan-style contraction; ties between equal reduced weights go to the edge
+ *     created first;
+ *   - Delta rows and the branch/BFS arrays follow cbm_extensions.cpp:191-299, with
+ *     children visited in ascending node id.
+ * val must be all ones (binary matrix); duplicate (row, col) pairs are merged.
+ * n_threads <= 0 uses the OpenMP default.
+ */
+int cbm_build_create(const int32_t *row, const int32_t *col, const float *val, int64_t nnz,
+                     int32_t n_rows, int32_t alpha, int32_t n_threads, cbm_build **out);
+int64_t cbm_build_size(const cbm_build *b, int32_t which);
+const void *cbm_build_array(const cbm_build *b, int32_t which);
+int64_t cbm_build_stat(const cbm_build *b, int32_t which);
+void cbm_build_free(cbm_build *b);
+
+/* ------------------------------------------------------------------------- */
+/* Device plan: the layout the HIP kernels read, built on the host.           */
+/* ------------------------------------------------------------------------- */
+
+/* Arrays held by a cbm_plan, addressed by cbm_plan_array(). */
+enum {
+    CBM_P_LVL_PTR = 0,     /* int32 [L+1]  level offsets into the level order                     */
+    CBM_P_LVL_NODE = 1,    /* int32 [N]    node at each level position (level 0 = root children)   */
+    CBM_P_LVL_PARENT = 2,  /* int32 [N]    parent of the node at each position, -1 for level 0     */
+    CBM_P_FWD_PTR = 3,     /* int32 [N+1]  Delta row offsets, rows in level order                  */
+    CBM_P_FWD_COL = 4,     /* int32 [nnz]  Delta columns, bit 31 set for a -1 entry                 */
+    CBM_P_BWD_PTR = 5,     /* int32 [N+1]  Delta^T row offsets (= Delta CSC), natural order         */
+    CBM_P_BWD_ROW = 6,     /* int32 [nnz]  Delta rows per column, bit 31 set for a -1 entry         */
+    CBM_P_CH_PTR = 7,      /* int32 [N+1]  child offsets per level position                          */
+    CBM_P_CH_NODE = 8,     /* int32 [E]    children (node ids) of the node at each position          */
+    CBM_P_N_ARRAYS = 9
+};
+
+typedef struct cbm_plan cbm_plan;
+
+/*
+ * Lays out a CBM structure for the GPU.  Inputs are the Delta CSR (any value
+ * array whose entries are +-1 times a positive scale; only the signs are kept)
+ * and the tree edges (src = parent, dst = child) in any order.
+ * Fails with CBM_ERR_VALUE if the tree edges do not form a forest over N nodes
+ * (a node with two parents, or a cycle).
+ */
+int cbm_plan_create(int32_t n_rows, const int32_t *crow, const int32_t *col, const float *val,
+                    int64_t nnz, const int32_t *tree_src, const int32_t *tree_dst, int64_t n_edges,
+                    int32_t n_threads, cbm_plan **out);
+int64_t cbm_plan_size(const cbm_plan *p, int32_t which);
+const int32_t *cbm_plan_array(const cbm_plan *p, int32_t which);
+int32_t cbm_plan_depth(const cbm_plan *p);
+void cbm_plan_free(cbm_plan *p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CBM_HOST_H */
