@@ -1,0 +1,196 @@
+"""The product's host builder (libcbm_host.so) against the oracle, and the
+device plan's invariants (CPU only)."""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from cbm import cbm_mkl_cpp, structure
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "cbm_small.json")
+
+# Workload generator parameters (bench.py) and the nnz each is meant to give.
+WORKLOAD_GRAPHS = [
+    (2708, 7, 0.8, 0.58, 0, 1001, 10556),
+    (19717, 3, 0.85, 0.44, 0, 1002, 88648),
+    (169343, 40, 0.8, 1.63, 0, 1003, 2315598),
+]
+
+
+def _build(ei, n, alpha):
+    ei = torch.as_tensor(np.asarray(ei), dtype=torch.int32)
+    return structure.build(ei[0], ei[1], torch.ones(ei.shape[1]), n, alpha)
+
+
+def _same_structure(b, ref):
+    for k in ("delta_row", "delta_col", "delta_val", "branches", "tree_src", "tree_dst"):
+        assert np.array_equal(getattr(b, k).numpy(), ref[k]), k
+
+
+def test_golden_fixtures_exactly():
+    """Unique-optimum fixtures pin the parent array and every init array."""
+    with open(GOLDEN) as fh:
+        cases = json.load(fh)["cases"]
+    for case in cases:
+        n = case["n"]
+        b = _build(case["edge_index"], n, case["alpha"])
+        assert b.stats["candidates"] == len(case["candidates"])
+        assert b.stats["msa_weight"] == case["weight"] == b.stats["nnz_delta"]
+        assert b.parent.tolist() == case["parent"]
+        a = O.csr_of(np.array(case["edge_index"]), n)
+        _same_structure(b, O.delta_for_parent(a, b.parent.numpy()))
+
+
+@pytest.mark.parametrize("alpha", [0, 1, 3])
+def test_random_graphs_match_oracle(alpha):
+    rng = np.random.default_rng(alpha)
+    for trial in range(30):
+        n = int(rng.integers(2, 120))
+        dens = float(rng.uniform(0.02, 0.3))
+        A = (rng.random((n, n)) < dens).astype(np.int64)
+        if trial % 2:
+            for i in range(1, n):  # copied rows make deeper trees
+                j = int(rng.integers(0, i))
+                A[i] = np.where(rng.random(n) < 0.9, A[j], A[i])
+        A = np.triu(A, 1)
+        A = A + A.T
+        r, c = np.nonzero(A)
+        if len(r) == 0:
+            continue
+        n_eff = int(max(r.max(), c.max())) + 1
+        ei = np.stack([r, c])
+        b = _build(ei, n_eff, alpha)
+        ref = O.cbm_init(ei, n_eff, alpha)
+        assert b.stats["candidates"] == len(ref["candidates"][0])
+        assert b.stats["msa_weight"] == ref["weight"] == b.stats["nnz_delta"]
+        # Delta and the branch arrays are a function of the parent array
+        a = O.csr_of(ei, n_eff)
+        _same_structure(b, O.delta_for_parent(a, b.parent.numpy()))
+        # A = (I - P)^-1 Delta
+        delta = sp.csr_matrix((b.csr_val.numpy().astype(float), b.csr_col.numpy(), b.csr_ptr.numpy()),
+                              shape=(n_eff, n_eff))
+        P = O.parent_matrix(b.parent.numpy(), n_eff)
+        assert np.array_equal(((sp.identity(n_eff) - P) @ a).toarray(), delta.toarray())
+
+
+def test_workload_graph_matches_oracle():
+    """The smallest workload graph: candidates, weight, parents and Delta."""
+    n, nb, p, lam, sd, seed, _ = WORKLOAD_GRAPHS[0]
+    ei = structure.copying_model(n, nb, p, lam, sd, seed)
+    for alpha in (0, 4):
+        b = _build(ei.numpy(), n, alpha)
+        ref = O.cbm_init(ei.numpy(), n, alpha)
+        s, d, w = ref["candidates"]
+        assert b.stats["candidates"] == len(s)
+        assert b.stats["msa_weight"] == ref["weight"]
+        # Both implementations break ties toward the earliest candidate edge;
+        # report, but do not require, parent agreement on tied instances.
+        agree = float((b.parent.numpy() == ref["parent"]).mean())
+        assert agree > 0.95
+        _same_structure(b, O.delta_for_parent(O.csr_of(ei.numpy(), n), b.parent.numpy()))
+
+
+def test_init_matches_reference_contract():
+    """init(): six CPU tensors with the reference dtypes (cbm_extensions.cpp:300-344)
+    and CHECK_DTYPE errors (:65-67)."""
+    ei = structure.copying_model(300, 2, 0.8, 0.6, 0, 3)
+    out = cbm_mkl_cpp.init(ei[0], ei[1], torch.ones(ei.shape[1]), 300, 0)
+    assert len(out) == 6
+    assert [t.dtype for t in out] == [torch.int32, torch.int32, torch.float32, torch.int32, torch.int32, torch.int32]
+    assert all(t.device.type == "cpu" for t in out)
+    assert len(out[0]) == len(out[1]) == len(out[2])
+    assert len(out[4]) == len(out[5]) == int(out[3][-1])
+    with pytest.raises(RuntimeError, match="int32"):
+        cbm_mkl_cpp.init(ei[0].long(), ei[1], torch.ones(ei.shape[1]), 300, 0)
+    with pytest.raises(RuntimeError, match="float32"):
+        cbm_mkl_cpp.init(ei[0], ei[1], torch.ones(ei.shape[1], dtype=torch.float64), 300, 0)
+
+
+def test_init_rejects_bad_input():
+    ei = torch.tensor([[0, 1], [1, 0]], dtype=torch.int32)
+    with pytest.raises(ValueError, match="binary"):
+        structure.build(ei[0], ei[1], torch.tensor([1.0, 2.0]), 2, 0)
+    with pytest.raises(ValueError, match="outside"):
+        structure.build(ei[0], ei[1] + 5, torch.ones(2), 2, 0)
+
+
+def test_duplicates_are_merged():
+    ei = torch.tensor([[0, 0, 1, 1, 2], [1, 1, 0, 2, 1]], dtype=torch.int32)
+    b = structure.build(ei[0], ei[1], torch.ones(5), 3, 0)
+    assert b.stats["nnz_a"] == 4
+
+
+@pytest.mark.parametrize("n,nb,p,lam,sd,seed,target", WORKLOAD_GRAPHS)
+def test_generator_sizes_and_format(n, nb, p, lam, sd, seed, target):
+    ei = structure.copying_model(n, nb, p, lam, sd, seed)
+    assert abs(ei.shape[1] - target) <= 0.1 * target
+    r, c = ei[0].long(), ei[1].long()
+    key = r * n + c
+    assert bool((key[1:] > key[:-1]).all())  # sorted, no duplicates
+    assert not bool((r == c).any())  # no self-loops
+    rev = torch.sort(c * n + r).values
+    assert torch.equal(rev, key)  # symmetric
+    assert int(max(r.max(), c.max())) == n - 1
+    if n < 50000:
+        assert torch.equal(structure.copying_model(n, nb, p, lam, sd, seed), ei)  # deterministic
+
+
+def _plan_for(ei, n, alpha):
+    b = _build(ei, n, alpha)
+    return b, structure.plan(n, b.csr_ptr, b.csr_col, b.csr_val, b.tree_src, b.tree_dst)
+
+
+def test_plan_invariants():
+    ei = structure.copying_model(3000, 5, 0.85, 0.5, 0, 77).numpy()
+    b, pl = _plan_for(ei, 3000, 0)
+    n = 3000
+    node = pl.lvl_node.numpy()
+    par = pl.lvl_parent.numpy()
+    lp = pl.lvl_ptr.numpy()
+    assert sorted(node.tolist()) == list(range(n))
+    level = np.empty(n, np.int64)
+    for l in range(pl.depth):
+        level[node[lp[l]:lp[l + 1]]] = l
+    # level order is topological: a parent sits exactly one level up
+    has = par >= 0
+    assert np.all(level[par[has]] == level[node[has]] - 1)
+    assert np.all(par[lp[0]:lp[1]] == -1)
+    assert np.array_equal(par, b.parent.numpy()[node])
+    # forward rows = Delta rows in level order, signs packed in bit 31
+    ptr, col = b.csr_ptr.numpy(), b.csr_col.numpy()
+    fp, fc = pl.fwd_ptr.numpy(), pl.fwd_col.numpy().view(np.uint32)
+    vals = b.csr_val.numpy()
+    for p in range(0, n, 7):
+        v = node[p]
+        got = fc[fp[p]:fp[p + 1]]
+        assert np.array_equal(got & 0x7FFFFFFF, col[ptr[v]:ptr[v + 1]])
+        assert np.array_equal((got >> 31).astype(bool), vals[ptr[v]:ptr[v + 1]] < 0)
+    # backward = CSC of Delta
+    bp, br = pl.bwd_ptr.numpy(), pl.bwd_row.numpy().view(np.uint32)
+    delta = sp.csr_matrix((vals.astype(float), col, ptr), shape=(n, n)).tocsc()
+    delta.sort_indices()
+    assert np.array_equal(bp, delta.indptr)
+    assert np.array_equal(br & 0x7FFFFFFF, delta.indices)
+    assert np.array_equal((br >> 31).astype(bool), delta.data < 0)
+    # children per position
+    cp, cn = pl.ch_ptr.numpy(), pl.ch_node.numpy()
+    for p in range(n):
+        kids = cn[cp[p]:cp[p + 1]]
+        assert np.all(b.parent.numpy()[kids] == node[p])
+    assert cp[-1] == b.stats["tree_edges"]
+
+
+def test_plan_rejects_cycles_and_double_parents():
+    crow = torch.tensor([0, 1, 2, 3], dtype=torch.int32)
+    col = torch.tensor([1, 2, 0], dtype=torch.int32)
+    val = torch.ones(3)
+    with pytest.raises(ValueError, match="cycle"):
+        structure.plan(3, crow, col, val, torch.tensor([0, 1, 2], dtype=torch.int32),
+                       torch.tensor([1, 2, 0], dtype=torch.int32))
+    with pytest.raises(ValueError, match="more than one parent"):
+        structure.plan(3, crow, col, val, torch.tensor([0, 1], dtype=torch.int32),
+                       torch.tensor([2, 2], dtype=torch.int32))
