@@ -44,9 +44,14 @@ def declare(lib, sig):
     sig(lib, "csr_destroy", c_i32, c_vp)
     sig(lib, "csr_set_algorithm", c_i32, c_vp, c_i32)
     sig(lib, "csr_matmul", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
+    sig(lib, "cbm_op_last_error", ctypes.c_char_p)
+    sig(lib, "cbm_op_spmm", c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_i32, ctypes.c_float,
+        c_i32, c_vp)
+    sig(lib, "cbm_op_update", c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_vp)
+    sig(lib, "cbm_op_axpy", c_i32, c_i32, c_i32, ctypes.c_float, c_vp, c_i64, c_vp, c_i64, c_vp)
 
 
 # Every symbol include/cbm_hip.h declares (checked by the CPU test suite).
 EXPORTED = ("cbm_last_error", "cbm_create", "cbm_destroy", "cbm_matmul", "cbm_matmul_t", "cbm_delta_spmm",
             "cbm_propagate", "cbm_last_launches", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
-            "csr_last_error")
+            "csr_last_error", "cbm_op_spmm", "cbm_op_update", "cbm_op_axpy", "cbm_op_last_error")

@@ -41,3 +41,16 @@ def init(src_row_idx, src_col_idx, src_values, src_n_rows, alpha):
         if not isinstance(t, torch.Tensor) or t.dtype != dt:
             raise RuntimeError(f'"{name}" is not a tensor of type "{dt}"')
     return structure.build(src_row_idx, src_col_idx, src_values, int(src_n_rows), int(alpha)).init_tuple()
+
+
+# The eight operators registered next to init (cbm/cbm_extensions.cpp:911-918).
+from .ops import (  # noqa: E402,F401
+    s_fused_update_csr_int32,
+    s_spmm_add_csr_int32,
+    s_spmm_csr_int32,
+    s_spmm_csr_t_int32,
+    s_spmm_fused_update_csr_int32,
+    s_spmm_update_add_csr_int32,
+    s_spmm_update_csr_int32,
+    s_update_csr_int32,
+)

@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
     if (p64 >= int64_t(a.p_end)) return;  // whole groups leave together
     const int32_t p = int32_t(p64);
     const int32_t row = a.node_of ? a.node_of[p] : p;
-    const int32_t kb = a.ptr[p], ke = a.ptr[p + 1];
+    const int32_t kb = a.ptr[p], ke = a.ptr_end ? a.ptr_end[p] : a.ptr[p + 1];
     const Tg *__restrict__ g = static_cast<const Tg *>(a.g);
     const float rs = a.rowscale ? a.rowscale[p] : 1.0f;
     constexpr int32_t kSpan = LPR * VE;
@@ -151,7 +151,8 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
             float mine_scale = 0.0f;
             if (base + l < ke) {
                 mine = a.idx[base + l];
-                const float cs = a.colscale ? a.colscale[mine & kIndexMask] : 1.0f;
+                float cs = a.colscale ? a.colscale[mine & kIndexMask] : 1.0f;
+                if (a.val) cs *= a.val[base + l];
                 mine_scale = signed_scale(cs, mine);
             }
             const int32_t cnt = min(LPR, ke - base);
@@ -191,14 +192,14 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
         if constexpr (EXTRA == kExtraParent) {
             const int32_t par = a.parent_of[p];
             if (par >= 0) {
-                const float es = a.escale ? a.escale[p] : 1.0f;
+                const float es = (a.escale ? a.escale[p] : 1.0f) * a.ecoef;
                 float v[VE];
                 Load<Te, VE>::run(static_cast<const Te *>(a.e) + int64_t(par) * a.lde + c0, v);
 #pragma unroll
                 for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], es, acc[q]);
             }
         } else if constexpr (EXTRA == kExtraSelf) {
-            const float es = a.escale ? a.escale[row] : 1.0f;
+            const float es = (a.escale ? a.escale[row] : 1.0f) * a.ecoef;
             float v[VE];
             Load<Te, VE>::run(static_cast<const Te *>(a.e) + int64_t(row) * a.lde + c0, v);
 #pragma unroll
@@ -233,6 +234,35 @@ __global__ __launch_bounds__(256) void k_propagate(int32_t p_begin, int32_t p_en
         for (int q = 0; q < VE; ++q) b[q] = fmaf(a[q], ps, b[q]);
         store_row<VE>(y + int64_t(row) * ldy + c0, b);
     }
+}
+
+__global__ __launch_bounds__(256) void k_branch_update(int32_t n_branches, int32_t col_blocks,
+                                                       const int32_t *__restrict__ branches,
+                                                       const int32_t *__restrict__ src, const int32_t *__restrict__ dst,
+                                                       const float *__restrict__ mult, float *y, int64_t ldy,
+                                                       int32_t ncols) {
+    const int32_t b = int32_t(blockIdx.x) / col_blocks;
+    const int32_t c = (int32_t(blockIdx.x) % col_blocks) * int32_t(blockDim.x) + int32_t(threadIdx.x);
+    if (b >= n_branches || c >= ncols) return;
+    for (int32_t e = branches[b]; e < branches[b + 1]; ++e) {
+        const int32_t s = src[e], d = dst[e];
+        const float ys = y[int64_t(s) * ldy + c];
+        float *yd = y + int64_t(d) * ldy + c;
+        if (mult) {
+            const float t = (ys * mult[d]) / mult[s];
+            *yd = *yd + t;
+        } else {
+            *yd = *yd + ys;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_axpy(int32_t n_rows, int32_t ncols, float coef, const float *__restrict__ x,
+                                              int64_t ldx, float *__restrict__ y, int64_t ldy) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= int64_t(n_rows) * ncols) return;
+    const int64_t r = i / ncols, c = i % ncols;
+    y[r * ldy + c] = fmaf(coef, x[r * ldx + c], y[r * ldy + c]);
 }
 
 __global__ __launch_bounds__(256) void k_dad_scales(int32_t n, const int32_t *__restrict__ node_of,
@@ -332,6 +362,23 @@ hipError_t launch_propagate(int32_t p_begin, int32_t p_end, const int32_t *node_
     CBM_PROP_CASE(1, 32) CBM_PROP_CASE(1, 64)
 #undef CBM_PROP_CASE
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_branch_update(int32_t n_branches, const int32_t *branches, const int32_t *src, const int32_t *dst,
+                                const float *mult, float *y, int64_t ldy, int32_t ncols, hipStream_t stream) {
+    if (n_branches <= 0 || ncols <= 0) return hipSuccess;
+    const int32_t col_blocks = (ncols + 255) / 256;
+    k_branch_update<<<dim3(unsigned(int64_t(n_branches) * col_blocks)), dim3(256), 0, stream>>>(
+        n_branches, col_blocks, branches, src, dst, mult, y, ldy, ncols);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy(int32_t n_rows, int32_t ncols, float coef, const float *x, int64_t ldx, float *y, int64_t ldy,
+                       hipStream_t stream) {
+    const int64_t total = int64_t(n_rows) * ncols;
+    if (total <= 0) return hipSuccess;
+    k_axpy<<<dim3(unsigned((total + 255) / 256)), dim3(256), 0, stream>>>(n_rows, ncols, coef, x, ldx, y, ldy);
+    return hipGetLastError();
 }
 
 hipError_t launch_dad_scales(int32_t n, const int32_t *node_of, const int32_t *parent_of, const float *d, float *rs,

@@ -24,7 +24,9 @@ struct RowPass {
     const int32_t *node_of = nullptr;
     const int32_t *parent_of = nullptr;
     const int32_t *ptr = nullptr;
+    const int32_t *ptr_end = nullptr; // row ends (4-array CSR); null = ptr[p + 1]
     const int32_t *idx = nullptr;
+    const float *val = nullptr;       // per-entry values; null = +-1 from the sign bit
     const float *colscale = nullptr;  // indexed by gathered row; null = 1
     const float *rowscale = nullptr;  // indexed by position;     null = 1
     const float *escale = nullptr;    // by position (parent) or by row (self); null = 1
@@ -35,6 +37,7 @@ struct RowPass {
     int64_t lde = 0;
     bool e_bf16 = false;
     Extra extra = kExtraNone;
+    float ecoef = 1.0f;               // extra term multiplier, on top of escale
     float *y = nullptr;
     int64_t ldy = 0;
     int32_t ncols = 0;
@@ -46,6 +49,17 @@ hipError_t launch_row_pass(const RowPass &rp, hipStream_t stream);
 // In-place propagation of one level: y[node_of[p]] += parscale[p] * y[parent_of[p]].
 hipError_t launch_propagate(int32_t p_begin, int32_t p_end, const int32_t *node_of, const int32_t *parent_of,
                             const float *parscale, float *y, int64_t ldy, int32_t ncols, hipStream_t stream);
+
+// The reference's branch-serial propagation (cbm/cbm.cu:25-91) for the
+// operator surface: one workgroup per branch and column block walks the
+// branch's edges in order, y[dst] += y[src] (mult null) or
+// y[dst] += (y[src] * mult[dst]) / mult[src].
+hipError_t launch_branch_update(int32_t n_branches, const int32_t *branches, const int32_t *src, const int32_t *dst,
+                                const float *mult, float *y, int64_t ldy, int32_t ncols, hipStream_t stream);
+
+// y[r, :] += coef * x[r, :] for r < n_rows (FP32).
+hipError_t launch_axpy(int32_t n_rows, int32_t ncols, float coef, const float *x, int64_t ldx, float *y, int64_t ldy,
+                       hipStream_t stream);
 
 // Per-position scales of the D A D product: rs[p] = d[node[p]],
 // ps[p] = d[node[p]] / d[parent[p]] (0 for roots and zero-degree parents).

@@ -146,6 +146,38 @@ int csr_set_algorithm(csr_handle *h, int32_t alg);
 /* Message describing the last csr_* failure on the calling thread. */
 const char *csr_last_error(void);
 
+/* ------------------------------------------------------------------------- */
+/* Operator surface: the s_*_csr_int32 ops of cbm/cbm_extensions.cpp:351-907  */
+/* on device arrays.  A general CSR is given as MKL-style row begin / end     */
+/* arrays (row_e may be NULL: row_b[r + 1] ends row r) with FP32 values.      */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * dst = A rhs, or dst = A rhs + self_coef * rhs when add_self != 0.
+ * Backs s_spmm_csr_int32 (:700-740) and s_spmm_add_csr_int32 (:446-492,
+ * self_coef = 1 + epsilon).
+ */
+int cbm_op_spmm(int32_t n_rows, const int32_t *row_b, const int32_t *row_e, const int32_t *col, const float *val,
+                const float *rhs, int64_t ldr, float *dst, int64_t ldd, int32_t n_cols, float self_coef,
+                int32_t add_self, hipStream_t stream);
+
+/*
+ * Branch-serial propagation in place over y: for each branch b, for each
+ * edge e in [branches[b], branches[b+1]) in order, y[dst[e]] += y[src[e]]
+ * (mult == NULL) or y[dst[e]] += (y[src[e]] * mult[dst[e]]) / mult[src[e]].
+ * Backs s_update_csr_int32 (:784-838) and s_fused_update_csr_int32 (:841-907),
+ * and the propagation halves of the s_spmm_*update* ops.
+ */
+int cbm_op_update(int32_t n_branches, const int32_t *branches, const int32_t *src, const int32_t *dst,
+                  const float *mult, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream);
+
+/* y += coef * x over n_rows x n_cols (the cblas_saxpy of :491,590). */
+int cbm_op_axpy(int32_t n_rows, int32_t n_cols, float coef, const float *x, int64_t ldx, float *y, int64_t ldy,
+                hipStream_t stream);
+
+/* Message describing the last cbm_op_* failure on the calling thread. */
+const char *cbm_op_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,13 +114,17 @@ def test_reference_protocol_vs_rocsparse(mats, cuda, mode):
 
 @pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
 def test_adjoint_vs_rocsparse_transpose(mats, cuda, mode):
+    """M^T G against rocSPARSE's transposed product.  Even with G >= 0 the
+    Delta^T sums mix signs, so an element far below the row's scale can lose
+    relative accuracy to cancellation: the bound is rtol = 1e-5 plus an
+    absolute floor of 1e-5 * max|M^T G|."""
     m, c = mats[mode]
     g = torch.from_numpy(_rand((N1, 32), 91, "uniform")).to(cuda)
     d_cbm = torch.empty(N1, 32, device=cuda)
     d_csr = torch.empty(N1, 32, device=cuda)
     m.matmul_t(g, d_cbm)
     c.matmul_t(g, d_csr)
-    torch.testing.assert_close(d_csr, d_cbm, rtol=1e-5, atol=0)
+    torch.testing.assert_close(d_csr, d_cbm, rtol=1e-5, atol=1e-5 * float(d_csr.abs().max()))
 
 
 @pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
