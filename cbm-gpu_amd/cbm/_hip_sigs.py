@@ -17,23 +17,52 @@ class CbmDesc(ctypes.Structure):
         ("n_edges", c_i64),
         ("mode", c_i32),
         ("lvl_ptr_host", c_vp),
+        ("lvl_nlong_host", c_vp),
+        ("fck_ptr_host", c_vp),
+        ("fred_ptr_host", c_vp),
         ("lvl_node", c_vp),
         ("lvl_parent", c_vp),
         ("fwd_ptr", c_vp),
         ("fwd_col", c_vp),
-        ("bwd_ptr", c_vp),
-        ("bwd_row", c_vp),
         ("ch_ptr", c_vp),
         ("ch_node", c_vp),
+        ("fck_pos", c_vp),
+        ("fck_meta", c_vp),
+        ("fck_slot", c_vp),
+        ("fred_pos", c_vp),
+        ("fred_slot", c_vp),
+        ("fred_cnt", c_vp),
+        ("fwd_slots", c_i32),
+        ("bwd_node", c_vp),
+        ("bwd_ptr", c_vp),
+        ("bwd_row", c_vp),
+        ("bck_pos", c_vp),
+        ("bck_meta", c_vp),
+        ("bck_slot", c_vp),
+        ("bred_pos", c_vp),
+        ("bred_slot", c_vp),
+        ("bred_cnt", c_vp),
+        ("bwd_nlong", c_i32),
+        ("n_bck", c_i32),
+        ("n_bred", c_i32),
+        ("bwd_slots", c_i32),
         ("d", c_vp),
     ]
+
+# Device arrays of the plan that the descriptor points at.
+DEVICE_ARRAYS = ("lvl_node", "lvl_parent", "fwd_ptr", "fwd_col", "ch_ptr", "ch_node", "fck_pos", "fck_meta",
+                 "fck_slot", "fred_pos", "fred_slot", "fred_cnt", "bwd_node", "bwd_ptr", "bwd_row", "bck_pos",
+                 "bck_meta", "bck_slot", "bred_pos", "bred_slot", "bred_cnt")
+HOST_ARRAYS = ("lvl_ptr", "lvl_nlong", "fck_ptr", "fred_ptr")
 
 
 def declare(lib, sig):
     P_vp = ctypes.POINTER(c_vp)
     sig(lib, "cbm_last_error", ctypes.c_char_p)
+    sig(lib, "cbm_desc_size", c_i32)
     sig(lib, "cbm_create", c_i32, ctypes.POINTER(CbmDesc), c_vp, P_vp)
     sig(lib, "cbm_destroy", c_i32, c_vp)
+    sig(lib, "cbm_reserve", c_i32, c_vp, c_i32)
     sig(lib, "cbm_matmul", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
     sig(lib, "cbm_matmul_t", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
     sig(lib, "cbm_delta_spmm", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
@@ -52,6 +81,6 @@ def declare(lib, sig):
 
 
 # Every symbol include/cbm_hip.h declares (checked by the CPU test suite).
-EXPORTED = ("cbm_last_error", "cbm_create", "cbm_destroy", "cbm_matmul", "cbm_matmul_t", "cbm_delta_spmm",
+EXPORTED = ("cbm_last_error", "cbm_desc_size", "cbm_create", "cbm_destroy", "cbm_reserve", "cbm_matmul", "cbm_matmul_t", "cbm_delta_spmm",
             "cbm_propagate", "cbm_last_launches", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
             "csr_last_error", "cbm_op_spmm", "cbm_op_update", "cbm_op_axpy", "cbm_op_last_error")

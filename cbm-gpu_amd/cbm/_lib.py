@@ -69,6 +69,7 @@ def host():
             _sig(lib, "cbm_plan_size", c_i64, c_vp, c_i32)
             _sig(lib, "cbm_plan_array", c_vp, c_vp, c_i32)
             _sig(lib, "cbm_plan_depth", c_i32, c_vp)
+            _sig(lib, "cbm_plan_stat", c_i64, c_vp, c_i32)
             _sig(lib, "cbm_plan_free", None, c_vp)
             _host = lib
     return _host

@@ -12,7 +12,7 @@ import ctypes
 import torch
 
 from . import _lib, structure
-from ._hip_sigs import BF16, F32, MODE_AX, MODE_ADX, MODE_DADX, CbmDesc
+from ._hip_sigs import BF16, DEVICE_ARRAYS, F32, HOST_ARRAYS, MODE_AX, MODE_ADX, MODE_DADX, CbmDesc
 
 _MODE_NAMES = {MODE_AX: "A", MODE_ADX: "A D^-1/2", MODE_DADX: "D^-1/2 A D^-1/2"}
 
@@ -106,8 +106,9 @@ class cusparseCBM:
         self.nnz = int(col.numel())
         self.n_edges = int(mca_src_node_idx.numel())
         self.level_sizes = plan.lvl_ptr.diff().tolist()
-        self._lvl_ptr_host = plan.lvl_ptr.contiguous()
-        self._dev = {k: v.to(self.device) for k, v in plan.arrays.items() if k != "lvl_ptr"}
+        self.plan_stats = dict(plan.stats)
+        self._host = {k: plan.arrays[k].contiguous() for k in HOST_ARRAYS}
+        self._dev = {k: plan.arrays[k].to(self.device) for k in DEVICE_ARRAYS}
         self._lib = _lib.hip()
         self._bind(mode, scale)
 
@@ -115,12 +116,13 @@ class cusparseCBM:
         """Creates the native handle for product `mode` over this layout."""
         self.mode = int(mode)
         self._d = None if self.mode == MODE_AX else scale.to(self.device, torch.float32).contiguous()
-        dp = {k: v.data_ptr() for k, v in self._dev.items()}
+        ptrs = {k: (v.data_ptr() or None) for k, v in self._dev.items()}
+        ptrs.update({k + "_host": v.data_ptr() for k, v in self._host.items()})
+        st = self.plan_stats
         desc = CbmDesc(n_rows=self.num_rows, n_levels=self.depth, nnz=self.nnz, n_edges=self.n_edges,
-                       mode=self.mode, lvl_ptr_host=self._lvl_ptr_host.data_ptr(), lvl_node=dp["lvl_node"],
-                       lvl_parent=dp["lvl_parent"], fwd_ptr=dp["fwd_ptr"], fwd_col=dp["fwd_col"] or None,
-                       bwd_ptr=dp["bwd_ptr"], bwd_row=dp["bwd_row"] or None, ch_ptr=dp["ch_ptr"],
-                       ch_node=dp["ch_node"] or None, d=self._d.data_ptr() if self._d is not None else None)
+                       mode=self.mode, fwd_slots=st["fwd_slots"], bwd_nlong=st["bwd_nlong"],
+                       n_bck=int(self._dev["bck_pos"].numel()), n_bred=int(self._dev["bred_pos"].numel()),
+                       bwd_slots=st["bwd_slots"], d=self._d.data_ptr() if self._d is not None else None, **ptrs)
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             rc = self._lib.cbm_create(ctypes.byref(desc), current_stream_handle(self.device), ctypes.byref(h))
@@ -131,7 +133,8 @@ class cusparseCBM:
         """A handle for another product (A, A D or D A D) over the same device
         layout; the layout arrays are shared, not copied."""
         other = object.__new__(type(self))
-        for k in ("device", "num_rows", "depth", "nnz", "n_edges", "level_sizes", "_lvl_ptr_host", "_dev", "_lib"):
+        for k in ("device", "num_rows", "depth", "nnz", "n_edges", "level_sizes", "plan_stats", "_host", "_dev",
+                  "_lib"):
             setattr(other, k, getattr(self, k))
         if mode != MODE_AX and scale is None:
             raise ValueError(f"mode {mode} needs the D^-1/2 vector as `scale`")
@@ -242,6 +245,11 @@ class cusparseCBM:
         rc = self._lib.cbm_propagate(self._h, res.data_ptr(), res.stride(0), f, current_stream_handle(self.device))
         _lib.check_hip(rc, "propagate")
         return 0
+
+    def reserve(self, n_cols):
+        """Pre-sizes the long-row scratch so that products of up to n_cols
+        columns allocate nothing (needed before HIP-graph capture)."""
+        _lib.check_hip(self._lib.cbm_reserve(self._h, int(n_cols)), "reserve")
 
     def last_launches(self):
         return int(self._lib.cbm_last_launches(self._h))

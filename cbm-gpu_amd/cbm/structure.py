@@ -12,7 +12,10 @@ from . import _lib
  _B_CSR_VAL) = range(10)
 _STAT_NAMES = ("n", "nnz_a", "nnz_delta", "candidates", "msa_weight", "tree_edges", "branches", "depth")
 _PLAN_NAMES = ("lvl_ptr", "lvl_node", "lvl_parent", "fwd_ptr", "fwd_col", "bwd_ptr", "bwd_row", "ch_ptr",
-               "ch_node")
+               "ch_node", "lvl_nlong", "fck_ptr", "fck_pos", "fck_meta", "fck_slot", "fred_ptr", "fred_pos",
+               "fred_slot", "fred_cnt", "bwd_node", "bck_pos", "bck_meta", "bck_slot", "bred_pos", "bred_slot",
+               "bred_cnt")
+_PLAN_STATS = ("depth", "fwd_slots", "bwd_slots", "bwd_nlong")
 
 
 def _copy_out(ptr, n, dtype):
@@ -101,6 +104,7 @@ class Plan:
     n: int
     depth: int
     arrays: dict
+    stats: dict = field(default_factory=dict)
 
     def __getattr__(self, name):
         arrays = self.__dict__.get("arrays")
@@ -130,6 +134,7 @@ def plan(n_rows, crow, col, val, tree_src, tree_dst, n_threads=0):
     try:
         arrays = {name: _copy_out(lib.cbm_plan_array(p, i), lib.cbm_plan_size(p, i), torch.int32)
                   for i, name in enumerate(_PLAN_NAMES)}
-        return Plan(n=n_rows, depth=int(lib.cbm_plan_depth(p)), arrays=arrays)
+        stats = {name: int(lib.cbm_plan_stat(p, i)) for i, name in enumerate(_PLAN_STATS)}
+        return Plan(n=n_rows, depth=int(lib.cbm_plan_depth(p)), arrays=arrays, stats=stats)
     finally:
         lib.cbm_plan_free(p)

@@ -5,6 +5,7 @@
 #include "cbm_kernels.hpp"
 #include "../../../include/cbm_hip.h"
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <new>
@@ -13,9 +14,11 @@
 
 struct cbm_handle {
     cbm_desc d{};
-    std::vector<int32_t> lvl_ptr;  // host copy of the level offsets
-    float *rs_pos = nullptr;       // D A D only: d[node] per level position
-    float *ps_pos = nullptr;       // D A D only: d[node] / d[parent] per level position
+    // host copies of the per-level tables
+    std::vector<int32_t> lvl_ptr, lvl_nlong, fck_ptr, fred_ptr;
+    float *ps_pos = nullptr;  // D A D only: d[node] / d[parent] per level position
+    float *scratch = nullptr; // partial sums of long-row chunks
+    int64_t scratch_floats = 0;
     int last_launches = 0;
 };
 
@@ -69,12 +72,102 @@ int check_dense(const cbm_handle *h, int32_t dtype, const void *in, int64_t ldin
     return CBM_OK;
 }
 
-// Scale vectors of the forward Delta pass by mode.
-const float *fwd_colscale(const cbm_handle *h) { return h->d.mode == CBM_MODE_AX ? nullptr : h->d.d; }
+int ensure_scratch(cbm_handle *h, int32_t n_cols) {
+    const int64_t slots = std::max<int64_t>(h->d.fwd_slots, h->d.bwd_slots);
+    const int64_t need = slots * int64_t(n_cols);
+    if (need <= h->scratch_floats) return CBM_OK;
+    if (h->scratch) {
+        const hipError_t e = hipFree(h->scratch);
+        h->scratch = nullptr;
+        h->scratch_floats = 0;
+        if (e != hipSuccess) return device_error(e, "cbm scratch");
+    }
+    const hipError_t e = hipMalloc(&h->scratch, sizeof(float) * size_t(need));
+    if (e != hipSuccess) {
+        h->scratch = nullptr;
+        return device_error(e, "cbm scratch");
+    }
+    h->scratch_floats = need;
+    return CBM_OK;
+}
+
+// One ordered row range whose first n_long rows are long: the chunk kernel,
+// the short-row pass and the reduction of multi-chunk rows, in that order.
+int run_range(cbm_handle *h, cbmk::RowPass rp, int32_t p_begin, int32_t p_end, int32_t n_long,
+              const cbmk::ChunkArgs &ck, const cbmk::ReduceArgs &rd, hipStream_t stream, const char *what) {
+    hipError_t e = hipSuccess;
+    if (ck.count > 0) {
+        e = cbmk::launch_row_chunks(rp, ck, stream);
+        if (e != hipSuccess) return device_error(e, what);
+        ++h->last_launches;
+    }
+    rp.p_begin = p_begin + n_long;
+    rp.p_end = p_end;
+    if (rp.p_end > rp.p_begin) {
+        e = cbmk::launch_row_pass(rp, stream);
+        if (e != hipSuccess) return device_error(e, what);
+        ++h->last_launches;
+    }
+    if (rd.count > 0) {
+        e = cbmk::launch_row_reduce(rp, rd, stream);
+        if (e != hipSuccess) return device_error(e, what);
+        ++h->last_launches;
+    }
+    return CBM_OK;
+}
+
+// The forward Delta pass over every level; with_parent adds the parent term.
+int forward_levels(cbm_handle *h, cbmk::RowPass rp, bool with_parent, hipStream_t stream, const char *what) {
+    cbmk::ChunkArgs ck;
+    ck.pos = h->d.fck_pos;
+    ck.meta = h->d.fck_meta;
+    ck.slot = h->d.fck_slot;
+    ck.scratch = h->scratch;
+    cbmk::ReduceArgs rd;
+    rd.pos = h->d.fred_pos;
+    rd.slot = h->d.fred_slot;
+    rd.cnt = h->d.fred_cnt;
+    rd.scratch = h->scratch;
+    for (int32_t l = 0; l < h->d.n_levels; ++l) {
+        rp.extra = (with_parent && l > 0) ? cbmk::kExtraParent : cbmk::kExtraNone;
+        ck.first = h->fck_ptr[size_t(l)];
+        ck.count = h->fck_ptr[size_t(l) + 1] - ck.first;
+        rd.first = h->fred_ptr[size_t(l)];
+        rd.count = h->fred_ptr[size_t(l) + 1] - rd.first;
+        const int rc = run_range(h, rp, h->lvl_ptr[size_t(l)], h->lvl_ptr[size_t(l) + 1], h->lvl_nlong[size_t(l)], ck,
+                                 rd, stream, what);
+        if (rc != CBM_OK) return rc;
+    }
+    return CBM_OK;
+}
+
+cbmk::RowPass forward_pass(const cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy,
+                           int32_t n_cols) {
+    cbmk::RowPass rp;
+    rp.node_of = h->d.lvl_node;
+    rp.parent_of = h->d.lvl_parent;
+    rp.ptr = h->d.fwd_ptr;
+    rp.idx = h->d.fwd_col;
+    rp.colscale = h->d.mode == CBM_MODE_AX ? nullptr : h->d.d;
+    rp.rowscale = h->d.mode == CBM_MODE_DADX ? h->d.d : nullptr;
+    rp.escale = h->d.mode == CBM_MODE_DADX ? h->ps_pos : nullptr;
+    rp.g = x;
+    rp.ldg = ldx;
+    rp.g_bf16 = dtype == CBM_BF16;
+    rp.e = y;
+    rp.lde = ldy;
+    rp.e_bf16 = false;
+    rp.y = y;
+    rp.ldy = ldy;
+    rp.ncols = n_cols;
+    return rp;
+}
 
 }  // namespace
 
 extern "C" const char *cbm_last_error(void) { return g_error.c_str(); }
+
+extern "C" int32_t cbm_desc_size(void) { return int32_t(sizeof(cbm_desc)); }
 
 extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle **out) {
     if (!desc || !out) {
@@ -83,9 +176,13 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
     }
     *out = nullptr;
     const cbm_desc &d = *desc;
-    if (d.n_rows < 1 || d.n_levels < 1 || d.nnz < 0 || d.n_edges < 0 || d.n_edges >= d.n_rows || !d.lvl_ptr_host ||
-        !d.lvl_node || !d.lvl_parent || !d.fwd_ptr || !d.bwd_ptr || !d.ch_ptr || (d.nnz > 0 && (!d.fwd_col || !d.bwd_row)) ||
-        (d.n_edges > 0 && !d.ch_node)) {
+    const bool tables = d.lvl_ptr_host && d.lvl_nlong_host && d.fck_ptr_host && d.fred_ptr_host;
+    if (d.n_rows < 1 || d.n_levels < 1 || d.nnz < 0 || d.n_edges < 0 || d.n_edges >= d.n_rows || !tables ||
+        !d.lvl_node || !d.lvl_parent || !d.fwd_ptr || !d.bwd_ptr || !d.bwd_node || !d.ch_ptr ||
+        (d.nnz > 0 && (!d.fwd_col || !d.bwd_row)) || (d.n_edges > 0 && !d.ch_node) || d.fwd_slots < 0 ||
+        d.bwd_slots < 0 || d.n_bck < 0 || d.n_bred < 0 || d.bwd_nlong < 0 || d.bwd_nlong > d.n_rows ||
+        (d.n_bck > 0 && (!d.bck_pos || !d.bck_meta || !d.bck_slot)) ||
+        (d.n_bred > 0 && (!d.bred_pos || !d.bred_slot || !d.bred_cnt))) {
         set_error("cbm_create: incomplete descriptor (n_rows=%d levels=%d nnz=%lld edges=%lld)", d.n_rows, d.n_levels,
                   (long long)d.nnz, (long long)d.n_edges);
         return CBM_ERR_ARG;
@@ -98,29 +195,41 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
         set_error("cbm_create: mode %d needs the D^-1/2 vector", d.mode);
         return CBM_ERR_ARG;
     }
-    if (d.lvl_ptr_host[0] != 0 || d.lvl_ptr_host[d.n_levels] != d.n_rows) {
-        set_error("cbm_create: level offsets run from %d to %d, expected 0 to %d", d.lvl_ptr_host[0],
-                  d.lvl_ptr_host[d.n_levels], d.n_rows);
+    const int32_t L = d.n_levels;
+    if (d.lvl_ptr_host[0] != 0 || d.lvl_ptr_host[L] != d.n_rows || d.fck_ptr_host[0] != 0 || d.fred_ptr_host[0] != 0) {
+        set_error("cbm_create: level tables do not start at 0 / end at n_rows");
         return CBM_ERR_ARG;
     }
-    for (int32_t l = 0; l < d.n_levels; ++l)
-        if (d.lvl_ptr_host[l + 1] <= d.lvl_ptr_host[l]) {
-            set_error("cbm_create: level %d is empty", l);
+    for (int32_t l = 0; l < L; ++l) {
+        if (d.lvl_ptr_host[l + 1] <= d.lvl_ptr_host[l] || d.lvl_nlong_host[l] < 0 ||
+            d.lvl_nlong_host[l] > d.lvl_ptr_host[l + 1] - d.lvl_ptr_host[l] ||
+            d.fck_ptr_host[l + 1] < d.fck_ptr_host[l] || d.fred_ptr_host[l + 1] < d.fred_ptr_host[l]) {
+            set_error("cbm_create: inconsistent level tables at level %d", l);
             return CBM_ERR_ARG;
         }
+    }
+    if ((d.fck_ptr_host[L] > 0 && (!d.fck_pos || !d.fck_meta || !d.fck_slot)) ||
+        (d.fred_ptr_host[L] > 0 && (!d.fred_pos || !d.fred_slot || !d.fred_cnt))) {
+        set_error("cbm_create: missing forward chunk tables");
+        return CBM_ERR_ARG;
+    }
     cbm_handle *h = new (std::nothrow) cbm_handle;
     if (!h) {
         set_error("cbm_create: out of host memory");
         return CBM_ERR_NOMEM;
     }
     h->d = d;
-    h->lvl_ptr.assign(d.lvl_ptr_host, d.lvl_ptr_host + d.n_levels + 1);
+    h->lvl_ptr.assign(d.lvl_ptr_host, d.lvl_ptr_host + L + 1);
+    h->lvl_nlong.assign(d.lvl_nlong_host, d.lvl_nlong_host + L);
+    h->fck_ptr.assign(d.fck_ptr_host, d.fck_ptr_host + L + 1);
+    h->fred_ptr.assign(d.fred_ptr_host, d.fred_ptr_host + L + 1);
     h->d.lvl_ptr_host = h->lvl_ptr.data();
+    h->d.lvl_nlong_host = h->lvl_nlong.data();
+    h->d.fck_ptr_host = h->fck_ptr.data();
+    h->d.fred_ptr_host = h->fred_ptr.data();
     if (d.mode == CBM_MODE_DADX) {
-        hipError_t e = hipMalloc(&h->rs_pos, sizeof(float) * size_t(d.n_rows));
-        if (e == hipSuccess) e = hipMalloc(&h->ps_pos, sizeof(float) * size_t(d.n_rows));
-        if (e == hipSuccess)
-            e = cbmk::launch_dad_scales(d.n_rows, d.lvl_node, d.lvl_parent, d.d, h->rs_pos, h->ps_pos, stream);
+        hipError_t e = hipMalloc(&h->ps_pos, sizeof(float) * size_t(d.n_rows));
+        if (e == hipSuccess) e = cbmk::launch_dad_scales(d.n_rows, d.lvl_node, d.lvl_parent, d.d, h->ps_pos, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) {
             cbm_destroy(h);
@@ -133,45 +242,32 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
 
 extern "C" int cbm_destroy(cbm_handle *h) {
     if (!h) return CBM_OK;
-    hipError_t e1 = h->rs_pos ? hipFree(h->rs_pos) : hipSuccess;
-    hipError_t e2 = h->ps_pos ? hipFree(h->ps_pos) : hipSuccess;
+    hipError_t e = hipSuccess;
+    for (float *p : {h->ps_pos, h->scratch}) {
+        if (!p) continue;
+        const hipError_t ei = hipFree(p);
+        if (e == hipSuccess) e = ei;
+    }
     delete h;
-    if (e1 != hipSuccess) return device_error(e1, "cbm_destroy");
-    if (e2 != hipSuccess) return device_error(e2, "cbm_destroy");
+    if (e != hipSuccess) return device_error(e, "cbm_destroy");
     return CBM_OK;
+}
+
+extern "C" int cbm_reserve(cbm_handle *h, int32_t n_cols) {
+    if (!h || n_cols < 0) {
+        set_error("cbm_reserve: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    return ensure_scratch(h, n_cols);
 }
 
 extern "C" int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy,
                           int32_t n_cols, hipStream_t stream) {
     int rc = check_dense(h, dtype, x, ldx, y, ldy, n_cols, "cbm_matmul");
+    if (rc == CBM_OK) rc = ensure_scratch(h, n_cols);
     if (rc != CBM_OK) return rc;
-    cbmk::RowPass rp;
-    rp.node_of = h->d.lvl_node;
-    rp.parent_of = h->d.lvl_parent;
-    rp.ptr = h->d.fwd_ptr;
-    rp.idx = h->d.fwd_col;
-    rp.colscale = fwd_colscale(h);
-    rp.rowscale = h->d.mode == CBM_MODE_DADX ? h->rs_pos : nullptr;
-    rp.escale = h->d.mode == CBM_MODE_DADX ? h->ps_pos : nullptr;
-    rp.g = x;
-    rp.ldg = ldx;
-    rp.g_bf16 = dtype == CBM_BF16;
-    rp.e = y;
-    rp.lde = ldy;
-    rp.e_bf16 = false;
-    rp.y = y;
-    rp.ldy = ldy;
-    rp.ncols = n_cols;
     h->last_launches = 0;
-    for (int32_t l = 0; l < h->d.n_levels; ++l) {
-        rp.p_begin = h->lvl_ptr[size_t(l)];
-        rp.p_end = h->lvl_ptr[size_t(l) + 1];
-        rp.extra = l == 0 ? cbmk::kExtraNone : cbmk::kExtraParent;
-        const hipError_t e = cbmk::launch_row_pass(rp, stream);
-        if (e != hipSuccess) return device_error(e, "cbm_matmul");
-        ++h->last_launches;
-    }
-    return CBM_OK;
+    return forward_levels(h, forward_pass(h, dtype, x, ldx, y, ldy, n_cols), true, stream, "cbm_matmul");
 }
 
 extern "C" int cbm_matmul_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, float *dx, int64_t lddx,
@@ -182,13 +278,14 @@ extern "C" int cbm_matmul_t(cbm_handle *h, int32_t dtype, const void *g, int64_t
         set_error("cbm_matmul_t: invalid workspace (ldz=%lld)", (long long)ldz);
         return CBM_ERR_ARG;
     }
-    const int64_t n = h->d.n_rows;
-    if (overlaps(z, span_bytes(int32_t(n), ldz, n_cols, 4), dx, span_bytes(int32_t(n), lddx, n_cols, 4)) ||
-        overlaps(z, span_bytes(int32_t(n), ldz, n_cols, 4), g,
-                 span_bytes(int32_t(n), ldg, n_cols, dtype == CBM_BF16 ? 2 : 4))) {
+    const int32_t n = h->d.n_rows;
+    if (overlaps(z, span_bytes(n, ldz, n_cols, 4), dx, span_bytes(n, lddx, n_cols, 4)) ||
+        overlaps(z, span_bytes(n, ldz, n_cols, 4), g, span_bytes(n, ldg, n_cols, dtype == CBM_BF16 ? 2 : 4))) {
         set_error("cbm_matmul_t: workspace overlaps an operand");
         return CBM_ERR_ARG;
     }
+    rc = ensure_scratch(h, n_cols);
+    if (rc != CBM_OK) return rc;
     h->last_launches = 0;
     // Reverse pass, deepest level first: z_u = s_u g_u + sum_{children v} z_v
     // with s = D^-1/2 for D A D (the adjoint's (I-P)^-T applied to D G).
@@ -216,44 +313,39 @@ extern "C" int cbm_matmul_t(cbm_handle *h, int32_t dtype, const void *g, int64_t
     }
     // Delta^T pass: dx_j = s_j * sum_v sign * z_v over column j of Delta.
     cbmk::RowPass tp;
-    tp.p_begin = 0;
-    tp.p_end = int32_t(n);
+    tp.node_of = h->d.bwd_node;
     tp.ptr = h->d.bwd_ptr;
     tp.idx = h->d.bwd_row;
-    tp.rowscale = h->d.mode == CBM_MODE_AX ? nullptr : h->d.d;
     tp.g = z;
     tp.ldg = ldz;
     tp.y = dx;
     tp.ldy = lddx;
     tp.ncols = n_cols;
-    const hipError_t e = cbmk::launch_row_pass(tp, stream);
-    if (e != hipSuccess) return device_error(e, "cbm_matmul_t (Delta^T pass)");
-    ++h->last_launches;
-    return CBM_OK;
+    tp.rowscale = h->d.mode == CBM_MODE_AX ? nullptr : h->d.d;
+    cbmk::ChunkArgs ck;
+    ck.pos = h->d.bck_pos;
+    ck.meta = h->d.bck_meta;
+    ck.slot = h->d.bck_slot;
+    ck.first = 0;
+    ck.count = h->d.n_bck;
+    ck.scratch = h->scratch;
+    cbmk::ReduceArgs rd;
+    rd.pos = h->d.bred_pos;
+    rd.slot = h->d.bred_slot;
+    rd.cnt = h->d.bred_cnt;
+    rd.first = 0;
+    rd.count = h->d.n_bred;
+    rd.scratch = h->scratch;
+    return run_range(h, tp, 0, n, h->d.bwd_nlong, ck, rd, stream, "cbm_matmul_t (Delta^T pass)");
 }
 
 extern "C" int cbm_delta_spmm(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy,
                               int32_t n_cols, hipStream_t stream) {
     int rc = check_dense(h, dtype, x, ldx, y, ldy, n_cols, "cbm_delta_spmm");
+    if (rc == CBM_OK) rc = ensure_scratch(h, n_cols);
     if (rc != CBM_OK) return rc;
-    cbmk::RowPass rp;
-    rp.p_begin = 0;
-    rp.p_end = h->d.n_rows;
-    rp.node_of = h->d.lvl_node;
-    rp.ptr = h->d.fwd_ptr;
-    rp.idx = h->d.fwd_col;
-    rp.colscale = fwd_colscale(h);
-    rp.rowscale = h->d.mode == CBM_MODE_DADX ? h->rs_pos : nullptr;
-    rp.g = x;
-    rp.ldg = ldx;
-    rp.g_bf16 = dtype == CBM_BF16;
-    rp.y = y;
-    rp.ldy = ldy;
-    rp.ncols = n_cols;
-    const hipError_t e = cbmk::launch_row_pass(rp, stream);
-    if (e != hipSuccess) return device_error(e, "cbm_delta_spmm");
-    h->last_launches = 1;
-    return CBM_OK;
+    h->last_launches = 0;
+    return forward_levels(h, forward_pass(h, dtype, x, ldx, y, ldy, n_cols), false, stream, "cbm_delta_spmm");
 }
 
 extern "C" int cbm_propagate(cbm_handle *h, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream) {

@@ -122,6 +122,82 @@ __device__ inline float signed_scale(float scale, int32_t packed) {
     return __uint_as_float(__float_as_uint(scale) ^ (uint32_t(packed) & kSign));
 }
 
+// sum over k in [kb, ke) of sign_k * scale_k * g[j_k, c0 .. c0+VE), accumulated
+// into acc by the LPR lanes of one group (all lanes of the group call it with
+// the same kb, ke; lanes whose columns are past the end pass active = false
+// but still take part in the index broadcasts).
+template <typename Tg, int VE, int LPR>
+__device__ inline void gather_sum(const RowPass &a, int32_t kb, int32_t ke, int l, int32_t c0, bool active,
+                                  float (&acc)[VE]) {
+    const Tg *__restrict__ g = static_cast<const Tg *>(a.g);
+    for (int32_t base = kb; base < ke; base += LPR) {
+        int32_t mine = 0;
+        float mine_scale = 0.0f;
+        if (base + l < ke) {
+            mine = a.idx[base + l];
+            float cs = a.colscale ? a.colscale[mine & kIndexMask] : 1.0f;
+            if (a.val) cs *= a.val[base + l];
+            mine_scale = signed_scale(cs, mine);
+        }
+        const int32_t cnt = min(LPR, ke - base);
+        int32_t k = 0;
+        for (; k + 4 <= cnt; k += 4) {
+            int32_t j[4];
+            float sc[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                j[u] = group_bcast<LPR>(mine, k + u) & kIndexMask;
+                sc[u] = group_bcast<LPR>(mine_scale, k + u);
+            }
+            if (active) {
+                float v[4][VE];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) Load<Tg, VE>::run(g + int64_t(j[u]) * a.ldg + c0, v[u]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[u][q], sc[u], acc[q]);
+            }
+        }
+        for (; k < cnt; ++k) {
+            const int32_t j = group_bcast<LPR>(mine, k) & kIndexMask;
+            const float sc = group_bcast<LPR>(mine_scale, k);
+            if (active) {
+                float v[VE];
+                Load<Tg, VE>::run(g + int64_t(j) * a.ldg + c0, v);
+#pragma unroll
+                for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], sc, acc[q]);
+            }
+        }
+    }
+}
+
+// Row scale, the extra term, and the store of one finished row segment.
+template <typename Te, int VE, int EXTRA>
+__device__ inline void finish_row(const RowPass &a, int32_t p, int32_t row, int32_t c0, float (&acc)[VE]) {
+    const float rs = a.rowscale ? a.rowscale[row] : 1.0f;
+#pragma unroll
+    for (int q = 0; q < VE; ++q) acc[q] *= rs;
+    if constexpr (EXTRA == kExtraParent) {
+        const int32_t par = a.parent_of[p];
+        if (par >= 0) {
+            const float es = (a.escale ? a.escale[p] : 1.0f) * a.ecoef;
+            float v[VE];
+            Load<Te, VE>::run(static_cast<const Te *>(a.e) + int64_t(par) * a.lde + c0, v);
+#pragma unroll
+            for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], es, acc[q]);
+        }
+    } else if constexpr (EXTRA == kExtraSelf) {
+        const float es = (a.escale ? a.escale[row] : 1.0f) * a.ecoef;
+        float v[VE];
+        Load<Te, VE>::run(static_cast<const Te *>(a.e) + int64_t(row) * a.lde + c0, v);
+#pragma unroll
+        for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], es, acc[q]);
+    }
+    store_row<VE>(a.y + int64_t(row) * a.ldy + c0, acc);
+}
+
+// Short rows: one group of LPR lanes per row, 64 / LPR rows per wavefront.
 template <typename Tg, typename Te, int VE, int LPR, int EXTRA>
 __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
     constexpr int kGroupsPerWave = 64 / LPR;
@@ -134,78 +210,87 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
     const int32_t p = int32_t(p64);
     const int32_t row = a.node_of ? a.node_of[p] : p;
     const int32_t kb = a.ptr[p], ke = a.ptr_end ? a.ptr_end[p] : a.ptr[p + 1];
-    const Tg *__restrict__ g = static_cast<const Tg *>(a.g);
-    const float rs = a.rowscale ? a.rowscale[p] : 1.0f;
     constexpr int32_t kSpan = LPR * VE;
     const int32_t passes = (a.ncols + kSpan - 1) / kSpan;
-
     for (int32_t pass = 0; pass < passes; ++pass) {
         const int32_t c0 = pass * kSpan + l * VE;
         const bool active = c0 < a.ncols;
         float acc[VE];
 #pragma unroll
         for (int q = 0; q < VE; ++q) acc[q] = 0.0f;
+        gather_sum<Tg, VE, LPR>(a, kb, ke, l, c0, active, acc);
+        if (active) finish_row<Te, VE, EXTRA>(a, p, row, c0, acc);
+    }
+}
 
-        for (int32_t base = kb; base < ke; base += LPR) {
-            int32_t mine = 0;
-            float mine_scale = 0.0f;
-            if (base + l < ke) {
-                mine = a.idx[base + l];
-                float cs = a.colscale ? a.colscale[mine & kIndexMask] : 1.0f;
-                if (a.val) cs *= a.val[base + l];
-                mine_scale = signed_scale(cs, mine);
-            }
-            const int32_t cnt = min(LPR, ke - base);
-            int32_t k = 0;
-            for (; k + 4 <= cnt; k += 4) {
-                int32_t j[4];
-                float sc[4];
+// Long rows: one workgroup of BS threads per chunk of a row.  The chunk's
+// entries are split into contiguous slices, one per group of LPR lanes; the
+// groups' partial sums are added in group order through LDS.  A one-chunk
+// row is finished here; otherwise the chunk's sum goes to its scratch slot.
+template <typename Tg, typename Te, int VE, int LPR, int EXTRA, int BS>
+__global__ __launch_bounds__(BS) void k_row_chunk(const RowPass a, const ChunkArgs c) {
+    constexpr int kGroups = BS / LPR;
+    __shared__ float part[BS * VE];
+    const int32_t chunk = c.first + int32_t(blockIdx.x);
+    const int32_t p = c.pos[chunk];
+    const uint32_t meta = uint32_t(c.meta[chunk]);
+    const int64_t idx = meta & 0xFFFFu, cnt = meta >> 16;
+    const int32_t kb = a.ptr[p], ke = a.ptr_end ? a.ptr_end[p] : a.ptr[p + 1];
+    const int64_t len = int64_t(ke) - kb;
+    const int64_t ck0 = kb + len * idx / cnt, ck1 = kb + len * (idx + 1) / cnt;
+    const int g = int(threadIdx.x) / LPR, l = int(threadIdx.x) % LPR;
+    const int32_t gb = int32_t(ck0 + (ck1 - ck0) * g / kGroups);
+    const int32_t ge = int32_t(ck0 + (ck1 - ck0) * (g + 1) / kGroups);
+    const int32_t row = a.node_of ? a.node_of[p] : p;
+    const int32_t slot = c.slot[chunk];
+    constexpr int32_t kSpan = LPR * VE;
+    const int32_t passes = (a.ncols + kSpan - 1) / kSpan;
+    for (int32_t pass = 0; pass < passes; ++pass) {
+        const int32_t c0 = pass * kSpan + l * VE;
+        const bool active = c0 < a.ncols;
+        float acc[VE];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    j[u] = group_bcast<LPR>(mine, k + u) & kIndexMask;
-                    sc[u] = group_bcast<LPR>(mine_scale, k + u);
-                }
-                if (active) {
-                    float v[4][VE];
+        for (int q = 0; q < VE; ++q) acc[q] = 0.0f;
+        gather_sum<Tg, VE, LPR>(a, gb, ge, l, c0, active, acc);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) Load<Tg, VE>::run(g + int64_t(j[u]) * a.ldg + c0, v[u]);
+        for (int q = 0; q < VE; ++q) part[threadIdx.x * VE + q] = acc[q];
+        __syncthreads();
+        if (g == 0 && active) {
+            for (int gg = 1; gg < kGroups; ++gg)
 #pragma unroll
-                    for (int u = 0; u < 4; ++u)
-#pragma unroll
-                        for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[u][q], sc[u], acc[q]);
-                }
-            }
-            for (; k < cnt; ++k) {
-                const int32_t j = group_bcast<LPR>(mine, k) & kIndexMask;
-                const float sc = group_bcast<LPR>(mine_scale, k);
-                if (active) {
-                    float v[VE];
-                    Load<Tg, VE>::run(g + int64_t(j) * a.ldg + c0, v);
-#pragma unroll
-                    for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], sc, acc[q]);
-                }
+                for (int q = 0; q < VE; ++q) acc[q] += part[(gg * LPR + l) * VE + q];
+            if (slot < 0) {
+                finish_row<Te, VE, EXTRA>(a, p, row, c0, acc);
+            } else {
+                store_row<VE>(c.scratch + int64_t(slot) * a.ncols + c0, acc);
             }
         }
-        if (!active) continue;
-#pragma unroll
-        for (int q = 0; q < VE; ++q) acc[q] *= rs;
-        if constexpr (EXTRA == kExtraParent) {
-            const int32_t par = a.parent_of[p];
-            if (par >= 0) {
-                const float es = (a.escale ? a.escale[p] : 1.0f) * a.ecoef;
-                float v[VE];
-                Load<Te, VE>::run(static_cast<const Te *>(a.e) + int64_t(par) * a.lde + c0, v);
-#pragma unroll
-                for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], es, acc[q]);
-            }
-        } else if constexpr (EXTRA == kExtraSelf) {
-            const float es = (a.escale ? a.escale[row] : 1.0f) * a.ecoef;
+        __syncthreads();
+    }
+}
+
+// Rows of several chunks: add the chunk sums in chunk order and finish.
+template <typename Te, int VE, int LPR, int EXTRA>
+__global__ __launch_bounds__(256) void k_row_reduce(const RowPass a, const ReduceArgs r) {
+    constexpr int kGroupsPerWave = 64 / LPR;
+    const int lane = int(threadIdx.x & 63u);
+    const int l = lane & (LPR - 1);
+    const int64_t i64 =
+        (int64_t(blockIdx.x) * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kGroupsPerWave + lane / LPR;
+    if (i64 >= int64_t(r.count)) return;
+    const int32_t i = r.first + int32_t(i64);
+    const int32_t p = r.pos[i], s0 = r.slot[i], ns = r.cnt[i];
+    const int32_t row = a.node_of ? a.node_of[p] : p;
+    for (int32_t c0 = l * VE; c0 < a.ncols; c0 += LPR * VE) {
+        float acc[VE];
+        Load<float, VE>::run(r.scratch + int64_t(s0) * a.ncols + c0, acc);
+        for (int32_t s = 1; s < ns; ++s) {
             float v[VE];
-            Load<Te, VE>::run(static_cast<const Te *>(a.e) + int64_t(row) * a.lde + c0, v);
+            Load<float, VE>::run(r.scratch + int64_t(s0 + s) * a.ncols + c0, v);
 #pragma unroll
-            for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], es, acc[q]);
+            for (int q = 0; q < VE; ++q) acc[q] += v[q];
         }
-        store_row<VE>(a.y + int64_t(row) * a.ldy + c0, acc);
+        finish_row<Te, VE, EXTRA>(a, p, row, c0, acc);
     }
 }
 
@@ -267,12 +352,11 @@ __global__ __launch_bounds__(256) void k_axpy(int32_t n_rows, int32_t ncols, flo
 
 __global__ __launch_bounds__(256) void k_dad_scales(int32_t n, const int32_t *__restrict__ node_of,
                                                     const int32_t *__restrict__ parent_of, const float *__restrict__ d,
-                                                    float *__restrict__ rs, float *__restrict__ ps) {
+                                                    float *__restrict__ ps) {
     const int32_t p = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
     if (p >= n) return;
     const float dv = d[node_of[p]];
     const int32_t par = parent_of[p];
-    rs[p] = dv;
     const float dp = par >= 0 ? d[par] : 0.0f;
     ps[p] = dp != 0.0f ? dv / dp : 0.0f;
 }
@@ -288,57 +372,90 @@ bool aligned16(const void *p, int64_t ld, int elem) {
     return (reinterpret_cast<uintptr_t>(p) & 15u) == 0 && ((ld * elem) & 15) == 0;
 }
 
-template <typename Tg, typename Te, int VE, int EXTRA>
-hipError_t dispatch_lpr(const RowPass &rp, int lpr, hipStream_t stream) {
-    const int64_t n = int64_t(rp.p_end) - rp.p_begin;
-    if (n <= 0) return hipSuccess;
-    const int64_t rows_per_block = 4 * (64 / lpr);
-    const dim3 grid(unsigned((n + rows_per_block - 1) / rows_per_block)), block(256);
-    switch (lpr) {
-#define CBM_LPR_CASE(L) \
-    case L: k_row_pass<Tg, Te, VE, L, EXTRA><<<grid, block, 0, stream>>>(rp); break;
-        CBM_LPR_CASE(1)
-        CBM_LPR_CASE(2)
-        CBM_LPR_CASE(4)
-        CBM_LPR_CASE(8)
-        CBM_LPR_CASE(16)
-        CBM_LPR_CASE(32)
-        CBM_LPR_CASE(64)
-#undef CBM_LPR_CASE
-        default:
-            return hipErrorInvalidValue;
+enum class Kind { kPass, kChunk, kReduce };
+
+struct Launch {
+    Kind kind;
+    const ChunkArgs *chunk;
+    const ReduceArgs *reduce;
+};
+
+template <typename Tg, typename Te, int VE, int EXTRA, int LPR>
+hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
+    constexpr int64_t kRowsPerBlock = 4 * (64 / LPR);
+    if (lk.kind == Kind::kPass) {
+        const int64_t n = int64_t(rp.p_end) - rp.p_begin;
+        if (n <= 0) return hipSuccess;
+        k_row_pass<Tg, Te, VE, LPR, EXTRA>
+            <<<dim3(unsigned((n + kRowsPerBlock - 1) / kRowsPerBlock)), dim3(256), 0, stream>>>(rp);
+    } else if (lk.kind == Kind::kChunk) {
+        if (lk.chunk->count <= 0) return hipSuccess;
+        constexpr int kBs = (16 * LPR > 256) ? 16 * LPR : 256;
+        k_row_chunk<Tg, Te, VE, LPR, EXTRA, kBs>
+            <<<dim3(unsigned(lk.chunk->count)), dim3(kBs), 0, stream>>>(rp, *lk.chunk);
+    } else {
+        const int64_t n = lk.reduce->count;
+        if (n <= 0) return hipSuccess;
+        k_row_reduce<Te, VE, LPR, EXTRA>
+            <<<dim3(unsigned((n + kRowsPerBlock - 1) / kRowsPerBlock)), dim3(256), 0, stream>>>(rp, *lk.reduce);
     }
     return hipGetLastError();
 }
 
+template <typename Tg, typename Te, int VE, int EXTRA>
+hipError_t dispatch_lpr(const RowPass &rp, const Launch &lk, int lpr, hipStream_t stream) {
+    switch (lpr) {
+        case 1: return launch_one<Tg, Te, VE, EXTRA, 1>(rp, lk, stream);
+        case 2: return launch_one<Tg, Te, VE, EXTRA, 2>(rp, lk, stream);
+        case 4: return launch_one<Tg, Te, VE, EXTRA, 4>(rp, lk, stream);
+        case 8: return launch_one<Tg, Te, VE, EXTRA, 8>(rp, lk, stream);
+        case 16: return launch_one<Tg, Te, VE, EXTRA, 16>(rp, lk, stream);
+        case 32: return launch_one<Tg, Te, VE, EXTRA, 32>(rp, lk, stream);
+        case 64: return launch_one<Tg, Te, VE, EXTRA, 64>(rp, lk, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 template <typename Tg, typename Te, int EXTRA>
-hipError_t dispatch_ve(const RowPass &rp, hipStream_t stream) {
+hipError_t dispatch_ve(const RowPass &rp, const Launch &lk, hipStream_t stream) {
     constexpr int kVe = 16 / int(sizeof(Tg));
     bool vec = (rp.ncols % kVe) == 0 && aligned16(rp.g, rp.ldg, int(sizeof(Tg))) && aligned16(rp.y, rp.ldy, 4);
     if (EXTRA != kExtraNone) vec = vec && aligned16(rp.e, rp.lde, int(sizeof(Te)));
-    if (vec) return dispatch_lpr<Tg, Te, kVe, EXTRA>(rp, lanes_per_row(rp.ncols, kVe), stream);
-    return dispatch_lpr<Tg, Te, 1, EXTRA>(rp, lanes_per_row(rp.ncols, 1), stream);
+    if (vec) return dispatch_lpr<Tg, Te, kVe, EXTRA>(rp, lk, lanes_per_row(rp.ncols, kVe), stream);
+    return dispatch_lpr<Tg, Te, 1, EXTRA>(rp, lk, lanes_per_row(rp.ncols, 1), stream);
+}
+
+hipError_t dispatch(const RowPass &rp, const Launch &lk, hipStream_t stream) {
+    if (!rp.g_bf16 && (rp.extra == kExtraNone || !rp.e_bf16)) {
+        switch (rp.extra) {
+            case kExtraNone: return dispatch_ve<float, float, kExtraNone>(rp, lk, stream);
+            case kExtraParent: return dispatch_ve<float, float, kExtraParent>(rp, lk, stream);
+            case kExtraSelf: return dispatch_ve<float, float, kExtraSelf>(rp, lk, stream);
+        }
+    } else if (rp.g_bf16 && (rp.extra == kExtraNone || !rp.e_bf16)) {
+        switch (rp.extra) {
+            case kExtraNone: return dispatch_ve<bf16_t, float, kExtraNone>(rp, lk, stream);
+            case kExtraParent: return dispatch_ve<bf16_t, float, kExtraParent>(rp, lk, stream);
+            default: break;
+        }
+    } else if (!rp.g_bf16 && rp.e_bf16 && rp.extra == kExtraSelf) {
+        return dispatch_ve<float, bf16_t, kExtraSelf>(rp, lk, stream);
+    }
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
 
 hipError_t launch_row_pass(const RowPass &rp, hipStream_t stream) {
-    if (!rp.g_bf16 && (rp.extra == kExtraNone || !rp.e_bf16)) {
-        switch (rp.extra) {
-            case kExtraNone: return dispatch_ve<float, float, kExtraNone>(rp, stream);
-            case kExtraParent: return dispatch_ve<float, float, kExtraParent>(rp, stream);
-            case kExtraSelf: return dispatch_ve<float, float, kExtraSelf>(rp, stream);
-        }
-    } else if (rp.g_bf16 && (rp.extra == kExtraNone || !rp.e_bf16)) {
-        switch (rp.extra) {
-            case kExtraNone: return dispatch_ve<bf16_t, float, kExtraNone>(rp, stream);
-            case kExtraParent: return dispatch_ve<bf16_t, float, kExtraParent>(rp, stream);
-            default: break;
-        }
-    } else if (!rp.g_bf16 && rp.e_bf16 && rp.extra == kExtraSelf) {
-        return dispatch_ve<float, bf16_t, kExtraSelf>(rp, stream);
-    }
-    return hipErrorInvalidValue;
+    return dispatch(rp, Launch{Kind::kPass, nullptr, nullptr}, stream);
+}
+
+hipError_t launch_row_chunks(const RowPass &rp, const ChunkArgs &c, hipStream_t stream) {
+    return dispatch(rp, Launch{Kind::kChunk, &c, nullptr}, stream);
+}
+
+hipError_t launch_row_reduce(const RowPass &rp, const ReduceArgs &r, hipStream_t stream) {
+    return dispatch(rp, Launch{Kind::kReduce, nullptr, &r}, stream);
 }
 
 hipError_t launch_propagate(int32_t p_begin, int32_t p_end, const int32_t *node_of, const int32_t *parent_of,
@@ -381,10 +498,10 @@ hipError_t launch_axpy(int32_t n_rows, int32_t ncols, float coef, const float *x
     return hipGetLastError();
 }
 
-hipError_t launch_dad_scales(int32_t n, const int32_t *node_of, const int32_t *parent_of, const float *d, float *rs,
-                             float *ps, hipStream_t stream) {
+hipError_t launch_dad_scales(int32_t n, const int32_t *node_of, const int32_t *parent_of, const float *d, float *ps,
+                             hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    k_dad_scales<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream>>>(n, node_of, parent_of, d, rs, ps);
+    k_dad_scales<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream>>>(n, node_of, parent_of, d, ps);
     return hipGetLastError();
 }
 

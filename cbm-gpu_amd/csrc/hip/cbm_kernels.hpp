@@ -16,9 +16,9 @@ enum Extra : int32_t {
 
 // One row pass: for every position p in [p_begin, p_end), with row = node_of[p]
 // (or p when node_of is null),
-//   sum = sum_k  sign_k * colscale[j_k] * g[j_k, :]   over k in [ptr[p], ptr[p+1]),
-// where idx[k] = j_k | sign_k << 31, then the extra term per `extra`, and the
-// result is stored to y[row, :] in FP32.
+//   sum = sum_k  sign_k * colscale[j_k] * val_k * g[j_k, :]   over k in [ptr[p], ptr[p+1]),
+// where idx[k] = j_k | sign_k << 31; y[row, :] = rowscale[row] * sum + the
+// extra term per `extra`, stored in FP32.
 struct RowPass {
     int32_t p_begin = 0, p_end = 0;
     const int32_t *node_of = nullptr;
@@ -28,7 +28,7 @@ struct RowPass {
     const int32_t *idx = nullptr;
     const float *val = nullptr;       // per-entry values; null = +-1 from the sign bit
     const float *colscale = nullptr;  // indexed by gathered row; null = 1
-    const float *rowscale = nullptr;  // indexed by position;     null = 1
+    const float *rowscale = nullptr;  // indexed by output row;   null = 1
     const float *escale = nullptr;    // by position (parent) or by row (self); null = 1
     const void *g = nullptr;          // gathered matrix
     int64_t ldg = 0;
@@ -43,8 +43,31 @@ struct RowPass {
     int32_t ncols = 0;
 };
 
-// Launches one row pass; returns the launch status.
+// Launches one row pass over positions [p_begin, p_end); returns the launch status.
 hipError_t launch_row_pass(const RowPass &rp, hipStream_t stream);
+
+// Chunks [first, first + count) of the long rows of a row pass: chunk i covers
+// part meta & 0xffff of meta >> 16 equal parts of row pos[i]'s entries; its sum
+// finishes the row (slot[i] < 0) or goes to scratch row slot[i] (ld = ncols).
+struct ChunkArgs {
+    const int32_t *pos = nullptr;
+    const int32_t *meta = nullptr;
+    const int32_t *slot = nullptr;
+    int32_t first = 0, count = 0;
+    float *scratch = nullptr;
+};
+hipError_t launch_row_chunks(const RowPass &rp, const ChunkArgs &c, hipStream_t stream);
+
+// Rows [first, first + count) of a reduction list: row pos[i] is the sum of
+// scratch rows slot[i] .. slot[i] + cnt[i] - 1, added in order, then finished.
+struct ReduceArgs {
+    const int32_t *pos = nullptr;
+    const int32_t *slot = nullptr;
+    const int32_t *cnt = nullptr;
+    int32_t first = 0, count = 0;
+    const float *scratch = nullptr;
+};
+hipError_t launch_row_reduce(const RowPass &rp, const ReduceArgs &r, hipStream_t stream);
 
 // In-place propagation of one level: y[node_of[p]] += parscale[p] * y[parent_of[p]].
 hipError_t launch_propagate(int32_t p_begin, int32_t p_end, const int32_t *node_of, const int32_t *parent_of,
@@ -61,9 +84,9 @@ hipError_t launch_branch_update(int32_t n_branches, const int32_t *branches, con
 hipError_t launch_axpy(int32_t n_rows, int32_t ncols, float coef, const float *x, int64_t ldx, float *y, int64_t ldy,
                        hipStream_t stream);
 
-// Per-position scales of the D A D product: rs[p] = d[node[p]],
+// Per-position propagation factor of the D A D product:
 // ps[p] = d[node[p]] / d[parent[p]] (0 for roots and zero-degree parents).
-hipError_t launch_dad_scales(int32_t n, const int32_t *node_of, const int32_t *parent_of, const float *d, float *rs,
-                             float *ps, hipStream_t stream);
+hipError_t launch_dad_scales(int32_t n, const int32_t *node_of, const int32_t *parent_of, const float *d, float *ps,
+                             hipStream_t stream);
 
 }  // namespace cbmk

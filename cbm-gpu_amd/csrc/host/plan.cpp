@@ -22,12 +22,47 @@
 
 struct cbm_plan {
     std::vector<int32_t> arr[CBM_P_N_ARRAYS];
+    int64_t stat[CBM_PS_N_STATS] = {};
     int32_t depth = 0;
 };
 
 using namespace cbmh;
 
 static constexpr uint32_t kSignBit = 0x80000000u;
+
+namespace {
+
+// Chunk and reduction tables of the long rows [b, b + n_long) of an ordered
+// row list with offsets ptr.  Slots are numbered from 0 for every call, so
+// ranges processed one after another (tree levels) share the scratch rows.
+// Returns the number of slots used.
+struct Split {
+    std::vector<int32_t> ck_pos, ck_meta, ck_slot, red_pos, red_slot, red_cnt;
+};
+
+int32_t split_long_rows(const std::vector<int32_t> &ptr, int32_t b, int32_t n_long, Split &s) {
+    int32_t slots = 0;
+    for (int32_t p = b; p < b + n_long; ++p) {
+        const int64_t len = int64_t(ptr[size_t(p) + 1]) - ptr[size_t(p)];
+        int64_t cnt = (len + CBM_CHUNK - 1) / CBM_CHUNK;
+        if (cnt > 0xFFFF) cnt = 0xFFFF;  // chunks grow instead
+        if (cnt < 1) cnt = 1;
+        for (int64_t c = 0; c < cnt; ++c) {
+            s.ck_pos.push_back(p);
+            s.ck_meta.push_back(int32_t(uint32_t(c) | (uint32_t(cnt) << 16)));
+            s.ck_slot.push_back(cnt > 1 ? slots + int32_t(c) : -1);
+        }
+        if (cnt > 1) {
+            s.red_pos.push_back(p);
+            s.red_slot.push_back(slots);
+            s.red_cnt.push_back(int32_t(cnt));
+            slots += int32_t(cnt);
+        }
+    }
+    return slots;
+}
+
+}  // namespace
 
 extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *col, const float *val, int64_t nnz,
                                const int32_t *tree_src, const int32_t *tree_dst, int64_t n_edges, int32_t n_threads,
@@ -103,19 +138,28 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
         int32_t levels = 0;
         for (int32_t v = 0; v < n; ++v) levels = std::max(levels, depth[size_t(v)] + 1);
         plan->depth = levels;
+        plan->stat[CBM_PS_DEPTH] = levels;
 
-        // Level order by counting sort on depth (stable in node id).
+        // Level order: counting sort on (depth, short?) -- long rows first in
+        // every level, node ids ascending within each part.
         auto &lvl_ptr = plan->arr[CBM_P_LVL_PTR];
         auto &lvl_node = plan->arr[CBM_P_LVL_NODE];
         auto &lvl_parent = plan->arr[CBM_P_LVL_PARENT];
-        lvl_ptr.assign(size_t(levels) + 1, 0);
-        for (int32_t v = 0; v < n; ++v) ++lvl_ptr[size_t(depth[size_t(v)]) + 1];
-        std::partial_sum(lvl_ptr.begin(), lvl_ptr.end(), lvl_ptr.begin());
+        auto &lvl_nlong = plan->arr[CBM_P_LVL_NLONG];
+        auto is_long = [&](int32_t v) { return crow[v + 1] - crow[v] > CBM_LONG_ROW; };
+        std::vector<int32_t> key_ptr(size_t(2 * levels) + 1, 0);
+        for (int32_t v = 0; v < n; ++v) ++key_ptr[size_t(2 * depth[size_t(v)] + (is_long(v) ? 0 : 1)) + 1];
+        std::partial_sum(key_ptr.begin(), key_ptr.end(), key_ptr.begin());
+        lvl_ptr.resize(size_t(levels) + 1);
+        lvl_nlong.resize(size_t(levels));
+        for (int32_t l = 0; l <= levels; ++l) lvl_ptr[size_t(l)] = key_ptr[size_t(2 * l)];
+        for (int32_t l = 0; l < levels; ++l) lvl_nlong[size_t(l)] = key_ptr[size_t(2 * l + 1)] - key_ptr[size_t(2 * l)];
         lvl_node.resize(size_t(n));
         lvl_parent.resize(size_t(n));
         {
-            std::vector<int32_t> fill(lvl_ptr.begin(), lvl_ptr.end() - 1);
-            for (int32_t v = 0; v < n; ++v) lvl_node[size_t(fill[size_t(depth[size_t(v)])]++)] = v;
+            std::vector<int32_t> fill(key_ptr.begin(), key_ptr.end() - 1);
+            for (int32_t v = 0; v < n; ++v)
+                lvl_node[size_t(fill[size_t(2 * depth[size_t(v)] + (is_long(v) ? 0 : 1))]++)] = v;
         }
         for (int32_t p = 0; p < n; ++p) lvl_parent[size_t(p)] = parent[size_t(lvl_node[size_t(p)])];
 
@@ -145,18 +189,62 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
             return CBM_ERR_VALUE;
         }
 
-        // Backward: CSC of Delta (rows ascending within each column).
-        auto &bptr = plan->arr[CBM_P_BWD_PTR];
-        auto &brow = plan->arr[CBM_P_BWD_ROW];
-        bptr.assign(size_t(n) + 1, 0);
-        for (int64_t k = 0; k < nnz; ++k) ++bptr[size_t(col[k]) + 1];
-        std::partial_sum(bptr.begin(), bptr.end(), bptr.begin());
-        brow.resize(size_t(nnz));
+        // Forward chunk tables, one set per level.
         {
+            Split f;
+            auto &fck_ptr = plan->arr[CBM_P_FCK_PTR];
+            auto &fred_ptr = plan->arr[CBM_P_FRED_PTR];
+            fck_ptr.assign(size_t(levels) + 1, 0);
+            fred_ptr.assign(size_t(levels) + 1, 0);
+            int32_t max_slots = 0;
+            for (int32_t l = 0; l < levels; ++l) {
+                max_slots = std::max(max_slots, split_long_rows(fptr, lvl_ptr[size_t(l)], lvl_nlong[size_t(l)], f));
+                fck_ptr[size_t(l) + 1] = int32_t(f.ck_pos.size());
+                fred_ptr[size_t(l) + 1] = int32_t(f.red_pos.size());
+            }
+            plan->arr[CBM_P_FCK_POS].swap(f.ck_pos);
+            plan->arr[CBM_P_FCK_META].swap(f.ck_meta);
+            plan->arr[CBM_P_FCK_SLOT].swap(f.ck_slot);
+            plan->arr[CBM_P_FRED_POS].swap(f.red_pos);
+            plan->arr[CBM_P_FRED_SLOT].swap(f.red_slot);
+            plan->arr[CBM_P_FRED_CNT].swap(f.red_cnt);
+            plan->stat[CBM_PS_FWD_SLOTS] = max_slots;
+        }
+
+        // Backward: Delta^T (the CSC of Delta), columns ordered long first,
+        // then short, ascending within each part; rows ascending per column.
+        {
+            std::vector<int32_t> ccount(size_t(n), 0);
+            for (int64_t k = 0; k < nnz; ++k) ++ccount[size_t(col[k])];
+            auto &bnode = plan->arr[CBM_P_BWD_NODE];
+            bnode.resize(size_t(n));
+            int32_t n_long = 0;
+            for (int32_t j = 0; j < n; ++j)
+                if (ccount[size_t(j)] > CBM_LONG_ROW) bnode[size_t(n_long++)] = j;
+            int32_t q = n_long;
+            for (int32_t j = 0; j < n; ++j)
+                if (ccount[size_t(j)] <= CBM_LONG_ROW) bnode[size_t(q++)] = j;
+            std::vector<int32_t> pos_of(static_cast<size_t>(n));
+            for (int32_t p = 0; p < n; ++p) pos_of[size_t(bnode[size_t(p)])] = p;
+            auto &bptr = plan->arr[CBM_P_BWD_PTR];
+            auto &brow = plan->arr[CBM_P_BWD_ROW];
+            bptr.assign(size_t(n) + 1, 0);
+            for (int32_t p = 0; p < n; ++p) bptr[size_t(p) + 1] = bptr[size_t(p)] + ccount[size_t(bnode[size_t(p)])];
+            brow.resize(size_t(nnz));
             std::vector<int32_t> fill(bptr.begin(), bptr.end() - 1);
             for (int32_t r = 0; r < n; ++r)
                 for (int32_t k = crow[r]; k < crow[r + 1]; ++k)
-                    brow[size_t(fill[size_t(col[k])]++)] = int32_t(uint32_t(r) | (val[k] < 0.0f ? kSignBit : 0u));
+                    brow[size_t(fill[size_t(pos_of[size_t(col[k])])]++)] =
+                        int32_t(uint32_t(r) | (val[k] < 0.0f ? kSignBit : 0u));
+            Split b;
+            plan->stat[CBM_PS_BWD_SLOTS] = split_long_rows(bptr, 0, n_long, b);
+            plan->stat[CBM_PS_BWD_NLONG] = n_long;
+            plan->arr[CBM_P_BCK_POS].swap(b.ck_pos);
+            plan->arr[CBM_P_BCK_META].swap(b.ck_meta);
+            plan->arr[CBM_P_BCK_SLOT].swap(b.ck_slot);
+            plan->arr[CBM_P_BRED_POS].swap(b.red_pos);
+            plan->arr[CBM_P_BRED_SLOT].swap(b.red_slot);
+            plan->arr[CBM_P_BRED_CNT].swap(b.red_cnt);
         }
 
         // Children per level position.
@@ -192,5 +280,10 @@ extern "C" const int32_t *cbm_plan_array(const cbm_plan *p, int32_t which) {
 }
 
 extern "C" int32_t cbm_plan_depth(const cbm_plan *p) { return p ? p->depth : -1; }
+
+extern "C" int64_t cbm_plan_stat(const cbm_plan *p, int32_t which) {
+    if (!p || which < 0 || which >= CBM_PS_N_STATS) return -1;
+    return p->stat[which];
+}
 
 extern "C" void cbm_plan_free(cbm_plan *p) { delete p; }

@@ -10,7 +10,8 @@
  *
  * with A = (I - P)^-1 Delta.  X and G are row-major N x F in FP32 or BF16;
  * Y, dX and all accumulation are FP32.  Every call is asynchronous on the
- * given stream, allocates nothing and may be captured in a HIP graph.
+ * given stream; once the handle's scratch is reserved (cbm_reserve) a call
+ * allocates nothing and may be captured in a HIP graph.
  * Calls return CBM_OK (0) or a negative status; cbm_last_error() describes
  * the failure for the calling thread.
  *
@@ -50,10 +51,10 @@ enum cbm_mode {
 enum cbm_dtype { CBM_F32 = 0, CBM_BF16 = 1 };
 
 /*
- * Device layout of one CBM matrix.  All device pointers are borrowed and must
- * stay valid until cbm_destroy.  Arrays are those of cbm_plan_create:
- *   lvl_node[N], lvl_parent[N], fwd_ptr[N+1], fwd_col[nnz], bwd_ptr[N+1],
- *   bwd_row[nnz], ch_ptr[N+1], ch_node[E]; lvl_ptr is the HOST copy [L+1].
+ * Device layout of one CBM matrix.  All pointers are borrowed and must stay
+ * valid until cbm_destroy.  Arrays are those of cbm_plan_create
+ * (include/cbm_host.h); the per-level tables marked "host" are host copies,
+ * everything else lives in device memory.
  * d is the device vector D^-1/2 [N] (0 for nodes of degree 0) for ADX/DADX,
  * NULL for AX.
  */
@@ -63,15 +64,41 @@ typedef struct cbm_desc {
     int64_t nnz;
     int64_t n_edges;
     int32_t mode;
+    /* host: level offsets [L+1], long rows per level [L], chunk offsets [L+1],
+       reduction-row offsets [L+1] */
     const int32_t *lvl_ptr_host;
+    const int32_t *lvl_nlong_host;
+    const int32_t *fck_ptr_host;
+    const int32_t *fred_ptr_host;
+    /* device: level order, forward Delta, children */
     const int32_t *lvl_node;
     const int32_t *lvl_parent;
     const int32_t *fwd_ptr;
     const int32_t *fwd_col;
-    const int32_t *bwd_ptr;
-    const int32_t *bwd_row;
     const int32_t *ch_ptr;
     const int32_t *ch_node;
+    /* device: forward long-row chunks and reductions */
+    const int32_t *fck_pos;
+    const int32_t *fck_meta;
+    const int32_t *fck_slot;
+    const int32_t *fred_pos;
+    const int32_t *fred_slot;
+    const int32_t *fred_cnt;
+    int32_t fwd_slots;
+    /* device: Delta^T in backward order, its long-row chunks and reductions */
+    const int32_t *bwd_node;
+    const int32_t *bwd_ptr;
+    const int32_t *bwd_row;
+    const int32_t *bck_pos;
+    const int32_t *bck_meta;
+    const int32_t *bck_slot;
+    const int32_t *bred_pos;
+    const int32_t *bred_slot;
+    const int32_t *bred_cnt;
+    int32_t bwd_nlong;
+    int32_t n_bck;
+    int32_t n_bred;
+    int32_t bwd_slots;
     const float *d;
 } cbm_desc;
 
@@ -79,6 +106,9 @@ typedef struct cbm_handle cbm_handle;
 
 /* Message describing the last failure on the calling thread ("" if none). */
 const char *cbm_last_error(void);
+
+/* sizeof(cbm_desc) as compiled into the library (a binding's layout check). */
+int32_t cbm_desc_size(void);
 
 /*
  * Creates a handle.  Derived per-position scale vectors (N floats each) are
@@ -89,10 +119,20 @@ int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle **out);
 int cbm_destroy(cbm_handle *h);
 
 /*
+ * Sizes the handle's scratch for products of up to n_cols columns: the
+ * partial sums of long rows, max(fwd_slots, bwd_slots) x n_cols FP32.  A
+ * product call with more columns than reserved grows the scratch itself
+ * (one hipMalloc); reserve first when the calls must not allocate (graph
+ * capture).
+ */
+int cbm_reserve(cbm_handle *h, int32_t n_cols);
+
+/*
  * Forward product Y = M X.  x: N x n_cols, leading dimension ldx (elements);
  * y: N x n_cols FP32, leading dimension ldy; x and y must not overlap.
- * Fused: each level launch computes Delta rows and adds the parent's
- * finished row.  Replaces cusparseCBM::matmul_AX_ADX / matmul_DAX_DADX.
+ * Fused: per tree level, Delta rows are summed and the parent's finished row
+ * added in the same pass (long rows by whole workgroups, in chunks).
+ * Replaces cusparseCBM::matmul_AX_ADX / matmul_DAX_DADX.
  */
 int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy, int32_t n_cols,
                hipStream_t stream);

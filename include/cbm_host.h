@@ -116,19 +116,54 @@ void cbm_build_free(cbm_build *b);
 /* Device plan: the layout the HIP kernels read, built on the host.           */
 /* ------------------------------------------------------------------------- */
 
-/* Arrays held by a cbm_plan, addressed by cbm_plan_array(). */
+/*
+ * Arrays held by a cbm_plan, addressed by cbm_plan_array().  Positions are
+ * indices into an ordered row list.  A row with more than CBM_LONG_ROW
+ * entries is "long": it is split into chunks of about CBM_CHUNK entries,
+ * each summed by one workgroup; a row of several chunks gets one partial
+ * slot per chunk, and a second pass adds its slots in order.
+ */
 enum {
-    CBM_P_LVL_PTR = 0,     /* int32 [L+1]  level offsets into the level order                     */
-    CBM_P_LVL_NODE = 1,    /* int32 [N]    node at each level position (level 0 = root children)   */
-    CBM_P_LVL_PARENT = 2,  /* int32 [N]    parent of the node at each position, -1 for level 0     */
-    CBM_P_FWD_PTR = 3,     /* int32 [N+1]  Delta row offsets, rows in level order                  */
-    CBM_P_FWD_COL = 4,     /* int32 [nnz]  Delta columns, bit 31 set for a -1 entry                 */
-    CBM_P_BWD_PTR = 5,     /* int32 [N+1]  Delta^T row offsets (= Delta CSC), natural order         */
-    CBM_P_BWD_ROW = 6,     /* int32 [nnz]  Delta rows per column, bit 31 set for a -1 entry         */
-    CBM_P_CH_PTR = 7,      /* int32 [N+1]  child offsets per level position                          */
-    CBM_P_CH_NODE = 8,     /* int32 [E]    children (node ids) of the node at each position          */
-    CBM_P_N_ARRAYS = 9
+    CBM_P_LVL_PTR = 0,     /* int32 [L+1]  level offsets into the level order                      */
+    CBM_P_LVL_NODE = 1,    /* int32 [N]    node at each level position (level 0 = root children);
+                                           within a level, long rows first, each part by node id     */
+    CBM_P_LVL_PARENT = 2,  /* int32 [N]    parent of the node at each position, -1 for level 0      */
+    CBM_P_FWD_PTR = 3,     /* int32 [N+1]  Delta row offsets, rows in level order                   */
+    CBM_P_FWD_COL = 4,     /* int32 [nnz]  Delta columns, bit 31 set for a -1 entry                  */
+    CBM_P_BWD_PTR = 5,     /* int32 [N+1]  Delta^T row offsets, rows in BWD_NODE order              */
+    CBM_P_BWD_ROW = 6,     /* int32 [nnz]  Delta rows per column, bit 31 set for a -1 entry          */
+    CBM_P_CH_PTR = 7,      /* int32 [N+1]  child offsets per level position                           */
+    CBM_P_CH_NODE = 8,     /* int32 [E]    children (node ids) of the node at each position           */
+    CBM_P_LVL_NLONG = 9,   /* int32 [L]    long rows at the start of each level                       */
+    CBM_P_FCK_PTR = 10,    /* int32 [L+1]  forward chunk offsets per level                            */
+    CBM_P_FCK_POS = 11,    /* int32 [C]    level position of each forward chunk                       */
+    CBM_P_FCK_META = 12,   /* int32 [C]    chunk index (bits 0-15) and chunk count (bits 16-31)      */
+    CBM_P_FCK_SLOT = 13,   /* int32 [C]    partial slot of the chunk, -1 for a one-chunk row          */
+    CBM_P_FRED_PTR = 14,   /* int32 [L+1]  offsets per level of the rows that need a reduction       */
+    CBM_P_FRED_POS = 15,   /* int32 [R]    their level positions                                      */
+    CBM_P_FRED_SLOT = 16,  /* int32 [R]    first partial slot of the row                              */
+    CBM_P_FRED_CNT = 17,   /* int32 [R]    number of slots                                            */
+    CBM_P_BWD_NODE = 18,   /* int32 [N]    column of Delta (row of dX) at each backward position; long first */
+    CBM_P_BCK_POS = 19,    /* int32 [Cb]   backward chunks: position                                  */
+    CBM_P_BCK_META = 20,   /* int32 [Cb]   chunk index / count                                        */
+    CBM_P_BCK_SLOT = 21,   /* int32 [Cb]   partial slot or -1                                         */
+    CBM_P_BRED_POS = 22,   /* int32 [Rb]   backward rows that need a reduction                        */
+    CBM_P_BRED_SLOT = 23,  /* int32 [Rb]   first slot                                                 */
+    CBM_P_BRED_CNT = 24,   /* int32 [Rb]   number of slots                                            */
+    CBM_P_N_ARRAYS = 25
 };
+
+/* Scalars of a cbm_plan, addressed by cbm_plan_stat(). */
+enum {
+    CBM_PS_DEPTH = 0,      /* L, number of levels                                   */
+    CBM_PS_FWD_SLOTS = 1,  /* partial slots needed by the largest forward level      */
+    CBM_PS_BWD_SLOTS = 2,  /* partial slots needed by the backward pass              */
+    CBM_PS_BWD_NLONG = 3,  /* long rows at the start of the backward order           */
+    CBM_PS_N_STATS = 4
+};
+
+#define CBM_LONG_ROW 256
+#define CBM_CHUNK 1024
 
 typedef struct cbm_plan cbm_plan;
 
@@ -144,6 +179,7 @@ int cbm_plan_create(int32_t n_rows, const int32_t *crow, const int32_t *col, con
                     int32_t n_threads, cbm_plan **out);
 int64_t cbm_plan_size(const cbm_plan *p, int32_t which);
 const int32_t *cbm_plan_array(const cbm_plan *p, int32_t which);
+int64_t cbm_plan_stat(const cbm_plan *p, int32_t which);
 int32_t cbm_plan_depth(const cbm_plan *p);
 void cbm_plan_free(cbm_plan *p);
 

@@ -32,8 +32,7 @@ def test_python_declarations_cover_header():
 
 
 def test_desc_layout_matches_c_struct():
-    # int32, int32, int64, int64, int32 (+pad), 10 pointers
-    assert ctypes.sizeof(_hip_sigs.CbmDesc) == 4 + 4 + 8 + 8 + 8 + 10 * 8
+    assert ctypes.sizeof(_hip_sigs.CbmDesc) == _lib.hip().cbm_desc_size()
 
 
 def test_error_strings_are_empty_initially():

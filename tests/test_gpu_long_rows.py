@@ -1,0 +1,74 @@
+"""Rows longer than one workgroup chunk: the chunked long-row kernels and
+their in-order reduction, forward and adjoint, on a dense graph with hubs
+(rows up to ~2.4k entries, several multi-chunk rows in both directions)."""
+import numpy as np
+import pytest
+import torch
+
+from cbm.cbm4ad import cbm4ad
+from cbm.cbm4dad import cbm4dad
+from cbm.cbm4mm import cbm4mm
+from hubgraph import HUB_N, hub_graph
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CLASSES = {"ax": cbm4mm, "adx": cbm4ad, "dadx": cbm4dad}
+
+
+@pytest.fixture(scope="module")
+def hub(cuda):
+    ei = hub_graph()
+    a = O.csr_of(ei.numpy(), HUB_N)
+    mats = {mode: cls(ei, torch.ones(ei.shape[1]), alpha=0, device=cuda) for mode, cls in CLASSES.items()}
+    st = mats["ax"].cusparse_extension.plan_stats
+    assert st["fwd_slots"] > 0 and st["bwd_slots"] > 0  # multi-chunk rows in both directions
+    return a, mats
+
+
+def _x(shape, seed):
+    return np.random.default_rng(seed).standard_normal(shape).astype(np.float32)
+
+
+@pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
+@pytest.mark.parametrize("f", [1, 64, 256, 300])
+def test_long_rows_fp32(hub, cuda, mode, f):
+    a, mats = hub
+    m = mats[mode]
+    x = _x((HUB_N, f), f)
+    y = torch.empty(HUB_N, f, device=cuda)
+    m.matmul(torch.from_numpy(x).to(cuda), y)
+    assert O.max_rel_error(y.cpu().numpy(), O.forward(a, x, mode)) <= 1e-5
+    dx = torch.empty(HUB_N, f, device=cuda)
+    m.matmul_t(torch.from_numpy(x).to(cuda), dx)
+    assert O.max_rel_error(dx.cpu().numpy(), O.adjoint(a, x, mode)) <= 1e-5
+
+
+@pytest.mark.parametrize("mode", ["ax", "dadx"])
+def test_long_rows_bf16(hub, cuda, mode):
+    a, mats = hub
+    m = mats[mode]
+    xb = torch.from_numpy(_x((HUB_N, 128), 5)).to(cuda).to(torch.bfloat16)
+    xe = xb.float().cpu().numpy()
+    y = torch.empty(HUB_N, 128, device=cuda)
+    m.matmul(xb, y)
+    assert O.max_rel_error(y.cpu().numpy(), O.forward(a, xe, mode)) <= 1e-5
+    dx = torch.empty(HUB_N, 128, device=cuda)
+    m.matmul_t(xb, dx)
+    assert O.max_rel_error(dx.cpu().numpy(), O.adjoint(a, xe, mode)) <= 1e-5
+
+
+def test_long_rows_integer_exact_and_deterministic(hub, cuda):
+    a, mats = hub
+    h = mats["ax"].cusparse_extension
+    x = np.random.default_rng(3).integers(-3, 4, size=(HUB_N, 32)).astype(np.float32)
+    xt = torch.from_numpy(x).to(cuda)
+    y1 = torch.empty(HUB_N, 32, device=cuda)
+    y2 = torch.empty(HUB_N, 32, device=cuda)
+    h.forward(xt, y1)
+    h.forward(xt, y2)
+    assert np.array_equal(y1.cpu().numpy().astype(np.float64), O.forward(a, x, "ax"))
+    assert torch.equal(y1, y2)
+    d1 = torch.empty(HUB_N, 32, device=cuda)
+    h.matmul_t(xt, d1)
+    assert np.array_equal(d1.cpu().numpy().astype(np.float64), O.adjoint(a, x, "ax"))

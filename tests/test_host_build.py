@@ -9,6 +9,7 @@ import scipy.sparse as sp
 import torch
 
 from cbm import cbm_mkl_cpp, structure
+from hubgraph import HUB_N, hub_graph
 from oracle import oracle as O
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "cbm_small.json")
@@ -169,13 +170,17 @@ def test_plan_invariants():
         got = fc[fp[p]:fp[p + 1]]
         assert np.array_equal(got & 0x7FFFFFFF, col[ptr[v]:ptr[v + 1]])
         assert np.array_equal((got >> 31).astype(bool), vals[ptr[v]:ptr[v + 1]] < 0)
-    # backward = CSC of Delta
+    # backward = CSC of Delta, columns in the backward order
     bp, br = pl.bwd_ptr.numpy(), pl.bwd_row.numpy().view(np.uint32)
+    bnode = pl.bwd_node.numpy()
+    assert sorted(bnode.tolist()) == list(range(n))
     delta = sp.csr_matrix((vals.astype(float), col, ptr), shape=(n, n)).tocsc()
     delta.sort_indices()
-    assert np.array_equal(bp, delta.indptr)
-    assert np.array_equal(br & 0x7FFFFFFF, delta.indices)
-    assert np.array_equal((br >> 31).astype(bool), delta.data < 0)
+    for p in range(0, n, 5):
+        j = bnode[p]
+        got = br[bp[p]:bp[p + 1]]
+        assert np.array_equal(got & 0x7FFFFFFF, delta.indices[delta.indptr[j]:delta.indptr[j + 1]])
+        assert np.array_equal((got >> 31).astype(bool), delta.data[delta.indptr[j]:delta.indptr[j + 1]] < 0)
     # children per position
     cp, cn = pl.ch_ptr.numpy(), pl.ch_node.numpy()
     for p in range(n):
@@ -194,3 +199,52 @@ def test_plan_rejects_cycles_and_double_parents():
     with pytest.raises(ValueError, match="more than one parent"):
         structure.plan(3, crow, col, val, torch.tensor([0, 1], dtype=torch.int32),
                        torch.tensor([2, 2], dtype=torch.int32))
+
+
+def _check_split(ptr, first, n_long, ck_pos, ck_meta, ck_slot, red_pos, red_slot, red_cnt, slots):
+    """Chunks tile every long row exactly; slots are unique and in range."""
+    ptr = np.asarray(ptr, np.int64)
+    lens = np.diff(ptr)
+    long_rows = set(range(first, first + n_long))
+    assert all(lens[p] > 256 for p in long_rows)
+    seen = {}
+    for pos, meta, slot in zip(ck_pos, ck_meta, ck_slot):
+        idx, cnt = int(meta) & 0xFFFF, (int(meta) >> 16) & 0xFFFF
+        assert pos in long_rows and 0 <= idx < cnt
+        seen.setdefault(int(pos), []).append((idx, cnt, int(slot)))
+    assert set(seen) == long_rows
+    used = set()
+    for pos, parts in seen.items():
+        cnt = parts[0][1]
+        assert sorted(i for i, _, _ in parts) == list(range(cnt))
+        covered = sum(lens[pos] * (i + 1) // cnt - lens[pos] * i // cnt for i, _, _ in parts)
+        assert covered == lens[pos]
+        if cnt == 1:
+            assert parts[0][2] == -1
+        else:
+            sl = sorted(s for _, _, s in parts)
+            assert sl == list(range(sl[0], sl[0] + cnt)) and not (set(sl) & used)
+            used |= set(sl)
+            k = list(red_pos).index(pos)
+            assert red_slot[k] == sl[0] and red_cnt[k] == cnt
+    assert all(0 <= s < slots for s in used)
+
+
+def test_plan_chunk_tables_on_long_rows():
+    ei = hub_graph()
+    b, pl = _plan_for(ei.numpy(), HUB_N, 0)
+    st = pl.stats
+    assert st["fwd_slots"] > 0 and st["bwd_slots"] > 0
+    lp, nl = pl.lvl_ptr.numpy(), pl.lvl_nlong.numpy()
+    fck, fred = pl.fck_ptr.numpy(), pl.fred_ptr.numpy()
+    for l in range(pl.depth):
+        c0, c1, r0, r1 = fck[l], fck[l + 1], fred[l], fred[l + 1]
+        _check_split(pl.fwd_ptr.numpy(), lp[l], nl[l], pl.fck_pos.numpy()[c0:c1], pl.fck_meta.numpy()[c0:c1],
+                     pl.fck_slot.numpy()[c0:c1], pl.fred_pos.numpy()[r0:r1], pl.fred_slot.numpy()[r0:r1],
+                     pl.fred_cnt.numpy()[r0:r1], st["fwd_slots"])
+        # short rows follow the long ones within the level
+        lens = np.diff(pl.fwd_ptr.numpy())[lp[l]:lp[l + 1]]
+        assert np.all(lens[:nl[l]] > 256) and np.all(lens[nl[l]:] <= 256)
+    _check_split(pl.bwd_ptr.numpy(), 0, st["bwd_nlong"], pl.bck_pos.numpy(), pl.bck_meta.numpy(),
+                 pl.bck_slot.numpy(), pl.bred_pos.numpy(), pl.bred_slot.numpy(), pl.bred_cnt.numpy(),
+                 st["bwd_slots"])
