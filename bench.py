@@ -2,10 +2,16 @@
 
 A step is one forward product Y = A X plus one adjoint dX = A^T G (cbm4mm,
 the A@X path) over the full synthetic graph of the workload, inputs resident
-in HBM.  Reported: ms per step in BF16 storage / FP32 accumulation (the
-headline `value`) and in FP32, the rocSPARSE FP32 CSR product on the same GPU,
-the HBM roofline fraction of the step, and a CPU baseline (torch.sparse CSR,
-MKL) timed on the host cores.
+in HBM.  Reported on the JSON line:
+
+  value            ms per step, BF16 storage of X and G, FP32 accumulation
+  detail           the FP32 step, the two halves of each product, the
+                   rocSPARSE FP32 CSR step on the same GPU (fastest of its
+                   SpMM algorithms, A^T as an explicit CSR), the same
+                   gathers without compression (our kernel over plain A)
+  roofline         the dominant phase of the step against 8 TB/s HBM
+  cpu_baseline     torch.sparse CSR (MKL) FP32 A@X + A^T@G on the host cores
+  secondary        the compressible workload (c4) with the same fields
 
     python bench.py [--steps K] [--warmup W] [--workload c5] [--alpha 8] [--cols 128]
 """
@@ -20,9 +26,9 @@ sys.path.insert(0, os.path.join(ROOT, "cbm-gpu_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
-# Seeded community copying model per workload: (N, blocks, p_copy, lambda, seed degree, graph seed).
-# lambda (and the C4 seed degree) are set so that nnz lands on the stated size;
-# see the workload table of the design notes.
+# Seeded community copying model per workload: N, blocks, copy probability,
+# random-link rate lambda, first-node degree of a block, graph seed, alpha, F.
+# lambda (and the seed degree) are set so that nnz lands on the stated size.
 WORKLOADS = {
     "c1": dict(n=2708, blocks=7, p=0.8, lam=0.58, seed_deg=0, seed=1001, alpha=0, cols=64),
     "c2": dict(n=19717, blocks=3, p=0.85, lam=0.44, seed_deg=0, seed=1002, alpha=8, cols=128),
@@ -31,18 +37,23 @@ WORKLOADS = {
     "c5": dict(n=2449029, blocks=47, p=0.75, lam=6.6, seed_deg=0, seed=1005, alpha=8, cols=128),
 }
 
+# rocsparse_spmm_alg values tried for the CSR comparator.
+ROCSPARSE_ALGS = {"default": 0, "csr_row_split": 4, "csr_nnz_split": 5, "csr_merge_path": 9}
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def step_bytes(n, nnz_delta, n_edges, depth, f, s_in):
-    """Algorithmic (compulsory) HBM bytes of one forward + one adjoint of the
-    A@X CBM product: every structure array read once, X and G read once,
-    Y and dX written once; intermediates are not counted."""
-    b_fwd = 4 * (n + 1) + 4 * nnz_delta + 8 * n_edges + 4 * (depth + 1) + s_in * n * f + 4 * n * f
-    b_bwd = 8 * (n + 1) + 4 * nnz_delta + 4 * n_edges + 4 * (depth + 1) + s_in * n * f + 4 * n * f
-    return b_fwd, b_bwd
+def phase_bytes(n, nnz_delta, n_edges, n_internal, depth, f, s_in):
+    """Algorithmic (compulsory) HBM bytes of each phase: every structure
+    array read once, every input row read once, every output row written
+    once; gathered rows read again, and intermediates, are not counted."""
+    fwd = 4 * (n + 1) + 4 * nnz_delta + 8 * n_edges + 4 * (depth + 1) + s_in * n * f + 4 * n * f
+    rev = 4 * (n_internal + 1) + 4 * n_internal + 4 * n_edges + s_in * f * (n_internal + n_edges) + 4 * f * n_internal
+    dlt = 4 * (n + 1) + 4 * n + 4 * nnz_delta + s_in * n * f + 4 * n_internal * f + 4 * n * f
+    step_bwd = 8 * (n + 1) + 4 * nnz_delta + 4 * n_edges + 4 * (depth + 1) + s_in * n * f + 4 * n * f
+    return {"forward": fwd, "adjoint_reverse": rev, "adjoint_delta_t": dlt, "step": fwd + step_bwd}
 
 
 def csr_bytes(n, nnz, f):
@@ -64,43 +75,182 @@ def time_loop(fn, steps, warmup, stream):
     return start.elapsed_time(stop) / steps
 
 
-def cpu_baseline(ei, n, f, budget_s=20.0):
-    """torch.sparse CSR FP32 A @ X and A^T @ G (MKL) on the host cores: the
-    CPU counterpart of the step, on the full graph, for as many steps as fit
-    in the budget (at least one)."""
+def build_workload(name, alpha, cols, dev):
+    import torch
+    from cbm import structure
+    from cbm.cbm4mm import cbm4mm
+    w = dict(WORKLOADS[name])
+    if alpha is not None:
+        w["alpha"] = alpha
+    if cols is not None:
+        w["cols"] = cols
+    t0 = time.perf_counter()
+    ei = structure.copying_model(w["n"], w["blocks"], w["p"], w["lam"], w["seed_deg"], w["seed"])
+    log(f"graph {name}: N={w['n']} nnz={ei.shape[1]} ({time.perf_counter() - t0:.1f}s)")
+    t0 = time.perf_counter()
+    m = cbm4mm(ei, torch.ones(ei.shape[1]), alpha=w["alpha"], device=dev)
+    h = m.cusparse_extension
+    log(f"CBM built: nnz(Delta)={h.nnz} tree edges={h.n_edges} depth={h.depth} ({time.perf_counter() - t0:.1f}s)")
+    return w, ei, m
+
+
+def measure(name, w, ei, m, args, dev, stream):
+    """Times the CBM step and its phases, the CSR comparators, and returns the
+    record for one workload (plus the device tensors the CPU leg checks)."""
+    import torch
+    from cbm.cbm_cusparse import cusparseCBM
+    from cbm.csr4mm import csr4mm
+    from cbm.csr_cusparse import cusparseCSR
+
+    h = m.cusparse_extension
+    n, f = w["n"], w["cols"]
+    nnz = int(ei.shape[1])
+    gen = torch.Generator(device=dev).manual_seed(2005)
+    x32 = torch.randn(n, f, device=dev, generator=gen)
+    g32 = torch.randn(n, f, device=dev, generator=gen)
+    x16, g16 = x32.to(torch.bfloat16), g32.to(torch.bfloat16)
+    y = torch.empty(n, f, device=dev)
+    dx = torch.empty(n, f, device=dev)
+    z = torch.empty(n, f, device=dev)
+    h.reserve(f)
+
+    def step(xx, gg):
+        def run():
+            h.forward(xx, y)
+            h.matmul_t(gg, dx, workspace=z)
+        return run
+
+    K, W = args.steps, args.warmup
+    ms16 = time_loop(step(x16, g16), K, W, stream)
+    ms32 = time_loop(step(x32, g32), K, W, stream)
+    log(f"{name}: CBM step bf16 {ms16:.3f} ms, fp32 {ms32:.3f} ms")
+    ph = {
+        "forward": time_loop(lambda: h.forward(x16, y), K, W, stream),
+        "forward_delta_spmm": time_loop(lambda: h.delta_spmm(x16, y), K, W, stream),
+        "forward_propagate": time_loop(lambda: h.propagate(y), K, W, stream),
+        "adjoint": time_loop(lambda: h.matmul_t(g16, dx, workspace=z), K, W, stream),
+        "adjoint_reverse": time_loop(lambda: h.adjoint_reverse(g16, z), K, W, stream),
+        "adjoint_delta_t": time_loop(lambda: h.adjoint_delta_t(g16, z, dx), K, W, stream),
+    }
+    # results the CPU leg compares against FP64 (fresh, from the BF16 and FP32 steps)
+    out = {}
+    for tag, (xx, gg) in (("bf16", (x16, g16)), ("fp32", (x32, g32))):
+        yy, dd = torch.empty(n, f, device=dev), torch.empty(n, f, device=dev)
+        h.forward(xx, yy)
+        h.matmul_t(gg, dd, workspace=z)
+        out[tag] = (yy, dd)
+
+    # rocSPARSE CSR FP32: the fastest algorithm on the forward, A^T explicit
+    csr = csr4mm(ei, num_nodes=n, device=dev)
+    ext = csr.cusparse_extension
+    ext.transposed()
+    alg_ms = {}
+    for alg_name, alg in ROCSPARSE_ALGS.items():
+        try:
+            ext.set_algorithm(alg)
+            alg_ms[alg_name] = time_loop(lambda: csr.matmul(x32, y), max(3, K // 4), 2, stream)
+        except Exception as exc:  # an algorithm rocSPARSE rejects for this shape
+            log(f"rocSPARSE {alg_name}: {exc}")
+    best_alg = min(alg_ms, key=alg_ms.get)
+    ext.set_algorithm(ROCSPARSE_ALGS[best_alg])
+
+    def step_csr():
+        csr.matmul(x32, y)
+        csr.matmul_t(g32, dx)
+
+    ms_csr = time_loop(step_csr, K, W, stream)
+    log(f"{name}: rocSPARSE CSR fp32 step {ms_csr:.3f} ms ({best_alg})")
+    del csr, ext
+
+    # the same gathers without compression: our kernels over plain A, no tree
+    a = torch.sparse_csr_tensor(*_csr_of(ei, n), (n, n)).to(dev)
+    e0 = torch.zeros(0, dtype=torch.int32, device=dev)
+    hip_csr = cusparseCBM(a, torch.zeros(1, dtype=torch.int32, device=dev), e0, e0, torch.empty(0, device=dev))
+    hip_csr.reserve(f)
+    ms_hip_csr16 = time_loop(lambda: (hip_csr.forward(x16, y), hip_csr.matmul_t(g16, dx, workspace=z)), K, W, stream)
+    ms_hip_csr32 = time_loop(lambda: (hip_csr.forward(x32, y), hip_csr.matmul_t(g32, dx, workspace=z)), K, W, stream)
+    del hip_csr, a
+
+    n_internal = int(h._dev["rev_node"].numel())
+    b16 = phase_bytes(n, h.nnz, h.n_edges, n_internal, h.depth, f, 2)
+    b32 = phase_bytes(n, h.nnz, h.n_edges, n_internal, h.depth, f, 4)
+    dom = max(("forward", "adjoint_reverse", "adjoint_delta_t"), key=lambda k: ph[k])
+    dom_gbs = b16[dom] / (ph[dom] * 1e-3) / 1e9
+    rec = {
+        "workload": f"{name}: cbm4mm fwd+bwd, N={n}, nnz(A)={nnz}, alpha={w['alpha']}, F={f}",
+        "nnz_delta": h.nnz, "tree_edges": h.n_edges, "internal_nodes": n_internal, "depth": h.depth,
+        "compression": round(nnz / max(1, h.nnz), 3),
+        "ms_step_bf16": round(ms16, 4), "ms_step_fp32": round(ms32, 4),
+        "ms_phase_bf16": {k: round(v, 4) for k, v in ph.items()},
+        "ms_step_rocsparse_csr_fp32": round(ms_csr, 4), "rocsparse_alg": best_alg,
+        "ms_rocsparse_fwd_by_alg": {k: round(v, 4) for k, v in alg_ms.items()},
+        "ms_step_hip_csr_bf16": round(ms_hip_csr16, 4), "ms_step_hip_csr_fp32": round(ms_hip_csr32, 4),
+        "speedup_bf16_vs_rocsparse_fp32": round(ms_csr / ms16, 3),
+        "speedup_fp32_vs_rocsparse_fp32": round(ms_csr / ms32, 3),
+        "gbs_step_bf16": round(b16["step"] / (ms16 * 1e-3) / 1e9, 1),
+        "gbs_step_fp32": round(b32["step"] / (ms32 * 1e-3) / 1e9, 1),
+        "gbs_rocsparse_csr": round(2 * csr_bytes(n, nnz, f) / (ms_csr * 1e-3) / 1e9, 1),
+        "bytes_step_bf16": b16["step"], "bytes_phase_bf16": b16,
+        "dominant_phase": dom, "dominant_gbs": round(dom_gbs, 1),
+    }
+    return rec, out, (x32, g32, x16, g16)
+
+
+def _csr_of(ei, n):
+    import torch
+    crow = torch.zeros(n + 1, dtype=torch.int64)
+    crow[1:] = torch.bincount(ei[0].long(), minlength=n).cumsum(0)
+    return crow.to(torch.int32), ei[1].to(torch.int32), torch.ones(ei.shape[1])
+
+
+def cpu_baseline(ei, n, f, gpu_out, inputs, budget_s=20.0):
+    """torch.sparse CSR FP32 A @ X and A^T @ G (MKL) on the host cores, on the
+    full graph, for as many steps as fit in the budget; then one FP64 product
+    on the host, against which the GPU results' max relative error is taken."""
     import torch
     cores = len(os.sched_getaffinity(0))
     torch.set_num_threads(cores)
     idx = ei.to(torch.int64)
-    a = torch.sparse_coo_tensor(idx, torch.ones(idx.shape[1]), (n, n)).coalesce().to_sparse_csr()
-    at = a.t().to_sparse_csr()
-    gen = torch.Generator().manual_seed(2005)
-    x = torch.randn(n, f, generator=gen)
-    g = torch.randn(n, f, generator=gen)
-    _ = a @ x[:, :8]  # warm MKL
+    a = torch.sparse_csr_tensor(*[t.to(torch.int64) if i < 2 else t for i, t in enumerate(_csr_of(ei, n))], (n, n))
+    at = torch.sparse_coo_tensor(idx.flip(0), torch.ones(idx.shape[1]), (n, n)).coalesce().to_sparse_csr()
+    x32, g32 = inputs[0].cpu(), inputs[1].cpu()
+    _ = a @ x32[:, :8]  # warm MKL
     times = []
     t_end = time.perf_counter() + budget_s
     while True:
         t0 = time.perf_counter()
-        y = a @ x
-        dx = at @ g
+        y = a @ x32
+        dx = at @ g32
         times.append(time.perf_counter() - t0)
         if time.perf_counter() > t_end or len(times) >= 10:
             break
-    del y, dx
     ms = 1e3 * sum(times) / len(times)
+    errors = {}
+    a64, at64 = a.to(torch.float64), at.to(torch.float64)
+    for tag, xx, gg in (("fp32", x32, g32), ("bf16", inputs[2].float().cpu(), inputs[3].float().cpu())):
+        ry, rd = a64 @ xx.double(), at64 @ gg.double()
+        gy, gd = (t.double().cpu() for t in gpu_out[tag])
+        errors[tag] = {"forward": float((gy - ry).abs().max() / ry.abs().max()),
+                       "adjoint": float((gd - rd).abs().max() / rd.abs().max())}
+    # BF16 results against FP64 of the original FP32 inputs (end-to-end)
+    gy, gd = (t.double().cpu() for t in gpu_out["bf16"])
+    ry, rd = a64 @ x32.double(), at64 @ g32.double()
+    errors["bf16_vs_fp32_inputs"] = {"forward": float((gy - ry).abs().max() / ry.abs().max()),
+                                     "adjoint": float((gd - rd).abs().max() / rd.abs().max())}
+    del y, dx
     return {"value": round(ms, 3), "unit": "ms per A@X fwd+bwd (FP32)", "cores": cores, "kind": "port",
-            "sample": f"full graph N={n} F={f}, {len(times)} steps of torch.sparse CSR A@X + A^T@G (MKL, FP32)"}
+            "sample": f"full graph N={n} F={f}, {len(times)} steps of torch.sparse CSR A@X + A^T@G (MKL, FP32)"}, errors
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
     ap.add_argument("--alpha", type=int, default=None)
     ap.add_argument("--cols", type=int, default=None)
+    ap.add_argument("--secondary", default="c4", help="second workload reported under 'secondary' ('' for none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -109,96 +259,50 @@ def main():
         sys.exit(2)
 
     import torch
-    from cbm import structure
-    from cbm.cbm4mm import cbm4mm
-    from cbm.csr4mm import csr4mm
-
-    w = dict(WORKLOADS[args.workload])
-    if args.alpha is not None:
-        w["alpha"] = args.alpha
-    if args.cols is not None:
-        w["cols"] = args.cols
-    n, f = w["n"], w["cols"]
     dev = torch.device("cuda:0")
-
-    t0 = time.perf_counter()
-    ei = structure.copying_model(n, w["blocks"], w["p"], w["lam"], w["seed_deg"], w["seed"])
-    log(f"graph {args.workload}: N={n} nnz={ei.shape[1]} ({time.perf_counter() - t0:.1f}s)")
-    t0 = time.perf_counter()
-    m = cbm4mm(ei, torch.ones(ei.shape[1]), alpha=w["alpha"], device=dev)
-    h = m.cusparse_extension
-    st = m.build_stats
-    log(f"CBM built: nnz(Delta)={st['nnz_delta']} tree edges={st['tree_edges']} depth={h.depth} "
-        f"({time.perf_counter() - t0:.1f}s)")
-    csr = csr4mm(ei, num_nodes=n, device=dev)
-
-    gen = torch.Generator(device=dev).manual_seed(2005)
-    x32 = torch.randn(n, f, device=dev, generator=gen)
-    g32 = torch.randn(n, f, device=dev, generator=gen)
-    x16, g16 = x32.to(torch.bfloat16), g32.to(torch.bfloat16)
-    y = torch.empty(n, f, device=dev)
-    dx = torch.empty(n, f, device=dev)
-    z = torch.empty(n, f, device=dev)
     stream = torch.cuda.current_stream()
 
-    def step(xx, gg):
-        def run():
-            h.forward(xx, y)
-            h.matmul_t(gg, dx, workspace=z)
-        return run
-
-    def step_csr():
-        csr.matmul(x32, y)
-        csr.matmul_t(g32, dx)
-
-    ms16 = time_loop(step(x16, g16), args.steps, args.warmup, stream)
-    log(f"CBM bf16 step {ms16:.3f} ms")
-    ms32 = time_loop(step(x32, g32), args.steps, args.warmup, stream)
-    log(f"CBM fp32 step {ms32:.3f} ms")
-    ms_fwd16 = time_loop(lambda: h.forward(x16, y), args.steps, args.warmup, stream)
-    ms_bwd16 = time_loop(lambda: h.matmul_t(g16, dx, workspace=z), args.steps, args.warmup, stream)
-    ms_csr = time_loop(step_csr, args.steps, args.warmup, stream)
-    log(f"rocSPARSE CSR fp32 step {ms_csr:.3f} ms")
-
-    b_fwd16, b_bwd16 = step_bytes(n, h.nnz, h.n_edges, h.depth, f, 2)
-    b_fwd32, b_bwd32 = step_bytes(n, h.nnz, h.n_edges, h.depth, f, 4)
-    gbs16 = (b_fwd16 + b_bwd16) / (ms16 * 1e-3) / 1e9
-    gbs32 = (b_fwd32 + b_bwd32) / (ms32 * 1e-3) / 1e9
-    gbs_csr = 2 * csr_bytes(n, int(ei.shape[1]), f) / (ms_csr * 1e-3) / 1e9
-
-    cpu = None
+    w, ei, m = build_workload(args.workload, args.alpha, args.cols, dev)
+    rec, gpu_out, inputs = measure(args.workload, w, ei, m, args, dev, stream)
+    cpu, errors = None, None
     if not args.no_cpu:
         t0 = time.perf_counter()
-        cpu = cpu_baseline(ei, n, f)
+        cpu, errors = cpu_baseline(ei, w["n"], w["cols"], gpu_out, inputs)
+        rec["max_rel_error_vs_fp64"] = errors
         log(f"CPU baseline {cpu['value']} ms on {cpu['cores']} cores ({time.perf_counter() - t0:.1f}s)")
+    del gpu_out, inputs, m
 
+    secondary = None
+    if args.secondary and args.secondary != args.workload:
+        w2, ei2, m2 = build_workload(args.secondary, None, None, dev)
+        secondary, _, _ = measure(args.secondary, w2, ei2, m2, args, dev, stream)
+        del m2
+
+    ms16 = rec["ms_step_bf16"]
+    dom = rec["dominant_phase"]
     out = {
         "metric": "A@X fwd+bwd ms & HBM GB/s (% roofline), CBM-HIP vs CSR on-GPU, FP32 & BF16",
-        "value": round(ms16, 4),
+        "value": ms16,
         "unit": "ms per A@X fwd+bwd (BF16 storage, FP32 accumulate)",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms16, 4),
+        "ms_per_step": ms16,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16-storage/f32-accumulate",
         "data": "synthetic (seeded community copying model; X, G ~ N(0,1))",
-        "config": {"workload": f"{args.workload}: cbm4mm fwd+bwd, N={n}, nnz(A)={int(ei.shape[1])}, "
-                               f"alpha={w['alpha']}, F={f}",
-                   "nnz_delta": h.nnz, "tree_edges": h.n_edges, "depth": h.depth,
-                   "compression": round(int(ei.shape[1]) / max(1, h.nnz), 3)},
-        "roofline": {"bound": "hbm", "achieved": round(gbs16, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs16 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "scope": "whole fwd+bwd step, algorithmic bytes"},
-        "detail": {"ms_fwd_bf16": round(ms_fwd16, 4), "ms_bwd_bf16": round(ms_bwd16, 4),
-                   "ms_step_fp32": round(ms32, 4), "gbs_fp32": round(gbs32, 1),
-                   "frac_fp32": round(gbs32 / HBM_PEAK_GBS, 4),
-                   "ms_step_rocsparse_csr_fp32": round(ms_csr, 4), "gbs_csr": round(gbs_csr, 1),
-                   "speedup_bf16_vs_csr": round(ms_csr / ms16, 3), "speedup_fp32_vs_csr": round(ms_csr / ms32, 3),
-                   "bytes_step_bf16": b_fwd16 + b_bwd16, "bytes_step_fp32": b_fwd32 + b_bwd32},
+        "config": {"workload": rec["workload"], "nnz_delta": rec["nnz_delta"], "depth": rec["depth"],
+                   "compression": rec["compression"]},
+        "roofline": {"bound": "hbm", "achieved": rec["dominant_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(rec["dominant_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"{dom} phase (BF16)",
+                     "algorithmic_bytes": rec["bytes_phase_bf16"][dom],
+                     "ms": rec["ms_phase_bf16"][dom]},
+        "detail": rec,
         "cpu_baseline": cpu,
+        "secondary": secondary,
     }
     print(json.dumps(out), flush=True)
 

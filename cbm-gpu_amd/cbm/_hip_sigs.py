@@ -20,12 +20,15 @@ class CbmDesc(ctypes.Structure):
         ("lvl_nlong_host", c_vp),
         ("fck_ptr_host", c_vp),
         ("fred_ptr_host", c_vp),
+        ("rev_ptr_host", c_vp),
         ("lvl_node", c_vp),
         ("lvl_parent", c_vp),
         ("fwd_ptr", c_vp),
         ("fwd_col", c_vp),
+        ("rev_node", c_vp),
         ("ch_ptr", c_vp),
         ("ch_node", c_vp),
+        ("ch_mid", c_vp),
         ("fck_pos", c_vp),
         ("fck_meta", c_vp),
         ("fck_slot", c_vp),
@@ -36,6 +39,7 @@ class CbmDesc(ctypes.Structure):
         ("bwd_node", c_vp),
         ("bwd_ptr", c_vp),
         ("bwd_row", c_vp),
+        ("bwd_mid", c_vp),
         ("bck_pos", c_vp),
         ("bck_meta", c_vp),
         ("bck_slot", c_vp),
@@ -50,10 +54,10 @@ class CbmDesc(ctypes.Structure):
     ]
 
 # Device arrays of the plan that the descriptor points at.
-DEVICE_ARRAYS = ("lvl_node", "lvl_parent", "fwd_ptr", "fwd_col", "ch_ptr", "ch_node", "fck_pos", "fck_meta",
-                 "fck_slot", "fred_pos", "fred_slot", "fred_cnt", "bwd_node", "bwd_ptr", "bwd_row", "bck_pos",
+DEVICE_ARRAYS = ("lvl_node", "lvl_parent", "fwd_ptr", "fwd_col", "rev_node", "ch_ptr", "ch_node", "ch_mid", "fck_pos", "fck_meta",
+                 "fck_slot", "fred_pos", "fred_slot", "fred_cnt", "bwd_node", "bwd_ptr", "bwd_row", "bwd_mid", "bck_pos",
                  "bck_meta", "bck_slot", "bred_pos", "bred_slot", "bred_cnt")
-HOST_ARRAYS = ("lvl_ptr", "lvl_nlong", "fck_ptr", "fred_ptr")
+HOST_ARRAYS = ("lvl_ptr", "lvl_nlong", "fck_ptr", "fred_ptr", "rev_ptr")
 
 
 def declare(lib, sig):
@@ -65,9 +69,12 @@ def declare(lib, sig):
     sig(lib, "cbm_reserve", c_i32, c_vp, c_i32)
     sig(lib, "cbm_matmul", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
     sig(lib, "cbm_matmul_t", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
+    sig(lib, "cbm_adjoint_reverse", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
+    sig(lib, "cbm_adjoint_delta_t", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
     sig(lib, "cbm_delta_spmm", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
     sig(lib, "cbm_propagate", c_i32, c_vp, c_vp, c_i64, c_i32, c_vp)
     sig(lib, "cbm_last_launches", c_i32, c_vp)
+    sig(lib, "cbm_set_slab_bytes", c_i32, c_vp, c_i64)
     sig(lib, "csr_last_error", ctypes.c_char_p)
     sig(lib, "csr_create", c_i32, c_i32, c_i32, c_i64, c_vp, c_vp, c_vp, P_vp)
     sig(lib, "csr_destroy", c_i32, c_vp)
@@ -81,6 +88,7 @@ def declare(lib, sig):
 
 
 # Every symbol include/cbm_hip.h declares (checked by the CPU test suite).
-EXPORTED = ("cbm_last_error", "cbm_desc_size", "cbm_create", "cbm_destroy", "cbm_reserve", "cbm_matmul", "cbm_matmul_t", "cbm_delta_spmm",
-            "cbm_propagate", "cbm_last_launches", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
+EXPORTED = ("cbm_last_error", "cbm_desc_size", "cbm_create", "cbm_destroy", "cbm_reserve", "cbm_matmul", "cbm_matmul_t", "cbm_adjoint_reverse",
+            "cbm_adjoint_delta_t", "cbm_delta_spmm",
+            "cbm_propagate", "cbm_last_launches", "cbm_set_slab_bytes", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
             "csr_last_error", "cbm_op_spmm", "cbm_op_update", "cbm_op_axpy", "cbm_op_last_error")

@@ -224,6 +224,37 @@ class cusparseCBM:
         _lib.check_hip(rc, "matmul_t")
         return 0
 
+    def adjoint_halves(self, grad, out, workspace):
+        """The adjoint in its two halves (reverse gather into `workspace`,
+        then the Delta^T pass into `out`), for separate timing."""
+        g, code = self._check_in(grad)
+        f = int(g.size(1))
+        self._check_out(out, f, "out")
+        self._check_out(workspace, f, "workspace")
+        st = current_stream_handle(self.device)
+        _lib.check_hip(self._lib.cbm_adjoint_reverse(self._h, code, g.data_ptr(), g.stride(0), workspace.data_ptr(),
+                                                     workspace.stride(0), f, st), "adjoint_reverse")
+        _lib.check_hip(self._lib.cbm_adjoint_delta_t(self._h, code, g.data_ptr(), g.stride(0), workspace.data_ptr(),
+                                                     workspace.stride(0), out.data_ptr(), out.stride(0), f, st),
+                       "adjoint_delta_t")
+        return 0
+
+    def adjoint_reverse(self, grad, workspace):
+        g, code = self._check_in(grad)
+        f = int(g.size(1))
+        self._check_out(workspace, f, "workspace")
+        _lib.check_hip(self._lib.cbm_adjoint_reverse(self._h, code, g.data_ptr(), g.stride(0), workspace.data_ptr(),
+                                                     workspace.stride(0), f, current_stream_handle(self.device)),
+                       "adjoint_reverse")
+
+    def adjoint_delta_t(self, grad, workspace, out):
+        g, code = self._check_in(grad)
+        f = int(g.size(1))
+        self._check_out(out, f, "out")
+        _lib.check_hip(self._lib.cbm_adjoint_delta_t(self._h, code, g.data_ptr(), g.stride(0), workspace.data_ptr(),
+                                                     workspace.stride(0), out.data_ptr(), out.stride(0), f,
+                                                     current_stream_handle(self.device)), "adjoint_delta_t")
+
     def delta_spmm(self, rhs, res):
         """res = Delta' rhs without the tree propagation."""
         x, code = self._check_in(rhs)
@@ -250,6 +281,10 @@ class cusparseCBM:
         """Pre-sizes the long-row scratch so that products of up to n_cols
         columns allocate nothing (needed before HIP-graph capture)."""
         _lib.check_hip(self._lib.cbm_reserve(self._h, int(n_cols)), "reserve")
+
+    def set_slab_bytes(self, nbytes):
+        """Column-slab size for cache residency of the gathers (0: whole rows)."""
+        _lib.check_hip(self._lib.cbm_set_slab_bytes(self._h, int(nbytes)), "set_slab_bytes")
 
     def last_launches(self):
         return int(self._lib.cbm_last_launches(self._h))

@@ -40,6 +40,8 @@ class cusparseCSR:
                                       self._col.data_ptr() or None, self._val.data_ptr() or None, ctypes.byref(h))
         self._check(rc, "cusparseCSR")
         self._h = h
+        self._alg = algorithm
+        self._t = None
         self.set_algorithm(algorithm)
 
     def _check(self, rc, what):
@@ -54,7 +56,26 @@ class cusparseCSR:
             self._h = None
 
     def set_algorithm(self, algorithm):
+        self._alg = algorithm
         self._check(self._lib.csr_set_algorithm(self._h, int(algorithm)), "set_algorithm")
+        if getattr(self, "_t", None) is not None:
+            self._t.set_algorithm(algorithm)
+
+    def transposed(self):
+        """A^T as its own CSR handle, built once on the device (a sort of the
+        entries by column); matmul_t runs it as a plain SpMM, which is far
+        faster than rocSPARSE's transposed-operand SpMM."""
+        if self._t is None:
+            dev = self.device
+            crow, col = self._crow.long(), self._col.long()
+            rows = torch.repeat_interleave(torch.arange(self.num_rows, device=dev), crow.diff())
+            order = torch.argsort(col * self.num_rows + rows)
+            t_crow = torch.zeros(self.num_cols + 1, dtype=torch.int64, device=dev)
+            t_crow[1:] = torch.cumsum(torch.bincount(col, minlength=self.num_cols), 0)
+            at = torch.sparse_csr_tensor(t_crow.to(torch.int32), rows[order].to(torch.int32), self._val[order],
+                                         (self.num_cols, self.num_rows))
+            self._t = cusparseCSR(at, algorithm=self._alg)
+        return self._t
 
     def _run(self, trans, rhs, res):
         _require_cuda(rhs)
@@ -81,5 +102,10 @@ class cusparseCSR:
         return self._run(0, rhs, res)
 
     def matmul_t(self, rhs, res):
-        """res = A^T rhs (the comparator of the adjoint)."""
+        """res = A^T rhs (the comparator of the adjoint), as a plain SpMM over
+        the explicitly transposed matrix."""
+        return self.transposed().matmul(rhs, res)
+
+    def matmul_t_rocsparse(self, rhs, res):
+        """res = A^T rhs with rocSPARSE's transposed-operand SpMM."""
         return self._run(1, rhs, res)

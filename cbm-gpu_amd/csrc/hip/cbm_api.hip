@@ -15,10 +15,11 @@
 struct cbm_handle {
     cbm_desc d{};
     // host copies of the per-level tables
-    std::vector<int32_t> lvl_ptr, lvl_nlong, fck_ptr, fred_ptr;
+    std::vector<int32_t> lvl_ptr, lvl_nlong, fck_ptr, fred_ptr, rev_ptr;
     float *ps_pos = nullptr;  // D A D only: d[node] / d[parent] per level position
     float *scratch = nullptr; // partial sums of long-row chunks
     int64_t scratch_floats = 0;
+    int64_t slab_bytes = 0;   // column slab target (gathered bytes), 0 = whole rows
     int last_launches = 0;
 };
 
@@ -116,6 +117,29 @@ int run_range(cbm_handle *h, cbmk::RowPass rp, int32_t p_begin, int32_t p_end, i
     return CBM_OK;
 }
 
+// Column slabs: the columns are processed in slabs so that the gathered
+// matrix's slab (N x width x elem bytes) stays within slab_bytes and keeps
+// hitting the device's last-level cache.  Slab starts stay 16-byte aligned.
+template <class F>
+int for_each_slab(const cbm_handle *h, int32_t n_cols, int elem, F &&fn) {
+    int32_t width = n_cols;
+    if (h->slab_bytes > 0) {
+        const int32_t align = 16 / elem;
+        const int64_t fit = h->slab_bytes / (int64_t(h->d.n_rows) * elem);
+        if (fit < n_cols) width = std::max<int32_t>(align, int32_t(fit / align) * align);
+    }
+    for (int32_t c0 = 0; c0 < n_cols; c0 += width) {
+        const int rc = fn(c0, std::min(width, n_cols - c0));
+        if (rc != CBM_OK) return rc;
+    }
+    return CBM_OK;
+}
+
+template <typename T>
+const T *shift(const void *p, int32_t c0, int elem) {
+    return reinterpret_cast<const T *>(static_cast<const char *>(p) + int64_t(c0) * elem);
+}
+
 // The forward Delta pass over every level; with_parent adds the parent term.
 int forward_levels(cbm_handle *h, cbmk::RowPass rp, bool with_parent, hipStream_t stream, const char *what) {
     cbmk::ChunkArgs ck;
@@ -176,10 +200,11 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
     }
     *out = nullptr;
     const cbm_desc &d = *desc;
-    const bool tables = d.lvl_ptr_host && d.lvl_nlong_host && d.fck_ptr_host && d.fred_ptr_host;
+    const bool tables = d.lvl_ptr_host && d.lvl_nlong_host && d.fck_ptr_host && d.fred_ptr_host && d.rev_ptr_host;
     if (d.n_rows < 1 || d.n_levels < 1 || d.nnz < 0 || d.n_edges < 0 || d.n_edges >= d.n_rows || !tables ||
-        !d.lvl_node || !d.lvl_parent || !d.fwd_ptr || !d.bwd_ptr || !d.bwd_node || !d.ch_ptr ||
-        (d.nnz > 0 && (!d.fwd_col || !d.bwd_row)) || (d.n_edges > 0 && !d.ch_node) || d.fwd_slots < 0 ||
+        !d.lvl_node || !d.lvl_parent || !d.fwd_ptr || !d.bwd_ptr || !d.bwd_node || !d.bwd_mid || !d.ch_ptr ||
+        (d.nnz > 0 && (!d.fwd_col || !d.bwd_row)) || (d.n_edges > 0 && (!d.ch_node || !d.rev_node || !d.ch_mid)) ||
+        d.fwd_slots < 0 ||
         d.bwd_slots < 0 || d.n_bck < 0 || d.n_bred < 0 || d.bwd_nlong < 0 || d.bwd_nlong > d.n_rows ||
         (d.n_bck > 0 && (!d.bck_pos || !d.bck_meta || !d.bck_slot)) ||
         (d.n_bred > 0 && (!d.bred_pos || !d.bred_slot || !d.bred_cnt))) {
@@ -196,14 +221,16 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
         return CBM_ERR_ARG;
     }
     const int32_t L = d.n_levels;
-    if (d.lvl_ptr_host[0] != 0 || d.lvl_ptr_host[L] != d.n_rows || d.fck_ptr_host[0] != 0 || d.fred_ptr_host[0] != 0) {
+    if (d.lvl_ptr_host[0] != 0 || d.lvl_ptr_host[L] != d.n_rows || d.fck_ptr_host[0] != 0 || d.fred_ptr_host[0] != 0 ||
+        d.rev_ptr_host[0] != 0 || d.rev_ptr_host[L] > d.n_rows) {
         set_error("cbm_create: level tables do not start at 0 / end at n_rows");
         return CBM_ERR_ARG;
     }
     for (int32_t l = 0; l < L; ++l) {
         if (d.lvl_ptr_host[l + 1] <= d.lvl_ptr_host[l] || d.lvl_nlong_host[l] < 0 ||
             d.lvl_nlong_host[l] > d.lvl_ptr_host[l + 1] - d.lvl_ptr_host[l] ||
-            d.fck_ptr_host[l + 1] < d.fck_ptr_host[l] || d.fred_ptr_host[l + 1] < d.fred_ptr_host[l]) {
+            d.fck_ptr_host[l + 1] < d.fck_ptr_host[l] || d.fred_ptr_host[l + 1] < d.fred_ptr_host[l] ||
+            d.rev_ptr_host[l + 1] < d.rev_ptr_host[l]) {
             set_error("cbm_create: inconsistent level tables at level %d", l);
             return CBM_ERR_ARG;
         }
@@ -223,6 +250,8 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
     h->lvl_nlong.assign(d.lvl_nlong_host, d.lvl_nlong_host + L);
     h->fck_ptr.assign(d.fck_ptr_host, d.fck_ptr_host + L + 1);
     h->fred_ptr.assign(d.fred_ptr_host, d.fred_ptr_host + L + 1);
+    h->rev_ptr.assign(d.rev_ptr_host, d.rev_ptr_host + L + 1);
+    h->d.rev_ptr_host = h->rev_ptr.data();
     h->d.lvl_ptr_host = h->lvl_ptr.data();
     h->d.lvl_nlong_host = h->lvl_nlong.data();
     h->d.fck_ptr_host = h->fck_ptr.data();
@@ -267,57 +296,87 @@ extern "C" int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t l
     if (rc == CBM_OK) rc = ensure_scratch(h, n_cols);
     if (rc != CBM_OK) return rc;
     h->last_launches = 0;
-    return forward_levels(h, forward_pass(h, dtype, x, ldx, y, ldy, n_cols), true, stream, "cbm_matmul");
+    const int elem = dtype == CBM_BF16 ? 2 : 4;
+    return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
+        return forward_levels(h, forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w), true, stream,
+                              "cbm_matmul");
+    });
 }
 
-extern "C" int cbm_matmul_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, float *dx, int64_t lddx,
-                            float *z, int64_t ldz, int32_t n_cols, hipStream_t stream) {
-    int rc = check_dense(h, dtype, g, ldg, dx, lddx, n_cols, "cbm_matmul_t");
+namespace {
+
+int check_adjoint(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, float *dx, int64_t lddx, float *z,
+                  int64_t ldz, int32_t n_cols, const char *what) {
+    int rc = check_dense(h, dtype, g, ldg, dx, lddx, n_cols, what);
     if (rc != CBM_OK) return rc;
     if (!z || ldz < n_cols) {
-        set_error("cbm_matmul_t: invalid workspace (ldz=%lld)", (long long)ldz);
+        set_error("%s: invalid workspace (ldz=%lld)", what, (long long)ldz);
         return CBM_ERR_ARG;
     }
     const int32_t n = h->d.n_rows;
     if (overlaps(z, span_bytes(n, ldz, n_cols, 4), dx, span_bytes(n, lddx, n_cols, 4)) ||
         overlaps(z, span_bytes(n, ldz, n_cols, 4), g, span_bytes(n, ldg, n_cols, dtype == CBM_BF16 ? 2 : 4))) {
-        set_error("cbm_matmul_t: workspace overlaps an operand");
+        set_error("%s: workspace overlaps an operand", what);
         return CBM_ERR_ARG;
     }
-    rc = ensure_scratch(h, n_cols);
-    if (rc != CBM_OK) return rc;
-    h->last_launches = 0;
-    // Reverse pass, deepest level first: z_u = s_u g_u + sum_{children v} z_v
-    // with s = D^-1/2 for D A D (the adjoint's (I-P)^-T applied to D G).
+    return ensure_scratch(h, n_cols);
+}
+
+// Reverse pass over the nodes with children, deepest level first:
+//   z_u = s_u g_u + sum_{children v} (v has children ? z_v : s_v g_v)
+// with s = D^-1/2 for D A D (the adjoint's (I-P)^-T applied to D G) and
+// 1 otherwise.  A leaf's z would be s g, so it is never written: both this
+// pass and the Delta^T pass read s g straight from G for leaves.
+int adjoint_reverse(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, float *z, int64_t ldz, int32_t n_cols,
+                    hipStream_t stream) {
+    const float *gs = h->d.mode == CBM_MODE_DADX ? h->d.d : nullptr;
     cbmk::RowPass rv;
-    rv.node_of = h->d.lvl_node;
+    rv.node_of = h->d.rev_node;
     rv.ptr = h->d.ch_ptr;
     rv.idx = h->d.ch_node;
+    rv.dual = true;
+    rv.mid = h->d.ch_mid;
     rv.g = z;
     rv.ldg = ldz;
-    rv.g_bf16 = false;
+    rv.g2 = g;
+    rv.ldg2 = ldg;
+    rv.g2_bf16 = dtype == CBM_BF16;
+    rv.g2scale = gs;
     rv.e = g;
     rv.lde = ldg;
     rv.e_bf16 = dtype == CBM_BF16;
-    rv.escale = h->d.mode == CBM_MODE_DADX ? h->d.d : nullptr;
+    rv.escale = gs;
     rv.extra = cbmk::kExtraSelf;
     rv.y = z;
     rv.ldy = ldz;
     rv.ncols = n_cols;
     for (int32_t l = h->d.n_levels - 1; l >= 0; --l) {
-        rv.p_begin = h->lvl_ptr[size_t(l)];
-        rv.p_end = h->lvl_ptr[size_t(l) + 1];
+        rv.p_begin = h->rev_ptr[size_t(l)];
+        rv.p_end = h->rev_ptr[size_t(l) + 1];
+        if (rv.p_end <= rv.p_begin) continue;
         const hipError_t e = cbmk::launch_row_pass(rv, stream);
-        if (e != hipSuccess) return device_error(e, "cbm_matmul_t (reverse pass)");
+        if (e != hipSuccess) return device_error(e, "adjoint reverse pass");
         ++h->last_launches;
     }
-    // Delta^T pass: dx_j = s_j * sum_v sign * z_v over column j of Delta.
+    return CBM_OK;
+}
+
+// Delta^T pass: dx_j = r_j * sum_v sign * (z_v or s_v g_v) over column j of
+// Delta, r = D^-1/2 for A D and D A D.
+int adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, const float *z, int64_t ldz, float *dx,
+                    int64_t lddx, int32_t n_cols, hipStream_t stream) {
     cbmk::RowPass tp;
     tp.node_of = h->d.bwd_node;
     tp.ptr = h->d.bwd_ptr;
     tp.idx = h->d.bwd_row;
+    tp.dual = true;
+    tp.mid = h->d.bwd_mid;
     tp.g = z;
     tp.ldg = ldz;
+    tp.g2 = g;
+    tp.ldg2 = ldg;
+    tp.g2_bf16 = dtype == CBM_BF16;
+    tp.g2scale = h->d.mode == CBM_MODE_DADX ? h->d.d : nullptr;
     tp.y = dx;
     tp.ldy = lddx;
     tp.ncols = n_cols;
@@ -336,7 +395,40 @@ extern "C" int cbm_matmul_t(cbm_handle *h, int32_t dtype, const void *g, int64_t
     rd.first = 0;
     rd.count = h->d.n_bred;
     rd.scratch = h->scratch;
-    return run_range(h, tp, 0, n, h->d.bwd_nlong, ck, rd, stream, "cbm_matmul_t (Delta^T pass)");
+    return run_range(h, tp, 0, h->d.n_rows, h->d.bwd_nlong, ck, rd, stream, "adjoint Delta^T pass");
+}
+
+}  // namespace
+
+extern "C" int cbm_matmul_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, float *dx, int64_t lddx,
+                            float *z, int64_t ldz, int32_t n_cols, hipStream_t stream) {
+    int rc = check_adjoint(h, dtype, g, ldg, dx, lddx, z, ldz, n_cols, "cbm_matmul_t");
+    if (rc != CBM_OK) return rc;
+    h->last_launches = 0;
+    const int elem = dtype == CBM_BF16 ? 2 : 4;
+    return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
+        const void *gs = shift<char>(g, c0, elem);
+        int r = adjoint_reverse(h, dtype, gs, ldg, z + c0, ldz, w, stream);
+        if (r == CBM_OK) r = adjoint_delta_t(h, dtype, gs, ldg, z + c0, ldz, dx + c0, lddx, w, stream);
+        return r;
+    });
+}
+
+extern "C" int cbm_adjoint_reverse(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, float *z, int64_t ldz,
+                                   int32_t n_cols, hipStream_t stream) {
+    // This half reads g and writes z only.
+    int rc = check_dense(h, dtype, g, ldg, z, ldz, n_cols, "cbm_adjoint_reverse");
+    if (rc != CBM_OK) return rc;
+    h->last_launches = 0;
+    return adjoint_reverse(h, dtype, g, ldg, z, ldz, n_cols, stream);
+}
+
+extern "C" int cbm_adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, const float *z,
+                                   int64_t ldz, float *dx, int64_t lddx, int32_t n_cols, hipStream_t stream) {
+    int rc = check_adjoint(h, dtype, g, ldg, dx, lddx, const_cast<float *>(z), ldz, n_cols, "cbm_adjoint_delta_t");
+    if (rc != CBM_OK) return rc;
+    h->last_launches = 0;
+    return adjoint_delta_t(h, dtype, g, ldg, z, ldz, dx, lddx, n_cols, stream);
 }
 
 extern "C" int cbm_delta_spmm(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy,
@@ -345,7 +437,11 @@ extern "C" int cbm_delta_spmm(cbm_handle *h, int32_t dtype, const void *x, int64
     if (rc == CBM_OK) rc = ensure_scratch(h, n_cols);
     if (rc != CBM_OK) return rc;
     h->last_launches = 0;
-    return forward_levels(h, forward_pass(h, dtype, x, ldx, y, ldy, n_cols), false, stream, "cbm_delta_spmm");
+    const int elem = dtype == CBM_BF16 ? 2 : 4;
+    return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
+        return forward_levels(h, forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w), false, stream,
+                              "cbm_delta_spmm");
+    });
 }
 
 extern "C" int cbm_propagate(cbm_handle *h, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream) {
@@ -365,3 +461,12 @@ extern "C" int cbm_propagate(cbm_handle *h, float *y, int64_t ldy, int32_t n_col
 }
 
 extern "C" int cbm_last_launches(const cbm_handle *h) { return h ? h->last_launches : -1; }
+
+extern "C" int cbm_set_slab_bytes(cbm_handle *h, int64_t bytes) {
+    if (!h || bytes < 0) {
+        set_error("cbm_set_slab_bytes: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    h->slab_bytes = bytes;
+    return CBM_OK;
+}

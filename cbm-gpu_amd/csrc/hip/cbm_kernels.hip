@@ -122,21 +122,35 @@ __device__ inline float signed_scale(float scale, int32_t packed) {
     return __uint_as_float(__float_as_uint(scale) ^ (uint32_t(packed) & kSign));
 }
 
-// sum over k in [kb, ke) of sign_k * scale_k * g[j_k, c0 .. c0+VE), accumulated
-// into acc by the LPR lanes of one group (all lanes of the group call it with
-// the same kb, ke; lanes whose columns are past the end pass active = false
-// but still take part in the index broadcasts).
-template <typename Tg, int VE, int LPR>
-__device__ inline void gather_sum(const RowPass &a, int32_t kb, int32_t ke, int l, int32_t c0, bool active,
-                                  float (&acc)[VE]) {
-    const Tg *__restrict__ g = static_cast<const Tg *>(a.g);
+// Compile-time shape of a row pass.
+//   G   type of the gathered matrix (g)
+//   G2  type of the second gathered matrix (g2), read when DUAL and the
+//       packed index lacks the internal bit
+//   E   type of the extra-term matrix (e)
+template <typename G, typename G2, typename E, int EXTRA, bool DUAL>
+struct Pass {
+    using TG = G;
+    using TG2 = G2;
+    using TE = E;
+    static constexpr int kExtra = EXTRA;
+    static constexpr bool kDual = DUAL;
+};
+
+// acc += sum over k in [kb, ke) of sign_k * scale[j_k] * val_k * src[j_k, c0 .. c0+VE),
+// by the LPR lanes of one group (all lanes of the group call it with the same
+// kb, ke; lanes whose columns are past the end pass active = false but still
+// take part in the index broadcasts).  scale and val may be null (1).
+template <typename TS, int VE, int LPR>
+__device__ inline void gather_src(const RowPass &a, const TS *__restrict__ src, int64_t ld,
+                                  const float *__restrict__ scale, const float *__restrict__ val, int32_t kb,
+                                  int32_t ke, int l, int32_t c0, bool active, float (&acc)[VE]) {
     for (int32_t base = kb; base < ke; base += LPR) {
         int32_t mine = 0;
         float mine_scale = 0.0f;
         if (base + l < ke) {
             mine = a.idx[base + l];
-            float cs = a.colscale ? a.colscale[mine & kIndexMask] : 1.0f;
-            if (a.val) cs *= a.val[base + l];
+            float cs = scale ? scale[mine & kIndexMask] : 1.0f;
+            if (val) cs *= val[base + l];
             mine_scale = signed_scale(cs, mine);
         }
         const int32_t cnt = min(LPR, ke - base);
@@ -152,7 +166,7 @@ __device__ inline void gather_sum(const RowPass &a, int32_t kb, int32_t ke, int 
             if (active) {
                 float v[4][VE];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) Load<Tg, VE>::run(g + int64_t(j[u]) * a.ldg + c0, v[u]);
+                for (int u = 0; u < 4; ++u) Load<TS, VE>::run(src + int64_t(j[u]) * ld + c0, v[u]);
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -164,7 +178,7 @@ __device__ inline void gather_sum(const RowPass &a, int32_t kb, int32_t ke, int 
             const float sc = group_bcast<LPR>(mine_scale, k);
             if (active) {
                 float v[VE];
-                Load<Tg, VE>::run(g + int64_t(j) * a.ldg + c0, v);
+                Load<TS, VE>::run(src + int64_t(j) * ld + c0, v);
 #pragma unroll
                 for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], sc, acc[q]);
             }
@@ -172,25 +186,44 @@ __device__ inline void gather_sum(const RowPass &a, int32_t kb, int32_t ke, int 
     }
 }
 
+// The gather of position p's entries in [kb, ke).  A dual pass reads the
+// entries before mid[p] from g2 (scaled by g2scale) and the rest from g, as
+// two branch-free loops.
+template <class P, int VE, int LPR>
+__device__ inline void gather_sum(const RowPass &a, int32_t p, int32_t kb, int32_t ke, int l, int32_t c0, bool active,
+                                  float (&acc)[VE]) {
+    using TG = typename P::TG;
+    if constexpr (P::kDual) {
+        using TG2 = typename P::TG2;
+        const int32_t m = min(max(a.mid[p], kb), ke);
+        gather_src<TG2, VE, LPR>(a, static_cast<const TG2 *>(a.g2), a.ldg2, a.g2scale, nullptr, kb, m, l, c0, active,
+                                 acc);
+        gather_src<TG, VE, LPR>(a, static_cast<const TG *>(a.g), a.ldg, nullptr, nullptr, m, ke, l, c0, active, acc);
+    } else {
+        gather_src<TG, VE, LPR>(a, static_cast<const TG *>(a.g), a.ldg, a.colscale, a.val, kb, ke, l, c0, active, acc);
+    }
+}
+
 // Row scale, the extra term, and the store of one finished row segment.
-template <typename Te, int VE, int EXTRA>
+template <class P, int VE>
 __device__ inline void finish_row(const RowPass &a, int32_t p, int32_t row, int32_t c0, float (&acc)[VE]) {
     const float rs = a.rowscale ? a.rowscale[row] : 1.0f;
 #pragma unroll
     for (int q = 0; q < VE; ++q) acc[q] *= rs;
-    if constexpr (EXTRA == kExtraParent) {
+    using TE = typename P::TE;
+    if constexpr (P::kExtra == kExtraParent) {
         const int32_t par = a.parent_of[p];
         if (par >= 0) {
             const float es = (a.escale ? a.escale[p] : 1.0f) * a.ecoef;
             float v[VE];
-            Load<Te, VE>::run(static_cast<const Te *>(a.e) + int64_t(par) * a.lde + c0, v);
+            Load<TE, VE>::run(static_cast<const TE *>(a.e) + int64_t(par) * a.lde + c0, v);
 #pragma unroll
             for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], es, acc[q]);
         }
-    } else if constexpr (EXTRA == kExtraSelf) {
+    } else if constexpr (P::kExtra == kExtraSelf) {
         const float es = (a.escale ? a.escale[row] : 1.0f) * a.ecoef;
         float v[VE];
-        Load<Te, VE>::run(static_cast<const Te *>(a.e) + int64_t(row) * a.lde + c0, v);
+        Load<TE, VE>::run(static_cast<const TE *>(a.e) + int64_t(row) * a.lde + c0, v);
 #pragma unroll
         for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], es, acc[q]);
     }
@@ -198,7 +231,7 @@ __device__ inline void finish_row(const RowPass &a, int32_t p, int32_t row, int3
 }
 
 // Short rows: one group of LPR lanes per row, 64 / LPR rows per wavefront.
-template <typename Tg, typename Te, int VE, int LPR, int EXTRA>
+template <class P, int VE, int LPR>
 __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
     constexpr int kGroupsPerWave = 64 / LPR;
     const int lane = int(threadIdx.x & 63u);
@@ -218,8 +251,8 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
         float acc[VE];
 #pragma unroll
         for (int q = 0; q < VE; ++q) acc[q] = 0.0f;
-        gather_sum<Tg, VE, LPR>(a, kb, ke, l, c0, active, acc);
-        if (active) finish_row<Te, VE, EXTRA>(a, p, row, c0, acc);
+        gather_sum<P, VE, LPR>(a, p, kb, ke, l, c0, active, acc);
+        if (active) finish_row<P, VE>(a, p, row, c0, acc);
     }
 }
 
@@ -227,7 +260,7 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
 // entries are split into contiguous slices, one per group of LPR lanes; the
 // groups' partial sums are added in group order through LDS.  A one-chunk
 // row is finished here; otherwise the chunk's sum goes to its scratch slot.
-template <typename Tg, typename Te, int VE, int LPR, int EXTRA, int BS>
+template <class P, int VE, int LPR, int BS>
 __global__ __launch_bounds__(BS) void k_row_chunk(const RowPass a, const ChunkArgs c) {
     constexpr int kGroups = BS / LPR;
     __shared__ float part[BS * VE];
@@ -251,7 +284,7 @@ __global__ __launch_bounds__(BS) void k_row_chunk(const RowPass a, const ChunkAr
         float acc[VE];
 #pragma unroll
         for (int q = 0; q < VE; ++q) acc[q] = 0.0f;
-        gather_sum<Tg, VE, LPR>(a, gb, ge, l, c0, active, acc);
+        gather_sum<P, VE, LPR>(a, p, gb, ge, l, c0, active, acc);
 #pragma unroll
         for (int q = 0; q < VE; ++q) part[threadIdx.x * VE + q] = acc[q];
         __syncthreads();
@@ -260,7 +293,7 @@ __global__ __launch_bounds__(BS) void k_row_chunk(const RowPass a, const ChunkAr
 #pragma unroll
                 for (int q = 0; q < VE; ++q) acc[q] += part[(gg * LPR + l) * VE + q];
             if (slot < 0) {
-                finish_row<Te, VE, EXTRA>(a, p, row, c0, acc);
+                finish_row<P, VE>(a, p, row, c0, acc);
             } else {
                 store_row<VE>(c.scratch + int64_t(slot) * a.ncols + c0, acc);
             }
@@ -270,7 +303,7 @@ __global__ __launch_bounds__(BS) void k_row_chunk(const RowPass a, const ChunkAr
 }
 
 // Rows of several chunks: add the chunk sums in chunk order and finish.
-template <typename Te, int VE, int LPR, int EXTRA>
+template <class P, int VE, int LPR>
 __global__ __launch_bounds__(256) void k_row_reduce(const RowPass a, const ReduceArgs r) {
     constexpr int kGroupsPerWave = 64 / LPR;
     const int lane = int(threadIdx.x & 63u);
@@ -290,7 +323,7 @@ __global__ __launch_bounds__(256) void k_row_reduce(const RowPass a, const Reduc
 #pragma unroll
             for (int q = 0; q < VE; ++q) acc[q] += v[q];
         }
-        finish_row<Te, VE, EXTRA>(a, p, row, c0, acc);
+        finish_row<P, VE>(a, p, row, c0, acc);
     }
 }
 
@@ -380,66 +413,85 @@ struct Launch {
     const ReduceArgs *reduce;
 };
 
-template <typename Tg, typename Te, int VE, int EXTRA, int LPR>
+template <class P, int VE, int LPR>
 hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
     constexpr int64_t kRowsPerBlock = 4 * (64 / LPR);
     if (lk.kind == Kind::kPass) {
         const int64_t n = int64_t(rp.p_end) - rp.p_begin;
         if (n <= 0) return hipSuccess;
-        k_row_pass<Tg, Te, VE, LPR, EXTRA>
-            <<<dim3(unsigned((n + kRowsPerBlock - 1) / kRowsPerBlock)), dim3(256), 0, stream>>>(rp);
+        k_row_pass<P, VE, LPR><<<dim3(unsigned((n + kRowsPerBlock - 1) / kRowsPerBlock)), dim3(256), 0, stream>>>(rp);
     } else if (lk.kind == Kind::kChunk) {
         if (lk.chunk->count <= 0) return hipSuccess;
         constexpr int kBs = (16 * LPR > 256) ? 16 * LPR : 256;
-        k_row_chunk<Tg, Te, VE, LPR, EXTRA, kBs>
-            <<<dim3(unsigned(lk.chunk->count)), dim3(kBs), 0, stream>>>(rp, *lk.chunk);
+        k_row_chunk<P, VE, LPR, kBs><<<dim3(unsigned(lk.chunk->count)), dim3(kBs), 0, stream>>>(rp, *lk.chunk);
     } else {
         const int64_t n = lk.reduce->count;
         if (n <= 0) return hipSuccess;
-        k_row_reduce<Te, VE, LPR, EXTRA>
+        k_row_reduce<P, VE, LPR>
             <<<dim3(unsigned((n + kRowsPerBlock - 1) / kRowsPerBlock)), dim3(256), 0, stream>>>(rp, *lk.reduce);
     }
     return hipGetLastError();
 }
 
-template <typename Tg, typename Te, int VE, int EXTRA>
+template <class P, int VE>
 hipError_t dispatch_lpr(const RowPass &rp, const Launch &lk, int lpr, hipStream_t stream) {
     switch (lpr) {
-        case 1: return launch_one<Tg, Te, VE, EXTRA, 1>(rp, lk, stream);
-        case 2: return launch_one<Tg, Te, VE, EXTRA, 2>(rp, lk, stream);
-        case 4: return launch_one<Tg, Te, VE, EXTRA, 4>(rp, lk, stream);
-        case 8: return launch_one<Tg, Te, VE, EXTRA, 8>(rp, lk, stream);
-        case 16: return launch_one<Tg, Te, VE, EXTRA, 16>(rp, lk, stream);
-        case 32: return launch_one<Tg, Te, VE, EXTRA, 32>(rp, lk, stream);
-        case 64: return launch_one<Tg, Te, VE, EXTRA, 64>(rp, lk, stream);
+        case 1: return launch_one<P, VE, 1>(rp, lk, stream);
+        case 2: return launch_one<P, VE, 2>(rp, lk, stream);
+        case 4: return launch_one<P, VE, 4>(rp, lk, stream);
+        case 8: return launch_one<P, VE, 8>(rp, lk, stream);
+        case 16: return launch_one<P, VE, 16>(rp, lk, stream);
+        case 32: return launch_one<P, VE, 32>(rp, lk, stream);
+        case 64: return launch_one<P, VE, 64>(rp, lk, stream);
         default: return hipErrorInvalidValue;
     }
 }
 
-template <typename Tg, typename Te, int EXTRA>
+// The vector path needs every operand 16-byte aligned with row strides that
+// keep it so; anything else runs one column per lane.
+template <class P>
 hipError_t dispatch_ve(const RowPass &rp, const Launch &lk, hipStream_t stream) {
-    constexpr int kVe = 16 / int(sizeof(Tg));
-    bool vec = (rp.ncols % kVe) == 0 && aligned16(rp.g, rp.ldg, int(sizeof(Tg))) && aligned16(rp.y, rp.ldy, 4);
-    if (EXTRA != kExtraNone) vec = vec && aligned16(rp.e, rp.lde, int(sizeof(Te)));
-    if (vec) return dispatch_lpr<Tg, Te, kVe, EXTRA>(rp, lk, lanes_per_row(rp.ncols, kVe), stream);
-    return dispatch_lpr<Tg, Te, 1, EXTRA>(rp, lk, lanes_per_row(rp.ncols, 1), stream);
+    using TG = typename P::TG;
+    // 16-byte loads of the narrowest gathered type (a dual pass over FP32 z
+    // and BF16 g reads 8 columns per lane: 16 bytes of g, 32 of z).
+    constexpr int kNarrow = (P::kDual && sizeof(typename P::TG2) < sizeof(TG)) ? int(sizeof(typename P::TG2))
+                                                                                  : int(sizeof(TG));
+    constexpr int kVe = 16 / kNarrow;
+    bool vec = (rp.ncols % kVe) == 0 && aligned16(rp.g, rp.ldg, int(sizeof(TG))) && aligned16(rp.y, rp.ldy, 4);
+    if (P::kExtra != kExtraNone) vec = vec && aligned16(rp.e, rp.lde, int(sizeof(typename P::TE)));
+    if (P::kDual) vec = vec && aligned16(rp.g2, rp.ldg2, int(sizeof(typename P::TG2)));
+    if (vec) return dispatch_lpr<P, kVe>(rp, lk, lanes_per_row(rp.ncols, kVe), stream);
+    return dispatch_lpr<P, 1>(rp, lk, lanes_per_row(rp.ncols, 1), stream);
 }
 
+// The passes the library runs (anything else is rejected):
+//   forward Delta pass:  g = X (FP32/BF16), extra none or parent (FP32 y)
+//   operator spmm:       g = rhs (FP32), extra none or self (FP32 rhs)
+//   adjoint reverse:     dual g = Z (FP32) / g2 = G, extra self (G)
+//   adjoint Delta^T:     dual g = Z (FP32) / g2 = G, extra none
 hipError_t dispatch(const RowPass &rp, const Launch &lk, hipStream_t stream) {
-    if (!rp.g_bf16 && (rp.extra == kExtraNone || !rp.e_bf16)) {
-        switch (rp.extra) {
-            case kExtraNone: return dispatch_ve<float, float, kExtraNone>(rp, lk, stream);
-            case kExtraParent: return dispatch_ve<float, float, kExtraParent>(rp, lk, stream);
-            case kExtraSelf: return dispatch_ve<float, float, kExtraSelf>(rp, lk, stream);
+    if (!rp.dual) {
+        if (!rp.g_bf16) {
+            if (rp.extra == kExtraNone) return dispatch_ve<Pass<float, float, float, kExtraNone, false>>(rp, lk, stream);
+            if (rp.e_bf16) return hipErrorInvalidValue;
+            if (rp.extra == kExtraParent)
+                return dispatch_ve<Pass<float, float, float, kExtraParent, false>>(rp, lk, stream);
+            return dispatch_ve<Pass<float, float, float, kExtraSelf, false>>(rp, lk, stream);
         }
-    } else if (rp.g_bf16 && (rp.extra == kExtraNone || !rp.e_bf16)) {
-        switch (rp.extra) {
-            case kExtraNone: return dispatch_ve<bf16_t, float, kExtraNone>(rp, lk, stream);
-            case kExtraParent: return dispatch_ve<bf16_t, float, kExtraParent>(rp, lk, stream);
-            default: break;
-        }
-    } else if (!rp.g_bf16 && rp.e_bf16 && rp.extra == kExtraSelf) {
-        return dispatch_ve<float, bf16_t, kExtraSelf>(rp, lk, stream);
+        if (rp.extra == kExtraNone) return dispatch_ve<Pass<bf16_t, bf16_t, float, kExtraNone, false>>(rp, lk, stream);
+        if (rp.extra == kExtraParent && !rp.e_bf16)
+            return dispatch_ve<Pass<bf16_t, bf16_t, float, kExtraParent, false>>(rp, lk, stream);
+        return hipErrorInvalidValue;
+    }
+    if (rp.g_bf16) return hipErrorInvalidValue;  // the dual pass gathers FP32 Z
+    if (rp.extra == kExtraSelf) {
+        if (!rp.g2_bf16 && !rp.e_bf16) return dispatch_ve<Pass<float, float, float, kExtraSelf, true>>(rp, lk, stream);
+        if (rp.g2_bf16 && rp.e_bf16) return dispatch_ve<Pass<float, bf16_t, bf16_t, kExtraSelf, true>>(rp, lk, stream);
+        return hipErrorInvalidValue;
+    }
+    if (rp.extra == kExtraNone) {
+        if (!rp.g2_bf16) return dispatch_ve<Pass<float, float, float, kExtraNone, true>>(rp, lk, stream);
+        return dispatch_ve<Pass<float, bf16_t, float, kExtraNone, true>>(rp, lk, stream);
     }
     return hipErrorInvalidValue;
 }

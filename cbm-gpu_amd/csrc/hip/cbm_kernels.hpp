@@ -33,6 +33,14 @@ struct RowPass {
     const void *g = nullptr;          // gathered matrix
     int64_t ldg = 0;
     bool g_bf16 = false;
+    // Dual pass: the entries [ptr[p], mid[p]) read g2 (scaled by
+    // g2scale[row]), the entries [mid[p], end) read g.
+    bool dual = false;
+    const int32_t *mid = nullptr;
+    const void *g2 = nullptr;
+    int64_t ldg2 = 0;
+    bool g2_bf16 = false;
+    const float *g2scale = nullptr;
     const void *e = nullptr;          // extra-term matrix
     int64_t lde = 0;
     bool e_bf16 = false;

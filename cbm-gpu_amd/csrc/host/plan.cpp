@@ -231,11 +231,24 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
             bptr.assign(size_t(n) + 1, 0);
             for (int32_t p = 0; p < n; ++p) bptr[size_t(p) + 1] = bptr[size_t(p)] + ccount[size_t(bnode[size_t(p)])];
             brow.resize(size_t(nnz));
-            std::vector<int32_t> fill(bptr.begin(), bptr.end() - 1);
+            // Rows of leaves first, then rows of internal nodes: the adjoint
+            // reads a leaf's term from G and an internal node's from Z, and
+            // this split keeps both loops free of per-entry branches.
+            auto internal = [&](int32_t r) { return kptr[size_t(r) + 1] > kptr[size_t(r)]; };
+            std::vector<int32_t> n_leaf(static_cast<size_t>(n), 0);
             for (int32_t r = 0; r < n; ++r)
+                if (!internal(r))
+                    for (int32_t k = crow[r]; k < crow[r + 1]; ++k) ++n_leaf[size_t(pos_of[size_t(col[k])])];
+            auto &bmid = plan->arr[CBM_P_BWD_MID];
+            bmid.resize(size_t(n));
+            std::vector<int32_t> fill_leaf(bptr.begin(), bptr.end() - 1), fill_int(static_cast<size_t>(n));
+            for (int32_t p = 0; p < n; ++p) bmid[size_t(p)] = fill_int[size_t(p)] = bptr[size_t(p)] + n_leaf[size_t(p)];
+            for (int32_t r = 0; r < n; ++r) {
+                auto &fill = internal(r) ? fill_int : fill_leaf;
                 for (int32_t k = crow[r]; k < crow[r + 1]; ++k)
                     brow[size_t(fill[size_t(pos_of[size_t(col[k])])]++)] =
                         int32_t(uint32_t(r) | (val[k] < 0.0f ? kSignBit : 0u));
+            }
             Split b;
             plan->stat[CBM_PS_BWD_SLOTS] = split_long_rows(bptr, 0, n_long, b);
             plan->stat[CBM_PS_BWD_NLONG] = n_long;
@@ -247,19 +260,44 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
             plan->arr[CBM_P_BRED_CNT].swap(b.red_cnt);
         }
 
-        // Children per level position.
-        auto &cptr = plan->arr[CBM_P_CH_PTR];
-        auto &cnode = plan->arr[CBM_P_CH_NODE];
-        cptr.assign(size_t(n) + 1, 0);
-        for (int32_t p = 0; p < n; ++p) {
-            const int32_t u = lvl_node[size_t(p)];
-            cptr[size_t(p) + 1] = cptr[size_t(p)] + (kptr[size_t(u) + 1] - kptr[size_t(u)]);
-        }
-        cnode.resize(size_t(n_edges));
+        // Internal nodes (those with children) by level, and their children;
+        // a child that has children of its own carries the internal bit.
+        {
+            auto &rptr = plan->arr[CBM_P_REV_PTR];
+            auto &rnode = plan->arr[CBM_P_REV_NODE];
+            auto &cptr = plan->arr[CBM_P_CH_PTR];
+            auto &cnode = plan->arr[CBM_P_CH_NODE];
+            auto has_kids = [&](int32_t u) { return kptr[size_t(u) + 1] > kptr[size_t(u)]; };
+            rptr.assign(size_t(levels) + 1, 0);
+            for (int32_t v = 0; v < n; ++v)
+                if (has_kids(v)) ++rptr[size_t(depth[size_t(v)]) + 1];
+            std::partial_sum(rptr.begin(), rptr.end(), rptr.begin());
+            rnode.resize(size_t(rptr[size_t(levels)]));
+            {
+                std::vector<int32_t> fill(rptr.begin(), rptr.end() - 1);
+                for (int32_t v = 0; v < n; ++v)
+                    if (has_kids(v)) rnode[size_t(fill[size_t(depth[size_t(v)])]++)] = v;
+            }
+            auto &cmid = plan->arr[CBM_P_CH_MID];
+            cptr.assign(rnode.size() + 1, 0);
+            cmid.resize(rnode.size());
+            for (size_t i = 0; i < rnode.size(); ++i) {
+                const int32_t u = rnode[i];
+                cptr[i + 1] = cptr[i] + (kptr[size_t(u) + 1] - kptr[size_t(u)]);
+            }
+            cnode.resize(size_t(n_edges));
 #pragma omp parallel for schedule(static)
-        for (int32_t p = 0; p < n; ++p) {
-            const int32_t u = lvl_node[size_t(p)];
-            std::copy(kid.begin() + kptr[size_t(u)], kid.begin() + kptr[size_t(u) + 1], cnode.begin() + cptr[size_t(p)]);
+            for (int64_t i = 0; i < int64_t(rnode.size()); ++i) {
+                const int32_t u = rnode[size_t(i)];
+                int32_t o = cptr[size_t(i)];
+                for (int pass = 0; pass < 2; ++pass) {  // leaf children, then internal ones
+                    if (pass == 1) cmid[size_t(i)] = o;
+                    for (int32_t k = kptr[size_t(u)]; k < kptr[size_t(u) + 1]; ++k) {
+                        const int32_t v = kid[size_t(k)];
+                        if (has_kids(v) == (pass == 1)) cnode[size_t(o++)] = v;
+                    }
+                }
+            }
         }
         *out = plan.release();
         return CBM_OK;

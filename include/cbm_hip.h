@@ -65,18 +65,21 @@ typedef struct cbm_desc {
     int64_t n_edges;
     int32_t mode;
     /* host: level offsets [L+1], long rows per level [L], chunk offsets [L+1],
-       reduction-row offsets [L+1] */
+       reduction-row offsets [L+1], internal-node offsets [L+1] */
     const int32_t *lvl_ptr_host;
     const int32_t *lvl_nlong_host;
     const int32_t *fck_ptr_host;
     const int32_t *fred_ptr_host;
-    /* device: level order, forward Delta, children */
+    const int32_t *rev_ptr_host;
+    /* device: level order, forward Delta; internal nodes and their children */
     const int32_t *lvl_node;
     const int32_t *lvl_parent;
     const int32_t *fwd_ptr;
     const int32_t *fwd_col;
+    const int32_t *rev_node;
     const int32_t *ch_ptr;
     const int32_t *ch_node;
+    const int32_t *ch_mid;
     /* device: forward long-row chunks and reductions */
     const int32_t *fck_pos;
     const int32_t *fck_meta;
@@ -89,6 +92,7 @@ typedef struct cbm_desc {
     const int32_t *bwd_node;
     const int32_t *bwd_ptr;
     const int32_t *bwd_row;
+    const int32_t *bwd_mid;
     const int32_t *bck_pos;
     const int32_t *bck_meta;
     const int32_t *bck_slot;
@@ -139,12 +143,24 @@ int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *
 
 /*
  * Adjoint dX = M^T G.  z is caller-provided FP32 workspace N x n_cols with
- * leading dimension ldz (it receives (I-P)^-T applied to the scaled G).
+ * leading dimension ldz; it receives (I-P)^-T applied to the scaled G for
+ * the nodes that have children (for a leaf that row is just the scaled G,
+ * which the kernels read from g directly; z's leaf rows are left untouched).
  * No reference counterpart on the GPU; the CPU op s_spmm_csr_t_int32
  * (cbm/cbm_extensions.cpp:742-782) is the closest.
  */
 int cbm_matmul_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, float *dx, int64_t lddx, float *z,
                  int64_t ldz, int32_t n_cols, hipStream_t stream);
+
+/*
+ * The two halves of the adjoint, for separate timing: cbm_adjoint_reverse
+ * writes z (the reverse tree gather), cbm_adjoint_delta_t then writes
+ * dx = M^T g from z and g.  Together they equal cbm_matmul_t.
+ */
+int cbm_adjoint_reverse(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, float *z, int64_t ldz,
+                        int32_t n_cols, hipStream_t stream);
+int cbm_adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, const float *z, int64_t ldz,
+                        float *dx, int64_t lddx, int32_t n_cols, hipStream_t stream);
 
 /*
  * The two halves of the forward product, for separate timing and for the
@@ -159,6 +175,16 @@ int cbm_propagate(cbm_handle *h, float *y, int64_t ldy, int32_t n_cols, hipStrea
 
 /* Per-kernel launch counts of the last call on this handle (for profiling). */
 int cbm_last_launches(const cbm_handle *h);
+
+/*
+ * Column slabs: cbm_matmul, cbm_matmul_t and cbm_delta_spmm process the
+ * columns in slabs whose gathered part (N x width x element size) is at most
+ * `bytes`, so that the random row gathers of one slab keep hitting the
+ * 256 MB last-level cache.  0 (the default) processes whole rows, which
+ * measured faster at every slab size on MI355X: partial-row gathers cost
+ * more than the cache residency returns.
+ */
+int cbm_set_slab_bytes(cbm_handle *h, int64_t bytes);
 
 /* ------------------------------------------------------------------------- */
 /* CSR comparator on the same GPU: rocSPARSE SpMM, FP32.                      */

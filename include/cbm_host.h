@@ -131,9 +131,10 @@ enum {
     CBM_P_FWD_PTR = 3,     /* int32 [N+1]  Delta row offsets, rows in level order                   */
     CBM_P_FWD_COL = 4,     /* int32 [nnz]  Delta columns, bit 31 set for a -1 entry                  */
     CBM_P_BWD_PTR = 5,     /* int32 [N+1]  Delta^T row offsets, rows in BWD_NODE order              */
-    CBM_P_BWD_ROW = 6,     /* int32 [nnz]  Delta rows per column, bit 31 set for a -1 entry          */
-    CBM_P_CH_PTR = 7,      /* int32 [N+1]  child offsets per level position                           */
-    CBM_P_CH_NODE = 8,     /* int32 [E]    children (node ids) of the node at each position           */
+    CBM_P_BWD_ROW = 6,     /* int32 [nnz]  Delta rows per column, bit 31 set for a -1 entry; rows of
+                                           leaves (no children) first, then rows of internal nodes    */
+    CBM_P_CH_PTR = 7,      /* int32 [I+1]  child offsets per internal-node position                   */
+    CBM_P_CH_NODE = 8,     /* int32 [E]    children (node ids): leaves first, then internal nodes     */
     CBM_P_LVL_NLONG = 9,   /* int32 [L]    long rows at the start of each level                       */
     CBM_P_FCK_PTR = 10,    /* int32 [L+1]  forward chunk offsets per level                            */
     CBM_P_FCK_POS = 11,    /* int32 [C]    level position of each forward chunk                       */
@@ -150,7 +151,11 @@ enum {
     CBM_P_BRED_POS = 22,   /* int32 [Rb]   backward rows that need a reduction                        */
     CBM_P_BRED_SLOT = 23,  /* int32 [Rb]   first slot                                                 */
     CBM_P_BRED_CNT = 24,   /* int32 [Rb]   number of slots                                            */
-    CBM_P_N_ARRAYS = 25
+    CBM_P_REV_PTR = 25,    /* int32 [L+1]  offsets per level into REV_NODE                                 */
+    CBM_P_REV_NODE = 26,   /* int32 [I]    internal nodes (those with children) by level, id ascending     */
+    CBM_P_BWD_MID = 27,    /* int32 [N]    first internal-node entry of each backward row              */
+    CBM_P_CH_MID = 28,     /* int32 [I]    first internal child of each internal-node position         */
+    CBM_P_N_ARRAYS = 29
 };
 
 /* Scalars of a cbm_plan, addressed by cbm_plan_stat(). */

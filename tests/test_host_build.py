@@ -176,16 +176,33 @@ def test_plan_invariants():
     assert sorted(bnode.tolist()) == list(range(n))
     delta = sp.csr_matrix((vals.astype(float), col, ptr), shape=(n, n)).tocsc()
     delta.sort_indices()
+    has_kids = np.bincount(b.parent.numpy()[b.parent.numpy() >= 0], minlength=n) > 0
+    bmid = pl.bwd_mid.numpy()
     for p in range(0, n, 5):
         j = bnode[p]
-        got = br[bp[p]:bp[p + 1]]
-        assert np.array_equal(got & 0x7FFFFFFF, delta.indices[delta.indptr[j]:delta.indptr[j + 1]])
-        assert np.array_equal((got >> 31).astype(bool), delta.data[delta.indptr[j]:delta.indptr[j + 1]] < 0)
-    # children per position
-    cp, cn = pl.ch_ptr.numpy(), pl.ch_node.numpy()
-    for p in range(n):
-        kids = cn[cp[p]:cp[p + 1]]
-        assert np.all(b.parent.numpy()[kids] == node[p])
+        got = br[bp[p]:bp[p + 1]].astype(np.int64)
+        rows = got & 0x7FFFFFFF
+        signs = got >> 31
+        want_rows = delta.indices[delta.indptr[j]:delta.indptr[j + 1]]
+        want_neg = delta.data[delta.indptr[j]:delta.indptr[j + 1]] < 0
+        # leaves first, then internal nodes, each part ascending
+        m = bmid[p] - bp[p]
+        assert not has_kids[rows[:m]].any() and has_kids[rows[m:]].all()
+        assert np.all(np.diff(rows[:m]) > 0) and np.all(np.diff(rows[m:]) > 0)
+        order = np.argsort(rows)
+        assert np.array_equal(rows[order], want_rows)
+        assert np.array_equal(signs[order].astype(bool), want_neg)
+    # internal nodes by level and their children: leaf children first
+    rp, rn = pl.rev_ptr.numpy(), pl.rev_node.numpy()
+    assert sorted(rn.tolist()) == sorted(np.nonzero(has_kids)[0].tolist())
+    for l in range(pl.depth):
+        assert np.all(level[rn[rp[l]:rp[l + 1]]] == l)
+    cp, cn, cm = pl.ch_ptr.numpy(), pl.ch_node.numpy(), pl.ch_mid.numpy()
+    for i, u in enumerate(rn):
+        kids = cn[cp[i]:cp[i + 1]]
+        assert len(kids) > 0 and np.all(b.parent.numpy()[kids] == u)
+        m = cm[i] - cp[i]
+        assert not has_kids[kids[:m]].any() and has_kids[kids[m:]].all()
     assert cp[-1] == b.stats["tree_edges"]
 
 
