@@ -196,6 +196,22 @@ def measure(name, w, ei, m, args, dev, stream):
     return rec, out, (x32, g32, x16, g16)
 
 
+def profile_steps(w, m, args, dev, stream):
+    """The CBM step alone (BF16 then FP32 storage), for kernel traces and
+    counter runs: args.steps timed steps after args.warmup, per dtype."""
+    import torch
+    h = m.cusparse_extension
+    n, f = w["n"], w["cols"]
+    gen = torch.Generator(device=dev).manual_seed(2005)
+    x32 = torch.randn(n, f, device=dev, generator=gen)
+    g32 = torch.randn(n, f, device=dev, generator=gen)
+    y, dx, z = (torch.empty(n, f, device=dev) for _ in range(3))
+    h.reserve(f)
+    for tag, xx, gg in (("bf16", x32.to(torch.bfloat16), g32.to(torch.bfloat16)), ("fp32", x32, g32)):
+        ms = time_loop(lambda: (h.forward(xx, y), h.matmul_t(gg, dx, workspace=z)), args.steps, args.warmup, stream)
+        log(f"profile {tag}: {ms:.3f} ms per step ({args.steps} steps)")
+
+
 def _csr_of(ei, n):
     import torch
     crow = torch.zeros(n + 1, dtype=torch.int64)
@@ -252,6 +268,10 @@ def main():
     ap.add_argument("--cols", type=int, default=None)
     ap.add_argument("--secondary", default="c4", help="second workload reported under 'secondary' ('' for none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile", action="store_true",
+                    help="run only the CBM steps (BF16, then FP32) for a profiler; prints their times")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run one plain device copy of MIB MiB (the calibration kernel of counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the CBM product here runs on a single MI355X; "
@@ -261,8 +281,22 @@ def main():
     import torch
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
+    if args.copy_probe > 0:
+        # A copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch give the
+        # counters' scale against bytes (read once, written once).
+        n_floats = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n_floats, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+        del src, dst
 
     w, ei, m = build_workload(args.workload, args.alpha, args.cols, dev)
+    if args.profile:
+        profile_steps(w, m, args, dev, stream)
+        return
     rec, gpu_out, inputs = measure(args.workload, w, ei, m, args, dev, stream)
     cpu, errors = None, None
     if not args.no_cpu:

@@ -23,6 +23,7 @@ class CbmDesc(ctypes.Structure):
         ("rev_ptr_host", c_vp),
         ("lvl_node", c_vp),
         ("lvl_parent", c_vp),
+        ("lvl_ppos", c_vp),
         ("fwd_ptr", c_vp),
         ("fwd_col", c_vp),
         ("rev_node", c_vp),
@@ -54,7 +55,7 @@ class CbmDesc(ctypes.Structure):
     ]
 
 # Device arrays of the plan that the descriptor points at.
-DEVICE_ARRAYS = ("lvl_node", "lvl_parent", "fwd_ptr", "fwd_col", "rev_node", "ch_ptr", "ch_node", "ch_mid", "fck_pos", "fck_meta",
+DEVICE_ARRAYS = ("lvl_node", "lvl_parent", "lvl_ppos", "fwd_ptr", "fwd_col", "rev_node", "ch_ptr", "ch_node", "ch_mid", "fck_pos", "fck_meta",
                  "fck_slot", "fred_pos", "fred_slot", "fred_cnt", "bwd_node", "bwd_ptr", "bwd_row", "bwd_mid", "bck_pos",
                  "bck_meta", "bck_slot", "bred_pos", "bred_slot", "bred_cnt")
 HOST_ARRAYS = ("lvl_ptr", "lvl_nlong", "fck_ptr", "fred_ptr", "rev_ptr")
@@ -75,6 +76,7 @@ def declare(lib, sig):
     sig(lib, "cbm_propagate", c_i32, c_vp, c_vp, c_i64, c_i32, c_vp)
     sig(lib, "cbm_last_launches", c_i32, c_vp)
     sig(lib, "cbm_set_slab_bytes", c_i32, c_vp, c_i64)
+    sig(lib, "cbm_set_kahan_depth", c_i32, c_vp, c_i32)
     sig(lib, "csr_last_error", ctypes.c_char_p)
     sig(lib, "csr_create", c_i32, c_i32, c_i32, c_i64, c_vp, c_vp, c_vp, P_vp)
     sig(lib, "csr_destroy", c_i32, c_vp)
@@ -90,5 +92,5 @@ def declare(lib, sig):
 # Every symbol include/cbm_hip.h declares (checked by the CPU test suite).
 EXPORTED = ("cbm_last_error", "cbm_desc_size", "cbm_create", "cbm_destroy", "cbm_reserve", "cbm_matmul", "cbm_matmul_t", "cbm_adjoint_reverse",
             "cbm_adjoint_delta_t", "cbm_delta_spmm",
-            "cbm_propagate", "cbm_last_launches", "cbm_set_slab_bytes", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
+            "cbm_propagate", "cbm_last_launches", "cbm_set_slab_bytes", "cbm_set_kahan_depth", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
             "csr_last_error", "cbm_op_spmm", "cbm_op_update", "cbm_op_axpy", "cbm_op_last_error")

@@ -282,6 +282,12 @@ class cusparseCBM:
         columns allocate nothing (needed before HIP-graph capture)."""
         _lib.check_hip(self._lib.cbm_reserve(self._h, int(n_cols)), "reserve")
 
+    def set_kahan_depth(self, level):
+        """Tree levels from `level` on propagate with an FP32 compensation
+        term (default 32; None or a negative value switches it off)."""
+        lvl = -1 if level is None or level < 0 else int(level)
+        _lib.check_hip(self._lib.cbm_set_kahan_depth(self._h, lvl), "set_kahan_depth")
+
     def set_slab_bytes(self, nbytes):
         """Column-slab size for cache residency of the gathers (0: whole rows)."""
         _lib.check_hip(self._lib.cbm_set_slab_bytes(self._h, int(nbytes)), "set_slab_bytes")

@@ -20,6 +20,9 @@ struct cbm_handle {
     float *scratch = nullptr; // partial sums of long-row chunks
     int64_t scratch_floats = 0;
     int64_t slab_bytes = 0;   // column slab target (gathered bytes), 0 = whole rows
+    int32_t kahan_level = 32; // levels from here on propagate with compensation
+    float *comp = nullptr;    // compensation rows of those levels
+    int64_t comp_floats = 0;
     int last_launches = 0;
 };
 
@@ -73,7 +76,31 @@ int check_dense(const cbm_handle *h, int32_t dtype, const void *in, int64_t ldin
     return CBM_OK;
 }
 
+int32_t comp_base(const cbm_handle *h) {
+    return h->kahan_level >= 0 && h->kahan_level < h->d.n_levels ? h->lvl_ptr[size_t(h->kahan_level)] : h->d.n_rows;
+}
+
+int ensure_comp(cbm_handle *h, int32_t n_cols) {
+    const int64_t need = int64_t(h->d.n_rows - comp_base(h)) * n_cols;
+    if (need <= h->comp_floats) return CBM_OK;
+    if (h->comp) {
+        const hipError_t e = hipFree(h->comp);
+        h->comp = nullptr;
+        h->comp_floats = 0;
+        if (e != hipSuccess) return device_error(e, "cbm compensation rows");
+    }
+    const hipError_t e = hipMalloc(&h->comp, sizeof(float) * size_t(need));
+    if (e != hipSuccess) {
+        h->comp = nullptr;
+        return device_error(e, "cbm compensation rows");
+    }
+    h->comp_floats = need;
+    return CBM_OK;
+}
+
 int ensure_scratch(cbm_handle *h, int32_t n_cols) {
+    const int rc = ensure_comp(h, n_cols);
+    if (rc != CBM_OK) return rc;
     const int64_t slots = std::max<int64_t>(h->d.fwd_slots, h->d.bwd_slots);
     const int64_t need = slots * int64_t(n_cols);
     if (need <= h->scratch_floats) return CBM_OK;
@@ -152,8 +179,14 @@ int forward_levels(cbm_handle *h, cbmk::RowPass rp, bool with_parent, hipStream_
     rd.slot = h->d.fred_slot;
     rd.cnt = h->d.fred_cnt;
     rd.scratch = h->scratch;
+    rp.parent_pos = h->d.lvl_ppos;
+    rp.comp = h->comp;
+    rp.comp_base = comp_base(h);
+    rp.ldc = rp.ncols;
     for (int32_t l = 0; l < h->d.n_levels; ++l) {
-        rp.extra = (with_parent && l > 0) ? cbmk::kExtraParent : cbmk::kExtraNone;
+        rp.extra = (with_parent && l > 0)
+                       ? (l >= h->kahan_level && h->kahan_level >= 0 ? cbmk::kExtraParentKahan : cbmk::kExtraParent)
+                       : cbmk::kExtraNone;
         ck.first = h->fck_ptr[size_t(l)];
         ck.count = h->fck_ptr[size_t(l) + 1] - ck.first;
         rd.first = h->fred_ptr[size_t(l)];
@@ -202,7 +235,7 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
     const cbm_desc &d = *desc;
     const bool tables = d.lvl_ptr_host && d.lvl_nlong_host && d.fck_ptr_host && d.fred_ptr_host && d.rev_ptr_host;
     if (d.n_rows < 1 || d.n_levels < 1 || d.nnz < 0 || d.n_edges < 0 || d.n_edges >= d.n_rows || !tables ||
-        !d.lvl_node || !d.lvl_parent || !d.fwd_ptr || !d.bwd_ptr || !d.bwd_node || !d.bwd_mid || !d.ch_ptr ||
+        !d.lvl_node || !d.lvl_parent || !d.lvl_ppos || !d.fwd_ptr || !d.bwd_ptr || !d.bwd_node || !d.bwd_mid || !d.ch_ptr ||
         (d.nnz > 0 && (!d.fwd_col || !d.bwd_row)) || (d.n_edges > 0 && (!d.ch_node || !d.rev_node || !d.ch_mid)) ||
         d.fwd_slots < 0 ||
         d.bwd_slots < 0 || d.n_bck < 0 || d.n_bred < 0 || d.bwd_nlong < 0 || d.bwd_nlong > d.n_rows ||
@@ -272,7 +305,7 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
 extern "C" int cbm_destroy(cbm_handle *h) {
     if (!h) return CBM_OK;
     hipError_t e = hipSuccess;
-    for (float *p : {h->ps_pos, h->scratch}) {
+    for (float *p : {h->ps_pos, h->scratch, h->comp}) {
         if (!p) continue;
         const hipError_t ei = hipFree(p);
         if (e == hipSuccess) e = ei;
@@ -461,6 +494,15 @@ extern "C" int cbm_propagate(cbm_handle *h, float *y, int64_t ldy, int32_t n_col
 }
 
 extern "C" int cbm_last_launches(const cbm_handle *h) { return h ? h->last_launches : -1; }
+
+extern "C" int cbm_set_kahan_depth(cbm_handle *h, int32_t level) {
+    if (!h) {
+        set_error("cbm_set_kahan_depth: null handle");
+        return CBM_ERR_ARG;
+    }
+    h->kahan_level = level < 0 ? -1 : level;
+    return CBM_OK;
+}
 
 extern "C" int cbm_set_slab_bytes(cbm_handle *h, int64_t bytes) {
     if (!h || bytes < 0) {

@@ -226,6 +226,39 @@ __device__ inline void finish_row(const RowPass &a, int32_t p, int32_t row, int3
         Load<TE, VE>::run(static_cast<const TE *>(a.e) + int64_t(row) * a.lde + c0, v);
 #pragma unroll
         for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], es, acc[q]);
+    } else if constexpr (P::kExtra == kExtraParentKahan) {
+        // y = acc + es * y_par + es * c_par, kept as the pair (y, c) with
+        // y the rounded sum and c its error: TwoProduct for es * y_par,
+        // TwoSum for the addition (Knuth), Fast2Sum to fold the error back.
+        float comp[VE];
+#pragma unroll
+        for (int q = 0; q < VE; ++q) comp[q] = 0.0f;
+        const int32_t par = a.parent_of[p];
+        if (par >= 0) {
+            const float es = (a.escale ? a.escale[p] : 1.0f) * a.ecoef;
+            float v[VE], cp[VE];
+            Load<TE, VE>::run(static_cast<const TE *>(a.e) + int64_t(par) * a.lde + c0, v);
+            const int32_t pp = a.parent_pos[p];
+            if (pp >= a.comp_base) {
+                Load<float, VE>::run(a.comp + int64_t(pp - a.comp_base) * a.ldc + c0, cp);
+            } else {
+#pragma unroll
+                for (int q = 0; q < VE; ++q) cp[q] = 0.0f;
+            }
+#pragma unroll
+            for (int q = 0; q < VE; ++q) {
+                const float t = __fmul_rn(es, v[q]);
+                const float t_err = fmaf(es, v[q], -t);
+                const float s = __fadd_rn(acc[q], t);
+                const float bb = __fsub_rn(s, acc[q]);
+                const float e = __fadd_rn(__fsub_rn(acc[q], __fsub_rn(s, bb)), __fsub_rn(t, bb));
+                const float err = __fadd_rn(e, __fadd_rn(t_err, __fmul_rn(es, cp[q])));
+                const float yq = __fadd_rn(s, err);
+                comp[q] = __fsub_rn(err, __fsub_rn(yq, s));
+                acc[q] = yq;
+            }
+        }
+        store_row<VE>(a.comp + int64_t(p - a.comp_base) * a.ldc + c0, comp);
     }
     store_row<VE>(a.y + int64_t(row) * a.ldy + c0, acc);
 }
@@ -460,6 +493,7 @@ hipError_t dispatch_ve(const RowPass &rp, const Launch &lk, hipStream_t stream) 
     bool vec = (rp.ncols % kVe) == 0 && aligned16(rp.g, rp.ldg, int(sizeof(TG))) && aligned16(rp.y, rp.ldy, 4);
     if (P::kExtra != kExtraNone) vec = vec && aligned16(rp.e, rp.lde, int(sizeof(typename P::TE)));
     if (P::kDual) vec = vec && aligned16(rp.g2, rp.ldg2, int(sizeof(typename P::TG2)));
+    if (P::kExtra == kExtraParentKahan) vec = vec && aligned16(rp.comp, rp.ldc, 4);
     if (vec) return dispatch_lpr<P, kVe>(rp, lk, lanes_per_row(rp.ncols, kVe), stream);
     return dispatch_lpr<P, 1>(rp, lk, lanes_per_row(rp.ncols, 1), stream);
 }
@@ -476,11 +510,15 @@ hipError_t dispatch(const RowPass &rp, const Launch &lk, hipStream_t stream) {
             if (rp.e_bf16) return hipErrorInvalidValue;
             if (rp.extra == kExtraParent)
                 return dispatch_ve<Pass<float, float, float, kExtraParent, false>>(rp, lk, stream);
+            if (rp.extra == kExtraParentKahan)
+                return dispatch_ve<Pass<float, float, float, kExtraParentKahan, false>>(rp, lk, stream);
             return dispatch_ve<Pass<float, float, float, kExtraSelf, false>>(rp, lk, stream);
         }
         if (rp.extra == kExtraNone) return dispatch_ve<Pass<bf16_t, bf16_t, float, kExtraNone, false>>(rp, lk, stream);
         if (rp.extra == kExtraParent && !rp.e_bf16)
             return dispatch_ve<Pass<bf16_t, bf16_t, float, kExtraParent, false>>(rp, lk, stream);
+        if (rp.extra == kExtraParentKahan && !rp.e_bf16)
+            return dispatch_ve<Pass<bf16_t, bf16_t, float, kExtraParentKahan, false>>(rp, lk, stream);
         return hipErrorInvalidValue;
     }
     if (rp.g_bf16) return hipErrorInvalidValue;  // the dual pass gathers FP32 Z

@@ -11,7 +11,9 @@ namespace cbmk {
 enum Extra : int32_t {
     kExtraNone = 0,    // y_row = rowscale * sum
     kExtraParent = 1,  // y_row = rowscale * sum + escale[p] * e[parent_of[p]]   (forward level pass)
-    kExtraSelf = 2     // y_row = sum + escale[row] * e[row]                      (adjoint reverse pass)
+    kExtraSelf = 2,    // y_row = sum + escale[row] * e[row]                      (adjoint reverse pass)
+    kExtraParentKahan = 3  // as kExtraParent, carrying a compensation row per position
+                           // (comp) so that deep levels add no rounding drift
 };
 
 // One row pass: for every position p in [p_begin, p_end), with row = node_of[p]
@@ -46,6 +48,12 @@ struct RowPass {
     bool e_bf16 = false;
     Extra extra = kExtraNone;
     float ecoef = 1.0f;               // extra term multiplier, on top of escale
+    // kExtraParentKahan: comp row of position p is comp[(p - comp_base) * ldc];
+    // parent_pos[p] is the parent's position (its comp row if >= comp_base).
+    const int32_t *parent_pos = nullptr;
+    float *comp = nullptr;
+    int32_t comp_base = 0;
+    int64_t ldc = 0;
     float *y = nullptr;
     int64_t ldy = 0;
     int32_t ncols = 0;

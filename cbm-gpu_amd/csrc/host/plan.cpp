@@ -162,6 +162,14 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
                 lvl_node[size_t(fill[size_t(2 * depth[size_t(v)] + (is_long(v) ? 0 : 1))]++)] = v;
         }
         for (int32_t p = 0; p < n; ++p) lvl_parent[size_t(p)] = parent[size_t(lvl_node[size_t(p)])];
+        {
+            auto &ppos = plan->arr[CBM_P_LVL_PPOS];
+            std::vector<int32_t> pos_of_node(static_cast<size_t>(n));
+            for (int32_t p = 0; p < n; ++p) pos_of_node[size_t(lvl_node[size_t(p)])] = p;
+            ppos.resize(size_t(n));
+            for (int32_t p = 0; p < n; ++p)
+                ppos[size_t(p)] = lvl_parent[size_t(p)] < 0 ? -1 : pos_of_node[size_t(lvl_parent[size_t(p)])];
+        }
 
         // Forward Delta in level order with packed signs.
         auto &fptr = plan->arr[CBM_P_FWD_PTR];

@@ -74,6 +74,7 @@ typedef struct cbm_desc {
     /* device: level order, forward Delta; internal nodes and their children */
     const int32_t *lvl_node;
     const int32_t *lvl_parent;
+    const int32_t *lvl_ppos;
     const int32_t *fwd_ptr;
     const int32_t *fwd_col;
     const int32_t *rev_node;
@@ -175,6 +176,15 @@ int cbm_propagate(cbm_handle *h, float *y, int64_t ldy, int32_t n_cols, hipStrea
 
 /* Per-kernel launch counts of the last call on this handle (for profiling). */
 int cbm_last_launches(const cbm_handle *h);
+
+/*
+ * Compensated propagation: the forward product's levels from `level` on
+ * (default 32; negative switches it off) carry an FP32 compensation term per
+ * row (TwoSum / TwoProduct on the parent add), so rounding error no longer
+ * grows with tree depth there.  It costs one extra row read and write per
+ * node at those levels and a buffer of (nodes at those levels) x n_cols FP32.
+ */
+int cbm_set_kahan_depth(cbm_handle *h, int32_t level);
 
 /*
  * Column slabs: cbm_matmul, cbm_matmul_t and cbm_delta_spmm process the

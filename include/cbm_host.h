@@ -155,7 +155,8 @@ enum {
     CBM_P_REV_NODE = 26,   /* int32 [I]    internal nodes (those with children) by level, id ascending     */
     CBM_P_BWD_MID = 27,    /* int32 [N]    first internal-node entry of each backward row              */
     CBM_P_CH_MID = 28,     /* int32 [I]    first internal child of each internal-node position         */
-    CBM_P_N_ARRAYS = 29
+    CBM_P_LVL_PPOS = 29,   /* int32 [N]    level position of the parent of each position, -1 for level 0 */
+    CBM_P_N_ARRAYS = 30
 };
 
 /* Scalars of a cbm_plan, addressed by cbm_plan_stat(). */
