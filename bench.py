@@ -41,6 +41,19 @@ WORKLOADS = {
 ROCSPARSE_ALGS = {"default": 0, "csr_row_split": 4, "csr_nnz_split": 5, "csr_merge_path": 9}
 
 
+def measured_traffic(workload, phase):
+    """HBM bytes per call of a phase from the committed counter summary
+    (profiles/hbm_traffic.json, written from rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs scaled by a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+        return data[workload][phase]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -174,7 +187,7 @@ def measure(name, w, ei, m, args, dev, stream):
     n_internal = int(h._dev["rev_node"].numel())
     b16 = phase_bytes(n, h.nnz, h.n_edges, n_internal, h.depth, f, 2)
     b32 = phase_bytes(n, h.nnz, h.n_edges, n_internal, h.depth, f, 4)
-    dom = max(("forward", "adjoint_reverse", "adjoint_delta_t"), key=lambda k: ph[k])
+    dom = "adjoint_delta_t"  # the step's largest single kernel launch is this pass's short-row kernel
     dom_gbs = b16[dom] / (ph[dom] * 1e-3) / 1e9
     rec = {
         "workload": f"{name}: cbm4mm fwd+bwd, N={n}, nnz(A)={nnz}, alpha={w['alpha']}, F={f}",
@@ -194,6 +207,44 @@ def measure(name, w, ei, m, args, dev, stream):
         "dominant_phase": dom, "dominant_gbs": round(dom_gbs, 1),
     }
     return rec, out, (x32, g32, x16, g16)
+
+
+def gcn_step(args, dev, stream):
+    """Training step of a 2-layer GCN over D^-1/2 A D^-1/2 (hidden 256, FP32)
+    on the c3 graph (N = 169,343, alpha = 16): forward, loss <out, G>, both
+    adjoints, SGD update.  CBM kernels vs the rocSPARSE CSR product."""
+    import torch
+    from cbm import structure
+    from cbm.cbm4dad import cbm4dad
+    from cbm.csr4dad import csr4dad
+    from cbm.gcn import GCN
+    w = WORKLOADS["c3"]
+    n = w["n"]
+    ei = structure.copying_model(n, w["blocks"], w["p"], w["lam"], w["seed_deg"], w["seed"])
+    gen = torch.Generator(device=dev).manual_seed(2003)
+    x = torch.randn(n, 128, device=dev, generator=gen)
+    g = torch.randn(n, 256, device=dev, generator=gen)
+    res = {}
+    for tag, adj in (("cbm", cbm4dad(ei, torch.ones(ei.shape[1]), alpha=w["alpha"], device=dev)),
+                     ("rocsparse_csr", csr4dad(ei, num_nodes=n, device=dev))):
+        model = GCN(adj, 128, 256, 256, seed=5, device=dev)
+        opt = torch.optim.SGD(model.parameters(), lr=1e-4)
+
+        def step():
+            opt.zero_grad(set_to_none=False)
+            loss = (model(x) * g).sum()
+            loss.backward()
+            opt.step()
+
+        res[tag] = time_loop(step, args.steps, args.warmup, stream)
+        log(f"gcn {tag}: {res[tag]:.3f} ms per training step")
+    out = {"metric": "2-layer GCN training step ms (cbm4dad fwd + adjoint, FP32)", "value": round(res["cbm"], 4),
+           "unit": "ms", "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "config": {"workload": f"c3: N={n}, nnz(A)={int(ei.shape[1])}, alpha={w['alpha']}, X Nx128, G Nx256, "
+                                  "hidden 256"},
+           "detail": {"ms_step_cbm": round(res["cbm"], 4), "ms_step_rocsparse_csr": round(res["rocsparse_csr"], 4),
+                      "speedup": round(res["rocsparse_csr"] / res["cbm"], 3)}}
+    print(json.dumps(out), flush=True)
 
 
 def profile_steps(w, m, args, dev, stream):
@@ -268,6 +319,8 @@ def main():
     ap.add_argument("--cols", type=int, default=None)
     ap.add_argument("--secondary", default="c4", help="second workload reported under 'secondary' ('' for none)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--gcn", action="store_true",
+                    help="time the 2-layer GCN training step (c3 graph, hidden 256) instead; CBM vs rocSPARSE CSR")
     ap.add_argument("--profile", action="store_true",
                     help="run only the CBM steps (BF16, then FP32) for a profiler; prints their times")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -293,6 +346,9 @@ def main():
         log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
         del src, dst
 
+    if args.gcn:
+        gcn_step(args, dev, stream)
+        return
     w, ei, m = build_workload(args.workload, args.alpha, args.cols, dev)
     if args.profile:
         profile_steps(w, m, args, dev, stream)
@@ -314,6 +370,7 @@ def main():
 
     ms16 = rec["ms_step_bf16"]
     dom = rec["dominant_phase"]
+    traffic = measured_traffic(args.workload, dom)
     out = {
         "metric": "A@X fwd+bwd ms & HBM GB/s (% roofline), CBM-HIP vs CSR on-GPU, FP32 & BF16",
         "value": ms16,
@@ -330,8 +387,12 @@ def main():
         "config": {"workload": rec["workload"], "nnz_delta": rec["nnz_delta"], "depth": rec["depth"],
                    "compression": rec["compression"]},
         "roofline": {"bound": "hbm", "achieved": rec["dominant_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(rec["dominant_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"{dom} phase (BF16)",
+                     "frac": round(rec["dominant_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic.get("bytes") if traffic else None,
+                     "traffic_gbs": round(traffic["bytes"] / (rec["ms_phase_bf16"][dom] * 1e-3) / 1e9, 1)
+                     if traffic else None,
+                     "traffic_source": traffic.get("source") if traffic else None,
+                     "kernel": f"{dom} phase (BF16): k_row_chunk + k_row_pass + k_row_reduce, dual pass",
                      "algorithmic_bytes": rec["bytes_phase_bf16"][dom],
                      "ms": rec["ms_phase_bf16"][dom]},
         "detail": rec,
