@@ -145,6 +145,10 @@ def measure(name, w, ei, m, args, dev, stream):
         "adjoint_reverse": time_loop(lambda: h.adjoint_reverse(g16, z), K, W, stream),
         "adjoint_delta_t": time_loop(lambda: h.adjoint_delta_t(g16, z, dx), K, W, stream),
     }
+    # the other form of the forward (same bits): per-level fused Delta + parent passes
+    h.set_forward_split(False)
+    ph["forward_fused_per_level"] = time_loop(lambda: h.forward(x16, y), K, W, stream)
+    h.set_forward_split(None)
     # results the CPU leg compares against FP64 (fresh, from the BF16 and FP32 steps)
     out = {}
     for tag, (xx, gg) in (("bf16", (x16, g16)), ("fp32", (x32, g32))):

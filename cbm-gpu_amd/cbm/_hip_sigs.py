@@ -52,12 +52,15 @@ class CbmDesc(ctypes.Structure):
         ("n_bred", c_i32),
         ("bwd_slots", c_i32),
         ("d", c_vp),
+        ("fck_gslot", c_vp),
+        ("fred_gslot", c_vp),
+        ("fwd_gslots", c_i32),
     ]
 
 # Device arrays of the plan that the descriptor points at.
 DEVICE_ARRAYS = ("lvl_node", "lvl_parent", "lvl_ppos", "fwd_ptr", "fwd_col", "rev_node", "ch_ptr", "ch_node", "ch_mid", "fck_pos", "fck_meta",
                  "fck_slot", "fred_pos", "fred_slot", "fred_cnt", "bwd_node", "bwd_ptr", "bwd_row", "bwd_mid", "bck_pos",
-                 "bck_meta", "bck_slot", "bred_pos", "bred_slot", "bred_cnt")
+                 "bck_meta", "bck_slot", "bred_pos", "bred_slot", "bred_cnt", "fck_gslot", "fred_gslot")
 HOST_ARRAYS = ("lvl_ptr", "lvl_nlong", "fck_ptr", "fred_ptr", "rev_ptr")
 
 
@@ -77,6 +80,7 @@ def declare(lib, sig):
     sig(lib, "cbm_last_launches", c_i32, c_vp)
     sig(lib, "cbm_set_slab_bytes", c_i32, c_vp, c_i64)
     sig(lib, "cbm_set_kahan_depth", c_i32, c_vp, c_i32)
+    sig(lib, "cbm_set_forward_split", c_i32, c_vp, c_i32)
     sig(lib, "csr_last_error", ctypes.c_char_p)
     sig(lib, "csr_create", c_i32, c_i32, c_i32, c_i64, c_vp, c_vp, c_vp, P_vp)
     sig(lib, "csr_destroy", c_i32, c_vp)
@@ -92,5 +96,6 @@ def declare(lib, sig):
 # Every symbol include/cbm_hip.h declares (checked by the CPU test suite).
 EXPORTED = ("cbm_last_error", "cbm_desc_size", "cbm_create", "cbm_destroy", "cbm_reserve", "cbm_matmul", "cbm_matmul_t", "cbm_adjoint_reverse",
             "cbm_adjoint_delta_t", "cbm_delta_spmm",
-            "cbm_propagate", "cbm_last_launches", "cbm_set_slab_bytes", "cbm_set_kahan_depth", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
+            "cbm_propagate", "cbm_last_launches", "cbm_set_slab_bytes", "cbm_set_kahan_depth",
+            "cbm_set_forward_split", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
             "csr_last_error", "cbm_op_spmm", "cbm_op_update", "cbm_op_axpy", "cbm_op_last_error")

@@ -122,7 +122,7 @@ class cusparseCBM:
         desc = CbmDesc(n_rows=self.num_rows, n_levels=self.depth, nnz=self.nnz, n_edges=self.n_edges,
                        mode=self.mode, fwd_slots=st["fwd_slots"], bwd_nlong=st["bwd_nlong"],
                        n_bck=int(self._dev["bck_pos"].numel()), n_bred=int(self._dev["bred_pos"].numel()),
-                       bwd_slots=st["bwd_slots"], d=self._d.data_ptr() if self._d is not None else None, **ptrs)
+                       bwd_slots=st["bwd_slots"], fwd_gslots=st["fwd_gslots"], d=self._d.data_ptr() if self._d is not None else None, **ptrs)
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             rc = self._lib.cbm_create(ctypes.byref(desc), current_stream_handle(self.device), ctypes.byref(h))
@@ -287,6 +287,13 @@ class cusparseCBM:
         term (default 32; None or a negative value switches it off)."""
         lvl = -1 if level is None or level < 0 else int(level)
         _lib.check_hip(self._lib.cbm_set_kahan_depth(self._h, lvl), "set_kahan_depth")
+
+    def set_forward_split(self, mode):
+        """Form of the forward product: True split (Delta pass over all levels,
+        then per-level propagation), False fused per level, None automatic.
+        Both give identical results."""
+        m = -1 if mode is None else int(bool(mode))
+        _lib.check_hip(self._lib.cbm_set_forward_split(self._h, m), "set_forward_split")
 
     def set_slab_bytes(self, nbytes):
         """Column-slab size for cache residency of the gathers (0: whole rows)."""

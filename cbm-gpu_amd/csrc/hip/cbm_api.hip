@@ -4,6 +4,7 @@
 // the caller's stream instead of the legacy default stream (:196,256).
 #include "cbm_kernels.hpp"
 #include "../../../include/cbm_hip.h"
+#include "../../../include/cbm_host.h"
 
 #include <algorithm>
 #include <cstdarg>
@@ -21,6 +22,7 @@ struct cbm_handle {
     int64_t scratch_floats = 0;
     int64_t slab_bytes = 0;   // column slab target (gathered bytes), 0 = whole rows
     int32_t kahan_level = 32; // levels from here on propagate with compensation
+    int32_t fwd_split = -1;   // forward form: 0 fused per level, 1 split, -1 automatic
     float *comp = nullptr;    // compensation rows of those levels
     int64_t comp_floats = 0;
     int last_launches = 0;
@@ -76,8 +78,15 @@ int check_dense(const cbm_handle *h, int32_t dtype, const void *in, int64_t ldin
     return CBM_OK;
 }
 
+// First level position that carries a compensation row.  Level 0 has no
+// parent term, so compensation starts at level max(kahan_level, 1).
 int32_t comp_base(const cbm_handle *h) {
-    return h->kahan_level >= 0 && h->kahan_level < h->d.n_levels ? h->lvl_ptr[size_t(h->kahan_level)] : h->d.n_rows;
+    const int32_t first = std::max<int32_t>(h->kahan_level, 1);
+    return h->kahan_level >= 0 && first < h->d.n_levels ? h->lvl_ptr[size_t(first)] : h->d.n_rows;
+}
+
+bool kahan_at(const cbm_handle *h, int32_t level) {
+    return h->kahan_level >= 0 && level >= h->kahan_level && level > 0;
 }
 
 int ensure_comp(cbm_handle *h, int32_t n_cols) {
@@ -101,7 +110,7 @@ int ensure_comp(cbm_handle *h, int32_t n_cols) {
 int ensure_scratch(cbm_handle *h, int32_t n_cols) {
     const int rc = ensure_comp(h, n_cols);
     if (rc != CBM_OK) return rc;
-    const int64_t slots = std::max<int64_t>(h->d.fwd_slots, h->d.bwd_slots);
+    const int64_t slots = std::max<int64_t>({h->d.fwd_slots, h->d.bwd_slots, h->d.fwd_gslots});
     const int64_t need = slots * int64_t(n_cols);
     if (need <= h->scratch_floats) return CBM_OK;
     if (h->scratch) {
@@ -184,9 +193,8 @@ int forward_levels(cbm_handle *h, cbmk::RowPass rp, bool with_parent, hipStream_
     rp.comp_base = comp_base(h);
     rp.ldc = rp.ncols;
     for (int32_t l = 0; l < h->d.n_levels; ++l) {
-        rp.extra = (with_parent && l > 0)
-                       ? (l >= h->kahan_level && h->kahan_level >= 0 ? cbmk::kExtraParentKahan : cbmk::kExtraParent)
-                       : cbmk::kExtraNone;
+        rp.extra = (with_parent && l > 0) ? (kahan_at(h, l) ? cbmk::kExtraParentKahan : cbmk::kExtraParent)
+                                          : cbmk::kExtraNone;
         ck.first = h->fck_ptr[size_t(l)];
         ck.count = h->fck_ptr[size_t(l) + 1] - ck.first;
         rd.first = h->fred_ptr[size_t(l)];
@@ -196,6 +204,60 @@ int forward_levels(cbm_handle *h, cbmk::RowPass rp, bool with_parent, hipStream_
         if (rc != CBM_OK) return rc;
     }
     return CBM_OK;
+}
+
+// The Delta pass of every level at once (no parent term): all long-row
+// chunks with slots numbered over the levels, one short-row pass over all
+// positions that skips the long rows, then all reductions.
+int delta_flat(cbm_handle *h, cbmk::RowPass rp, hipStream_t stream, const char *what) {
+    const int32_t L = h->d.n_levels;
+    cbmk::ChunkArgs ck;
+    ck.pos = h->d.fck_pos;
+    ck.meta = h->d.fck_meta;
+    ck.slot = h->d.fck_gslot;
+    ck.first = 0;
+    ck.count = h->fck_ptr[size_t(L)];
+    ck.scratch = h->scratch;
+    cbmk::ReduceArgs rd;
+    rd.pos = h->d.fred_pos;
+    rd.slot = h->d.fred_gslot;
+    rd.cnt = h->d.fred_cnt;
+    rd.first = 0;
+    rd.count = h->fred_ptr[size_t(L)];
+    rd.scratch = h->scratch;
+    rp.extra = cbmk::kExtraNone;
+    rp.skip_above = CBM_LONG_ROW;
+    return run_range(h, rp, 0, h->d.n_rows, 0, ck, rd, stream, what);
+}
+
+// In-place propagation of levels 1 .. L-1, y_v += s_v y_parent(v), with the
+// compensated form from the handle's Kahan level on.
+int propagate_levels(cbm_handle *h, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream, const char *what) {
+    const float *ps = h->d.mode == CBM_MODE_DADX ? h->ps_pos : nullptr;
+    cbmk::CompArgs cm;
+    cm.parent_pos = h->d.lvl_ppos;
+    cm.comp = h->comp;
+    cm.base = comp_base(h);
+    cm.ldc = n_cols;
+    for (int32_t l = 1; l < h->d.n_levels; ++l) {
+        const int32_t b = h->lvl_ptr[size_t(l)], e = h->lvl_ptr[size_t(l) + 1];
+        const hipError_t err =
+            kahan_at(h, l) ? cbmk::launch_propagate_comp(b, e, h->d.lvl_node, h->d.lvl_parent, ps, cm, y, ldy, n_cols,
+                                                         stream)
+                           : cbmk::launch_propagate(b, e, h->d.lvl_node, h->d.lvl_parent, ps, y, ldy, n_cols, stream);
+        if (err != hipSuccess) return device_error(err, what);
+        ++h->last_launches;
+    }
+    return CBM_OK;
+}
+
+// Which forward form a call uses.  The split form (one Delta pass over all
+// levels, then one light propagation launch per level) keeps the whole chip
+// busy on the gathers however thin the levels are, at the price of one more
+// FP32 read and write of each non-root row; both forms give the same bits.
+bool use_split(const cbm_handle *h) {
+    if (h->fwd_split >= 0) return h->fwd_split == 1;
+    return h->d.n_levels > 1;
 }
 
 cbmk::RowPass forward_pass(const cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy,
@@ -268,8 +330,9 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
             return CBM_ERR_ARG;
         }
     }
-    if ((d.fck_ptr_host[L] > 0 && (!d.fck_pos || !d.fck_meta || !d.fck_slot)) ||
-        (d.fred_ptr_host[L] > 0 && (!d.fred_pos || !d.fred_slot || !d.fred_cnt))) {
+    if ((d.fck_ptr_host[L] > 0 && (!d.fck_pos || !d.fck_meta || !d.fck_slot || !d.fck_gslot)) ||
+        (d.fred_ptr_host[L] > 0 && (!d.fred_pos || !d.fred_slot || !d.fred_cnt || !d.fred_gslot)) ||
+        d.fwd_gslots < 0) {
         set_error("cbm_create: missing forward chunk tables");
         return CBM_ERR_ARG;
     }
@@ -330,9 +393,12 @@ extern "C" int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t l
     if (rc != CBM_OK) return rc;
     h->last_launches = 0;
     const int elem = dtype == CBM_BF16 ? 2 : 4;
+    const bool split = use_split(h);
     return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
-        return forward_levels(h, forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w), true, stream,
-                              "cbm_matmul");
+        const cbmk::RowPass rp = forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w);
+        if (!split) return forward_levels(h, rp, true, stream, "cbm_matmul");
+        const int r = delta_flat(h, rp, stream, "cbm_matmul");
+        return r == CBM_OK ? propagate_levels(h, y + c0, ldy, w, stream, "cbm_matmul") : r;
     });
 }
 
@@ -472,8 +538,8 @@ extern "C" int cbm_delta_spmm(cbm_handle *h, int32_t dtype, const void *x, int64
     h->last_launches = 0;
     const int elem = dtype == CBM_BF16 ? 2 : 4;
     return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
-        return forward_levels(h, forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w), false, stream,
-                              "cbm_delta_spmm");
+        return delta_flat(h, forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w), stream,
+                          "cbm_delta_spmm");
     });
 }
 
@@ -482,15 +548,10 @@ extern "C" int cbm_propagate(cbm_handle *h, float *y, int64_t ldy, int32_t n_col
         set_error("cbm_propagate: invalid arguments");
         return CBM_ERR_ARG;
     }
+    const int rc = ensure_comp(h, n_cols);
+    if (rc != CBM_OK) return rc;
     h->last_launches = 0;
-    const float *ps = h->d.mode == CBM_MODE_DADX ? h->ps_pos : nullptr;
-    for (int32_t l = 1; l < h->d.n_levels; ++l) {
-        const hipError_t e = cbmk::launch_propagate(h->lvl_ptr[size_t(l)], h->lvl_ptr[size_t(l) + 1], h->d.lvl_node,
-                                                    h->d.lvl_parent, ps, y, ldy, n_cols, stream);
-        if (e != hipSuccess) return device_error(e, "cbm_propagate");
-        ++h->last_launches;
-    }
-    return CBM_OK;
+    return propagate_levels(h, y, ldy, n_cols, stream, "cbm_propagate");
 }
 
 extern "C" int cbm_last_launches(const cbm_handle *h) { return h ? h->last_launches : -1; }
@@ -501,6 +562,15 @@ extern "C" int cbm_set_kahan_depth(cbm_handle *h, int32_t level) {
         return CBM_ERR_ARG;
     }
     h->kahan_level = level < 0 ? -1 : level;
+    return CBM_OK;
+}
+
+extern "C" int cbm_set_forward_split(cbm_handle *h, int32_t mode) {
+    if (!h || mode < -1 || mode > 1) {
+        set_error("cbm_set_forward_split: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    h->fwd_split = mode;
     return CBM_OK;
 }
 

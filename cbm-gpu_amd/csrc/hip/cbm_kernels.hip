@@ -144,6 +144,53 @@ template <typename TS, int VE, int LPR>
 __device__ inline void gather_src(const RowPass &a, const TS *__restrict__ src, int64_t ld,
                                   const float *__restrict__ scale, const float *__restrict__ val, int32_t kb,
                                   int32_t ke, int l, int32_t c0, bool active, float (&acc)[VE]) {
+    if constexpr (LPR < 4) {
+        // Groups narrower than four lanes fetch four entries per step, 4 / LPR
+        // per lane (entry base + m * LPR + l), so that four gathers are still
+        // in flight per lane.
+        constexpr int M = 4 / LPR;
+        for (int32_t base = kb; base < ke; base += 4) {
+            int32_t mine[M];
+            float mine_scale[M];
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const int32_t e = base + m * LPR + l;
+                mine[m] = 0;
+                mine_scale[m] = 0.0f;
+                if (e < ke) {
+                    mine[m] = a.idx[e];
+                    float cs = scale ? scale[mine[m] & kIndexMask] : 1.0f;
+                    if (val) cs *= val[e];
+                    mine_scale[m] = signed_scale(cs, mine[m]);
+                }
+            }
+            const int32_t cnt = min(4, ke - base);
+            int32_t j[4];
+            float sc[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                j[u] = group_bcast<LPR>(mine[u / LPR], u % LPR) & kIndexMask;
+                sc[u] = group_bcast<LPR>(mine_scale[u / LPR], u % LPR);
+            }
+            if (active) {
+                float v[4][VE];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (u < cnt) {
+                        Load<TS, VE>::run(src + int64_t(j[u]) * ld + c0, v[u]);
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < VE; ++q) v[u][q] = 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[u][q], sc[u], acc[q]);
+            }
+        }
+        return;
+    }
     for (int32_t base = kb; base < ke; base += LPR) {
         int32_t mine = 0;
         float mine_scale = 0.0f;
@@ -204,6 +251,27 @@ __device__ inline void gather_sum(const RowPass &a, int32_t p, int32_t kb, int32
     }
 }
 
+// acc += es * (v + cp) as the pair (acc, comp) with acc the rounded sum and
+// comp its error: TwoProduct for es * v, TwoSum for the addition (Knuth),
+// Fast2Sum to fold the error back.  Every operation is an explicitly rounded
+// one, so contraction into fma cannot change the error terms.
+template <int VE>
+__device__ inline void comp_add(float (&acc)[VE], float es, const float (&v)[VE], const float (&cp)[VE],
+                                float (&comp)[VE]) {
+#pragma unroll
+    for (int q = 0; q < VE; ++q) {
+        const float t = __fmul_rn(es, v[q]);
+        const float t_err = fmaf(es, v[q], -t);
+        const float s = __fadd_rn(acc[q], t);
+        const float bb = __fsub_rn(s, acc[q]);
+        const float e = __fadd_rn(__fsub_rn(acc[q], __fsub_rn(s, bb)), __fsub_rn(t, bb));
+        const float err = __fadd_rn(e, __fadd_rn(t_err, __fmul_rn(es, cp[q])));
+        const float yq = __fadd_rn(s, err);
+        comp[q] = __fsub_rn(err, __fsub_rn(yq, s));
+        acc[q] = yq;
+    }
+}
+
 // Row scale, the extra term, and the store of one finished row segment.
 template <class P, int VE>
 __device__ inline void finish_row(const RowPass &a, int32_t p, int32_t row, int32_t c0, float (&acc)[VE]) {
@@ -245,18 +313,7 @@ __device__ inline void finish_row(const RowPass &a, int32_t p, int32_t row, int3
 #pragma unroll
                 for (int q = 0; q < VE; ++q) cp[q] = 0.0f;
             }
-#pragma unroll
-            for (int q = 0; q < VE; ++q) {
-                const float t = __fmul_rn(es, v[q]);
-                const float t_err = fmaf(es, v[q], -t);
-                const float s = __fadd_rn(acc[q], t);
-                const float bb = __fsub_rn(s, acc[q]);
-                const float e = __fadd_rn(__fsub_rn(acc[q], __fsub_rn(s, bb)), __fsub_rn(t, bb));
-                const float err = __fadd_rn(e, __fadd_rn(t_err, __fmul_rn(es, cp[q])));
-                const float yq = __fadd_rn(s, err);
-                comp[q] = __fsub_rn(err, __fsub_rn(yq, s));
-                acc[q] = yq;
-            }
+            comp_add<VE>(acc, es, v, cp, comp);
         }
         store_row<VE>(a.comp + int64_t(p - a.comp_base) * a.ldc + c0, comp);
     }
@@ -274,8 +331,9 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
     const int64_t p64 = int64_t(a.p_begin) + group;
     if (p64 >= int64_t(a.p_end)) return;  // whole groups leave together
     const int32_t p = int32_t(p64);
-    const int32_t row = a.node_of ? a.node_of[p] : p;
     const int32_t kb = a.ptr[p], ke = a.ptr_end ? a.ptr_end[p] : a.ptr[p + 1];
+    if (ke - kb > a.skip_above) return;  // a long row, done by the chunk kernel
+    const int32_t row = a.node_of ? a.node_of[p] : p;
     constexpr int32_t kSpan = LPR * VE;
     const int32_t passes = (a.ncols + kSpan - 1) / kSpan;
     for (int32_t pass = 0; pass < passes; ++pass) {
@@ -384,6 +442,45 @@ __global__ __launch_bounds__(256) void k_propagate(int32_t p_begin, int32_t p_en
 #pragma unroll
         for (int q = 0; q < VE; ++q) b[q] = fmaf(a[q], ps, b[q]);
         store_row<VE>(y + int64_t(row) * ldy + c0, b);
+    }
+}
+
+template <int VE, int LPR>
+__global__ __launch_bounds__(256) void k_propagate_comp(int32_t p_begin, int32_t p_end,
+                                                        const int32_t *__restrict__ node_of,
+                                                        const int32_t *__restrict__ parent_of,
+                                                        const float *__restrict__ parscale, const CompArgs c, float *y,
+                                                        int64_t ldy, int32_t ncols) {
+    constexpr int kGroupsPerWave = 64 / LPR;
+    const int lane = int(threadIdx.x & 63u);
+    const int l = lane & (LPR - 1);
+    const int64_t group =
+        (int64_t(blockIdx.x) * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kGroupsPerWave + lane / LPR;
+    const int64_t p64 = int64_t(p_begin) + group;
+    if (p64 >= int64_t(p_end)) return;
+    const int32_t p = int32_t(p64);
+    const int32_t par = parent_of[p];
+    const int32_t row = node_of[p];
+    const float es = parscale ? parscale[p] : 1.0f;
+    const int32_t pp = c.parent_pos[p];
+    for (int32_t c0 = l * VE; c0 < ncols; c0 += LPR * VE) {
+        float comp[VE];
+#pragma unroll
+        for (int q = 0; q < VE; ++q) comp[q] = 0.0f;
+        if (par >= 0) {
+            float acc[VE], v[VE], cp[VE];
+            Load<float, VE>::run(y + int64_t(row) * ldy + c0, acc);
+            Load<float, VE>::run(y + int64_t(par) * ldy + c0, v);
+            if (pp >= c.base) {
+                Load<float, VE>::run(c.comp + int64_t(pp - c.base) * c.ldc + c0, cp);
+            } else {
+#pragma unroll
+                for (int q = 0; q < VE; ++q) cp[q] = 0.0f;
+            }
+            comp_add<VE>(acc, es, v, cp, comp);
+            store_row<VE>(y + int64_t(row) * ldy + c0, acc);
+        }
+        store_row<VE>(c.comp + int64_t(p - c.base) * c.ldc + c0, comp);
     }
 }
 
@@ -568,6 +665,31 @@ hipError_t launch_propagate(int32_t p_begin, int32_t p_end, const int32_t *node_
     CBM_PROP_CASE(1, 1) CBM_PROP_CASE(1, 2) CBM_PROP_CASE(1, 4) CBM_PROP_CASE(1, 8) CBM_PROP_CASE(1, 16)
     CBM_PROP_CASE(1, 32) CBM_PROP_CASE(1, 64)
 #undef CBM_PROP_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_propagate_comp(int32_t p_begin, int32_t p_end, const int32_t *node_of, const int32_t *parent_of,
+                                 const float *parscale, const CompArgs &c, float *y, int64_t ldy, int32_t ncols,
+                                 hipStream_t stream) {
+    const int64_t n = int64_t(p_end) - p_begin;
+    if (n <= 0) return hipSuccess;
+    if (p_begin < c.base) return hipErrorInvalidValue;  // every position needs its compensation row
+    const bool vec = (ncols % 4) == 0 && aligned16(y, ldy, 4) && aligned16(c.comp, c.ldc, 4);
+    const int ve = vec ? 4 : 1;
+    const int lpr = lanes_per_row(ncols, ve);
+    const int64_t rows_per_block = 4 * (64 / lpr);
+    const dim3 grid(unsigned((n + rows_per_block - 1) / rows_per_block)), block(256);
+#define CBM_PROPC_CASE(V, L)                                                                                     \
+    if (ve == V && lpr == L) {                                                                                   \
+        k_propagate_comp<V, L><<<grid, block, 0, stream>>>(p_begin, p_end, node_of, parent_of, parscale, c, y, \
+                                                           ldy, ncols);                                          \
+        return hipGetLastError();                                                                                \
+    }
+    CBM_PROPC_CASE(4, 1) CBM_PROPC_CASE(4, 2) CBM_PROPC_CASE(4, 4) CBM_PROPC_CASE(4, 8) CBM_PROPC_CASE(4, 16)
+    CBM_PROPC_CASE(4, 32) CBM_PROPC_CASE(4, 64)
+    CBM_PROPC_CASE(1, 1) CBM_PROPC_CASE(1, 2) CBM_PROPC_CASE(1, 4) CBM_PROPC_CASE(1, 8) CBM_PROPC_CASE(1, 16)
+    CBM_PROPC_CASE(1, 32) CBM_PROPC_CASE(1, 64)
+#undef CBM_PROPC_CASE
     return hipErrorInvalidValue;
 }
 

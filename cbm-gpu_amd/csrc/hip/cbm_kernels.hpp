@@ -57,6 +57,9 @@ struct RowPass {
     float *y = nullptr;
     int64_t ldy = 0;
     int32_t ncols = 0;
+    // The short-row pass skips positions with more entries than this (rows
+    // that the chunk kernel covers when one pass spans several levels).
+    int32_t skip_above = INT32_MAX;
 };
 
 // Launches one row pass over positions [p_begin, p_end); returns the launch status.
@@ -88,6 +91,22 @@ hipError_t launch_row_reduce(const RowPass &rp, const ReduceArgs &r, hipStream_t
 // In-place propagation of one level: y[node_of[p]] += parscale[p] * y[parent_of[p]].
 hipError_t launch_propagate(int32_t p_begin, int32_t p_end, const int32_t *node_of, const int32_t *parent_of,
                             const float *parscale, float *y, int64_t ldy, int32_t ncols, hipStream_t stream);
+
+// Compensation rows of a compensated propagation: position p >= base owns
+// comp[(p - base) * ldc, ...]; parent_pos[p] is the parent's position.
+struct CompArgs {
+    const int32_t *parent_pos = nullptr;
+    float *comp = nullptr;
+    int32_t base = 0;
+    int64_t ldc = 0;
+};
+
+// The same level propagation with the parent's compensation row added and
+// this row's rounding error stored (the arithmetic of kExtraParentKahan, so a
+// Delta pass followed by this gives the fused pass's bits).
+hipError_t launch_propagate_comp(int32_t p_begin, int32_t p_end, const int32_t *node_of, const int32_t *parent_of,
+                                 const float *parscale, const CompArgs &c, float *y, int64_t ldy, int32_t ncols,
+                                 hipStream_t stream);
 
 // The reference's branch-serial propagation (cbm/cbm.cu:25-91) for the
 // operator surface: one workgroup per branch and column block walks the

@@ -210,6 +210,31 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
                 fck_ptr[size_t(l) + 1] = int32_t(f.ck_pos.size());
                 fred_ptr[size_t(l) + 1] = int32_t(f.red_pos.size());
             }
+            // The same chunks with slots numbered over all levels, for a pass
+            // that runs every level's rows in one launch: a row's chunks are
+            // consecutive, so a multi-chunk row takes the next cnt slots.
+            auto &gslot = plan->arr[CBM_P_FCK_GSLOT];
+            auto &rgslot = plan->arr[CBM_P_FRED_GSLOT];
+            gslot.assign(f.ck_pos.size(), -1);
+            rgslot.assign(f.red_pos.size(), 0);
+            int32_t next = 0;
+            size_t r = 0;
+            for (size_t i = 0; i < f.ck_pos.size();) {
+                const int32_t cnt = int32_t(uint32_t(f.ck_meta[i]) >> 16);
+                if (f.ck_slot[i] < 0) {
+                    ++i;
+                    continue;
+                }
+                if (r >= f.red_pos.size() || f.red_pos[r] != f.ck_pos[i] || f.red_cnt[r] != cnt) {
+                    set_error("plan: forward chunk and reduction tables disagree at chunk %zu", i);
+                    return CBM_ERR_INTERNAL;
+                }
+                rgslot[r++] = next;
+                for (int32_t c = 0; c < cnt; ++c) gslot[i + size_t(c)] = next + c;
+                i += size_t(cnt);
+                next += cnt;
+            }
+            plan->stat[CBM_PS_FWD_GSLOTS] = next;
             plan->arr[CBM_P_FCK_POS].swap(f.ck_pos);
             plan->arr[CBM_P_FCK_META].swap(f.ck_meta);
             plan->arr[CBM_P_FCK_SLOT].swap(f.ck_slot);

@@ -105,6 +105,11 @@ typedef struct cbm_desc {
     int32_t n_bred;
     int32_t bwd_slots;
     const float *d;
+    /* device: forward chunk slots numbered over all levels (plan arrays
+       FCK_GSLOT, FRED_GSLOT) and their count (plan stat FWD_GSLOTS) */
+    const int32_t *fck_gslot;
+    const int32_t *fred_gslot;
+    int32_t fwd_gslots;
 } cbm_desc;
 
 typedef struct cbm_handle cbm_handle;
@@ -125,7 +130,8 @@ int cbm_destroy(cbm_handle *h);
 
 /*
  * Sizes the handle's scratch for products of up to n_cols columns: the
- * partial sums of long rows, max(fwd_slots, bwd_slots) x n_cols FP32.  A
+ * partial sums of long rows, max(fwd_slots, bwd_slots, fwd_gslots) x n_cols
+ * FP32, and the compensation rows of the compensated levels.  A
  * product call with more columns than reserved grows the scratch itself
  * (one hipMalloc); reserve first when the calls must not allocate (graph
  * capture).
@@ -135,8 +141,11 @@ int cbm_reserve(cbm_handle *h, int32_t n_cols);
 /*
  * Forward product Y = M X.  x: N x n_cols, leading dimension ldx (elements);
  * y: N x n_cols FP32, leading dimension ldy; x and y must not overlap.
- * Fused: per tree level, Delta rows are summed and the parent's finished row
- * added in the same pass (long rows by whole workgroups, in chunks).
+ * Two forms with the same result bits (cbm_set_forward_split): split (the
+ * default when the tree has more than one level) sums every Delta row in one
+ * pass over all levels, then adds parents level by level in place; fused
+ * sums each level's Delta rows and adds the parent's finished row in the same
+ * pass.  Long rows are summed by whole workgroups, in chunks.
  * Replaces cusparseCBM::matmul_AX_ADX / matmul_DAX_DADX.
  */
 int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy, int32_t n_cols,
@@ -173,6 +182,8 @@ int cbm_adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg
 int cbm_delta_spmm(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy, int32_t n_cols,
                    hipStream_t stream);
 int cbm_propagate(cbm_handle *h, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream);
+/* cbm_propagate uses the compensated form from the Kahan level on (see
+   cbm_set_kahan_depth), so cbm_delta_spmm + cbm_propagate = cbm_matmul. */
 
 /* Per-kernel launch counts of the last call on this handle (for profiling). */
 int cbm_last_launches(const cbm_handle *h);
@@ -195,6 +206,13 @@ int cbm_set_kahan_depth(cbm_handle *h, int32_t level);
  * more than the cache residency returns.
  */
 int cbm_set_slab_bytes(cbm_handle *h, int64_t bytes);
+
+/*
+ * Form of cbm_matmul: 1 split (Delta pass over all levels, then per-level
+ * propagation), 0 fused per level, -1 automatic (split when the tree has
+ * more than one level).  Both give identical results.
+ */
+int cbm_set_forward_split(cbm_handle *h, int32_t mode);
 
 /* ------------------------------------------------------------------------- */
 /* CSR comparator on the same GPU: rocSPARSE SpMM, FP32.                      */

@@ -19,13 +19,14 @@
 extern "C" {
 #endif
 
-enum {
-    CBM_OK = 0,
-    CBM_ERR_ARG = -1,      /* invalid argument (null pointer, negative size, index out of range) */
-    CBM_ERR_VALUE = -2,    /* input violates a precondition (non-binary value, ...)             */
-    CBM_ERR_NOMEM = -3,    /* host allocation failed                                             */
-    CBM_ERR_INTERNAL = -4  /* internal invariant broken                                          */
-};
+/* Status codes (shared with cbm_hip.h). */
+#ifndef CBM_OK
+#define CBM_OK 0
+#define CBM_ERR_ARG -1      /* invalid argument (null pointer, negative size, index out of range) */
+#define CBM_ERR_VALUE -2    /* input violates a precondition (non-binary value, ...)             */
+#define CBM_ERR_NOMEM -3    /* host allocation failed                                             */
+#define CBM_ERR_INTERNAL -4 /* internal invariant broken                                          */
+#endif
 
 /* Message describing the last failure on the calling thread ("" if none). */
 const char *cbm_host_last_error(void);
@@ -156,7 +157,10 @@ enum {
     CBM_P_BWD_MID = 27,    /* int32 [N]    first internal-node entry of each backward row              */
     CBM_P_CH_MID = 28,     /* int32 [I]    first internal child of each internal-node position         */
     CBM_P_LVL_PPOS = 29,   /* int32 [N]    level position of the parent of each position, -1 for level 0 */
-    CBM_P_N_ARRAYS = 30
+    CBM_P_FCK_GSLOT = 30,  /* int32 [C]    partial slot of each forward chunk numbered over all levels
+                                           (one pass over every level at once), -1 for a one-chunk row */
+    CBM_P_FRED_GSLOT = 31, /* int32 [R]    first such slot of each forward reduction row               */
+    CBM_P_N_ARRAYS = 32
 };
 
 /* Scalars of a cbm_plan, addressed by cbm_plan_stat(). */
@@ -165,7 +169,8 @@ enum {
     CBM_PS_FWD_SLOTS = 1,  /* partial slots needed by the largest forward level      */
     CBM_PS_BWD_SLOTS = 2,  /* partial slots needed by the backward pass              */
     CBM_PS_BWD_NLONG = 3,  /* long rows at the start of the backward order           */
-    CBM_PS_N_STATS = 4
+    CBM_PS_FWD_GSLOTS = 4, /* partial slots of all forward levels together (FCK_GSLOT) */
+    CBM_PS_N_STATS = 5
 };
 
 #define CBM_LONG_ROW 256

@@ -72,3 +72,53 @@ def test_long_rows_integer_exact_and_deterministic(hub, cuda):
     d1 = torch.empty(HUB_N, 32, device=cuda)
     h.matmul_t(xt, d1)
     assert np.array_equal(d1.cpu().numpy().astype(np.float64), O.adjoint(a, x, "ax"))
+
+
+@pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
+@pytest.mark.parametrize("dtype,f", [(torch.float32, 1), (torch.float32, 3), (torch.float32, 4), (torch.float32, 8),
+                                     (torch.float32, 64), (torch.bfloat16, 16), (torch.bfloat16, 128)])
+def test_split_and_fused_forward_are_bit_identical(hub, cuda, mode, dtype, f):
+    """The split forward (one Delta pass over all levels with chunk slots
+    numbered over the levels, then per-level propagation) and the fused
+    per-level forward give the same bits, plain and compensated, and
+    delta_spmm + propagate gives them too.  The hub graph has multi-chunk
+    rows at several levels, so the global slot numbering is exercised."""
+    a, mats = hub
+    h = mats[mode].cusparse_extension
+    xt = torch.from_numpy(_x((HUB_N, f), 11 + f)).to(cuda).to(dtype)
+    for depth in (None, 10, 1, 0):
+        h.set_kahan_depth(depth)
+        ys = []
+        for split in (False, True):
+            h.set_forward_split(split)
+            y = torch.full((HUB_N, f), float("nan"), device=cuda)
+            h.forward(xt, y)
+            ys.append(y)
+        h.set_forward_split(None)
+        y3 = torch.full((HUB_N, f), float("nan"), device=cuda)
+        h.delta_spmm(xt, y3)
+        h.propagate(y3)
+        assert torch.equal(ys[0], ys[1]), (depth, f)
+        assert torch.equal(ys[0], y3), (depth, f)
+        assert O.max_rel_error(ys[1].cpu().numpy(), O.forward(a, xt.float().cpu().numpy(), mode)) <= 1e-5
+    h.set_kahan_depth(32)
+
+
+def test_compensation_from_level_zero_equals_level_one(hub, cuda):
+    """Level 0 has no parent term, so compensation from level 0 and from
+    level 1 are the same computation; the level-0 rows carry no compensation
+    row to read (a parent at level 0 contributes none)."""
+    _, mats = hub
+    h = mats["ax"].cusparse_extension
+    xt = torch.from_numpy(_x((HUB_N, 32), 21)).to(cuda)
+    out = []
+    for depth in (1, 0):
+        h.set_kahan_depth(depth)
+        for split in (False, True):
+            h.set_forward_split(split)
+            y = torch.empty(HUB_N, 32, device=cuda)
+            h.forward(xt, y)
+            out.append(y)
+    h.set_kahan_depth(32)
+    h.set_forward_split(None)
+    assert all(torch.equal(out[0], o) for o in out[1:])

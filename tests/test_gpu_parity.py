@@ -154,7 +154,7 @@ def test_split_halves_match_fused(mats, cuda):
         h.forward(x, y1)
         h.delta_spmm(x, y2)
         h.propagate(y2)
-        torch.testing.assert_close(y1, y2, rtol=1e-6, atol=1e-6)
+        assert torch.equal(y1, y2)
 
 
 def test_strided_operands(mats, graph1, cuda):

@@ -218,11 +218,13 @@ def test_plan_rejects_cycles_and_double_parents():
                        torch.tensor([2, 2], dtype=torch.int32))
 
 
-def _check_split(ptr, first, n_long, ck_pos, ck_meta, ck_slot, red_pos, red_slot, red_cnt, slots):
-    """Chunks tile every long row exactly; slots are unique and in range."""
+def _check_split(ptr, first, n_long, ck_pos, ck_meta, ck_slot, red_pos, red_slot, red_cnt, slots, long_rows=None):
+    """Chunks tile every long row exactly; slots are unique and in range.
+    The long rows are [first, first + n_long) unless given as a set."""
     ptr = np.asarray(ptr, np.int64)
     lens = np.diff(ptr)
-    long_rows = set(range(first, first + n_long))
+    if long_rows is None:
+        long_rows = set(range(first, first + n_long))
     assert all(lens[p] > 256 for p in long_rows)
     seen = {}
     for pos, meta, slot in zip(ck_pos, ck_meta, ck_slot):
@@ -265,3 +267,11 @@ def test_plan_chunk_tables_on_long_rows():
     _check_split(pl.bwd_ptr.numpy(), 0, st["bwd_nlong"], pl.bck_pos.numpy(), pl.bck_meta.numpy(),
                  pl.bck_slot.numpy(), pl.bred_pos.numpy(), pl.bred_slot.numpy(), pl.bred_cnt.numpy(),
                  st["bwd_slots"])
+    # The same forward chunks with slots numbered over all levels (one pass
+    # over every level): unique over the whole table, same chunks otherwise.
+    all_long = {int(p) for l in range(pl.depth) for p in range(lp[l], lp[l] + nl[l])}
+    _check_split(pl.fwd_ptr.numpy(), 0, 0, pl.fck_pos.numpy(), pl.fck_meta.numpy(), pl.fck_gslot.numpy(),
+                 pl.fred_pos.numpy(), pl.fred_gslot.numpy(), pl.fred_cnt.numpy(), st["fwd_gslots"],
+                 long_rows=all_long)
+    assert st["fwd_gslots"] == int(pl.fred_cnt.numpy().sum())
+    assert np.array_equal(pl.fck_gslot.numpy() < 0, pl.fck_slot.numpy() < 0)
