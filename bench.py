@@ -1,8 +1,9 @@
 """Benchmark of the CBM product on one MI355X.
 
-A step is one forward product Y = A X plus one adjoint dX = A^T G (cbm4mm,
-the A@X path) over the full synthetic graph of the workload, inputs resident
-in HBM.  Reported on the JSON line:
+A step is one forward product Y = M X plus one adjoint dX = M^T G over the
+full synthetic graph of the workload, inputs resident in HBM, with M the
+workload's product: A (cbm4mm: c1, c5), A D^-1/2 (cbm4ad: c4) or
+D^-1/2 A D^-1/2 (cbm4dad: c2, c3).  Reported on the JSON line:
 
   value            ms per step, BF16 storage of X and G, FP32 accumulation
   detail           the FP32 step, the two halves of each product, the
@@ -10,7 +11,7 @@ in HBM.  Reported on the JSON line:
                    SpMM algorithms, A^T as an explicit CSR), the same
                    gathers without compression (our kernel over plain A)
   roofline         the dominant phase of the step against 8 TB/s HBM
-  cpu_baseline     torch.sparse CSR (MKL) FP32 A@X + A^T@G on the host cores
+  cpu_baseline     torch.sparse CSR (MKL) FP32 M@X + M^T@G on the host cores
   secondary        the compressible workload (c4) with the same fields
 
     python bench.py [--steps K] [--warmup W] [--workload c5] [--alpha 8] [--cols 128]
@@ -27,14 +28,15 @@ sys.path.insert(0, os.path.join(ROOT, "cbm-gpu_amd"))
 HBM_PEAK_GBS = 8000.0
 
 # Seeded community copying model per workload: N, blocks, copy probability,
-# random-link rate lambda, first-node degree of a block, graph seed, alpha, F.
-# lambda (and the seed degree) are set so that nnz lands on the stated size.
+# random-link rate lambda, first-node degree of a block, graph seed, alpha, F,
+# and the product.  lambda (and the seed degree) are set so that nnz lands on
+# the stated size.
 WORKLOADS = {
-    "c1": dict(n=2708, blocks=7, p=0.8, lam=0.58, seed_deg=0, seed=1001, alpha=0, cols=64),
-    "c2": dict(n=19717, blocks=3, p=0.85, lam=0.44, seed_deg=0, seed=1002, alpha=8, cols=128),
-    "c3": dict(n=169343, blocks=40, p=0.8, lam=1.63, seed_deg=0, seed=1003, alpha=16, cols=128),
-    "c4": dict(n=132534, blocks=8, p=0.95, lam=20.0, seed_deg=225, seed=1004, alpha=32, cols=256),
-    "c5": dict(n=2449029, blocks=47, p=0.75, lam=6.6, seed_deg=0, seed=1005, alpha=8, cols=128),
+    "c1": dict(n=2708, blocks=7, p=0.8, lam=0.58, seed_deg=0, seed=1001, alpha=0, cols=64, product="mm"),
+    "c2": dict(n=19717, blocks=3, p=0.85, lam=0.44, seed_deg=0, seed=1002, alpha=8, cols=128, product="dad"),
+    "c3": dict(n=169343, blocks=40, p=0.8, lam=1.63, seed_deg=0, seed=1003, alpha=16, cols=128, product="dad"),
+    "c4": dict(n=132534, blocks=8, p=0.95, lam=20.0, seed_deg=225, seed=1004, alpha=32, cols=256, product="ad"),
+    "c5": dict(n=2449029, blocks=47, p=0.75, lam=6.6, seed_deg=0, seed=1005, alpha=8, cols=128, product="mm"),
 }
 
 # rocsparse_spmm_alg values tried for the CSR comparator.
@@ -58,19 +60,31 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def phase_bytes(n, nnz_delta, n_edges, n_internal, depth, f, s_in):
+def phase_bytes(n, nnz_delta, n_edges, n_internal, depth, f, s_in, scaled):
     """Algorithmic (compulsory) HBM bytes of each phase: every structure
     array read once, every input row read once, every output row written
-    once; gathered rows read again, and intermediates, are not counted."""
-    fwd = 4 * (n + 1) + 4 * nnz_delta + 8 * n_edges + 4 * (depth + 1) + s_in * n * f + 4 * n * f
+    once; gathered rows read again, and intermediates, are not counted.
+    `scaled`: the product carries the D^-1/2 vector (4N bytes)."""
+    dvec = 4 * n if scaled else 0
+    fwd = 4 * (n + 1) + 4 * nnz_delta + 8 * n_edges + 4 * (depth + 1) + s_in * n * f + 4 * n * f + dvec
     rev = 4 * (n_internal + 1) + 4 * n_internal + 4 * n_edges + s_in * f * (n_internal + n_edges) + 4 * f * n_internal
-    dlt = 4 * (n + 1) + 4 * n + 4 * nnz_delta + s_in * n * f + 4 * n_internal * f + 4 * n * f
-    step_bwd = 8 * (n + 1) + 4 * nnz_delta + 4 * n_edges + 4 * (depth + 1) + s_in * n * f + 4 * n * f
+    dlt = 4 * (n + 1) + 4 * n + 4 * nnz_delta + s_in * n * f + 4 * n_internal * f + 4 * n * f + dvec
+    step_bwd = 8 * (n + 1) + 4 * nnz_delta + 4 * n_edges + 4 * (depth + 1) + s_in * n * f + 4 * n * f + dvec
     return {"forward": fwd, "adjoint_reverse": rev, "adjoint_delta_t": dlt, "step": fwd + step_bwd}
 
 
 def csr_bytes(n, nnz, f):
     return 4 * (n + 1) + 8 * nnz + 8 * n * f
+
+
+CBM_CLASSES = {"mm": ("cbm.cbm4mm", "cbm4mm"), "ad": ("cbm.cbm4ad", "cbm4ad"), "dad": ("cbm.cbm4dad", "cbm4dad")}
+CSR_CLASSES = {"mm": ("cbm.csr4mm", "csr4mm"), "ad": ("cbm.csr4ad", "csr4ad"), "dad": ("cbm.csr4dad", "csr4dad")}
+
+
+def _class(table, product):
+    import importlib
+    mod, name = table[product]
+    return getattr(importlib.import_module(mod), name)
 
 
 def time_loop(fn, steps, warmup, stream):
@@ -91,7 +105,6 @@ def time_loop(fn, steps, warmup, stream):
 def build_workload(name, alpha, cols, dev):
     import torch
     from cbm import structure
-    from cbm.cbm4mm import cbm4mm
     w = dict(WORKLOADS[name])
     if alpha is not None:
         w["alpha"] = alpha
@@ -101,9 +114,10 @@ def build_workload(name, alpha, cols, dev):
     ei = structure.copying_model(w["n"], w["blocks"], w["p"], w["lam"], w["seed_deg"], w["seed"])
     log(f"graph {name}: N={w['n']} nnz={ei.shape[1]} ({time.perf_counter() - t0:.1f}s)")
     t0 = time.perf_counter()
-    m = cbm4mm(ei, torch.ones(ei.shape[1]), alpha=w["alpha"], device=dev)
+    m = _class(CBM_CLASSES, w["product"])(ei, torch.ones(ei.shape[1]), alpha=w["alpha"], device=dev)
     h = m.cusparse_extension
-    log(f"CBM built: nnz(Delta)={h.nnz} tree edges={h.n_edges} depth={h.depth} ({time.perf_counter() - t0:.1f}s)")
+    log(f"CBM built ({CBM_CLASSES[w['product']][1]}): nnz(Delta)={h.nnz} tree edges={h.n_edges} depth={h.depth} "
+        f"({time.perf_counter() - t0:.1f}s)")
     return w, ei, m
 
 
@@ -112,8 +126,6 @@ def measure(name, w, ei, m, args, dev, stream):
     record for one workload (plus the device tensors the CPU leg checks)."""
     import torch
     from cbm.cbm_cusparse import cusparseCBM
-    from cbm.csr4mm import csr4mm
-    from cbm.csr_cusparse import cusparseCSR
 
     h = m.cusparse_extension
     n, f = w["n"], w["cols"]
@@ -133,10 +145,27 @@ def measure(name, w, ei, m, args, dev, stream):
             h.matmul_t(gg, dx, workspace=z)
         return run
 
+    def graph_ms(fn):
+        """The same step captured once in a HIP graph and replayed: GPU time
+        without the per-call host work (Python checks, launches)."""
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            fn()
+        torch.cuda.current_stream().wait_stream(side)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            fn()
+        ms = time_loop(graph.replay, K, W, stream)
+        del graph
+        return ms
+
     K, W = args.steps, args.warmup
     ms16 = time_loop(step(x16, g16), K, W, stream)
     ms32 = time_loop(step(x32, g32), K, W, stream)
-    log(f"{name}: CBM step bf16 {ms16:.3f} ms, fp32 {ms32:.3f} ms")
+    ms16_graph = graph_ms(step(x16, g16))
+    ms32_graph = graph_ms(step(x32, g32))
+    log(f"{name}: CBM step bf16 {ms16:.3f} ms, fp32 {ms32:.3f} ms; graph-replayed {ms16_graph:.3f} / {ms32_graph:.3f}")
     ph = {
         "forward": time_loop(lambda: h.forward(x16, y), K, W, stream),
         "forward_delta_spmm": time_loop(lambda: h.delta_spmm(x16, y), K, W, stream),
@@ -157,8 +186,13 @@ def measure(name, w, ei, m, args, dev, stream):
         h.matmul_t(gg, dd, workspace=z)
         out[tag] = (yy, dd)
 
-    # rocSPARSE CSR FP32: the fastest algorithm on the forward, A^T explicit
-    csr = csr4mm(ei, num_nodes=n, device=dev)
+    # the BF16 results against the FP32 CBM results (same kernels, FP32 inputs)
+    vs_fp32 = {k: float(((out["bf16"][i] - out["fp32"][i]).abs().max() / out["fp32"][i].abs().max()).item())
+               for i, k in enumerate(("forward", "adjoint"))}
+
+    # rocSPARSE CSR FP32 of the same product M: the fastest algorithm on the
+    # forward, M^T as an explicit CSR
+    csr = _class(CSR_CLASSES, w["product"])(ei, num_nodes=n, device=dev)
     ext = csr.cusparse_extension
     ext.transposed()
     alg_ms = {}
@@ -177,27 +211,33 @@ def measure(name, w, ei, m, args, dev, stream):
 
     ms_csr = time_loop(step_csr, K, W, stream)
     log(f"{name}: rocSPARSE CSR fp32 step {ms_csr:.3f} ms ({best_alg})")
+    m_csr = (csr.a.to("cpu"), ext.transposed().a.to("cpu"))  # M and M^T (FP32 values) for the host checks
     del csr, ext
 
-    # the same gathers without compression: our kernels over plain A, no tree
+    # the same gathers without compression: our kernels over plain A (a tree
+    # with no edges) with M's scalings, no Delta
     a = torch.sparse_csr_tensor(*_csr_of(ei, n), (n, n)).to(dev)
     e0 = torch.zeros(0, dtype=torch.int32, device=dev)
     hip_csr = cusparseCBM(a, torch.zeros(1, dtype=torch.int32, device=dev), e0, e0, torch.empty(0, device=dev))
+    if h.mode != hip_csr.mode:
+        hip_csr = hip_csr.with_scaling(h.mode, h._d)
     hip_csr.reserve(f)
     ms_hip_csr16 = time_loop(lambda: (hip_csr.forward(x16, y), hip_csr.matmul_t(g16, dx, workspace=z)), K, W, stream)
     ms_hip_csr32 = time_loop(lambda: (hip_csr.forward(x32, y), hip_csr.matmul_t(g32, dx, workspace=z)), K, W, stream)
     del hip_csr, a
 
     n_internal = int(h._dev["rev_node"].numel())
-    b16 = phase_bytes(n, h.nnz, h.n_edges, n_internal, h.depth, f, 2)
-    b32 = phase_bytes(n, h.nnz, h.n_edges, n_internal, h.depth, f, 4)
+    scaled = w["product"] != "mm"
+    b16 = phase_bytes(n, h.nnz, h.n_edges, n_internal, h.depth, f, 2, scaled)
+    b32 = phase_bytes(n, h.nnz, h.n_edges, n_internal, h.depth, f, 4, scaled)
     dom = "adjoint_delta_t"  # the step's largest single kernel launch is this pass's short-row kernel
     dom_gbs = b16[dom] / (ph[dom] * 1e-3) / 1e9
     rec = {
-        "workload": f"{name}: cbm4mm fwd+bwd, N={n}, nnz(A)={nnz}, alpha={w['alpha']}, F={f}",
+        "workload": f"{name}: {CBM_CLASSES[w['product']][1]} fwd+bwd, N={n}, nnz(A)={nnz}, alpha={w['alpha']}, F={f}",
         "nnz_delta": h.nnz, "tree_edges": h.n_edges, "internal_nodes": n_internal, "depth": h.depth,
         "compression": round(nnz / max(1, h.nnz), 3),
         "ms_step_bf16": round(ms16, 4), "ms_step_fp32": round(ms32, 4),
+        "ms_step_bf16_graph_replay": round(ms16_graph, 4), "ms_step_fp32_graph_replay": round(ms32_graph, 4),
         "ms_phase_bf16": {k: round(v, 4) for k, v in ph.items()},
         "ms_step_rocsparse_csr_fp32": round(ms_csr, 4), "rocsparse_alg": best_alg,
         "ms_rocsparse_fwd_by_alg": {k: round(v, 4) for k, v in alg_ms.items()},
@@ -209,8 +249,9 @@ def measure(name, w, ei, m, args, dev, stream):
         "gbs_rocsparse_csr": round(2 * csr_bytes(n, nnz, f) / (ms_csr * 1e-3) / 1e9, 1),
         "bytes_step_bf16": b16["step"], "bytes_phase_bf16": b16,
         "dominant_phase": dom, "dominant_gbs": round(dom_gbs, 1),
+        "max_rel_error_bf16_vs_fp32_cbm": vs_fp32,
     }
-    return rec, out, (x32, g32, x16, g16)
+    return rec, out, (x32, g32, x16, g16), m_csr
 
 
 def gcn_step(args, dev, stream):
@@ -274,16 +315,20 @@ def _csr_of(ei, n):
     return crow.to(torch.int32), ei[1].to(torch.int32), torch.ones(ei.shape[1])
 
 
-def cpu_baseline(ei, n, f, gpu_out, inputs, budget_s=20.0):
-    """torch.sparse CSR FP32 A @ X and A^T @ G (MKL) on the host cores, on the
-    full graph, for as many steps as fit in the budget; then one FP64 product
-    on the host, against which the GPU results' max relative error is taken."""
+def _host_csr(t, dtype):
+    """A CPU CSR tensor with int64 indices and values of `dtype`."""
+    import torch
+    return torch.sparse_csr_tensor(t.crow_indices().to(torch.int64), t.col_indices().to(torch.int64),
+                                   t.values().to(dtype), t.shape)
+
+
+def cpu_baseline(m_csr, n, f, inputs, budget_s=20.0):
+    """torch.sparse CSR FP32 M @ X and M^T @ G (MKL) on the host cores, on the
+    full graph, for as many steps as fit in the budget (at most 10)."""
     import torch
     cores = len(os.sched_getaffinity(0))
     torch.set_num_threads(cores)
-    idx = ei.to(torch.int64)
-    a = torch.sparse_csr_tensor(*[t.to(torch.int64) if i < 2 else t for i, t in enumerate(_csr_of(ei, n))], (n, n))
-    at = torch.sparse_coo_tensor(idx.flip(0), torch.ones(idx.shape[1]), (n, n)).coalesce().to_sparse_csr()
+    a, at = _host_csr(m_csr[0], torch.float32), _host_csr(m_csr[1], torch.float32)
     x32, g32 = inputs[0].cpu(), inputs[1].cpu()
     _ = a @ x32[:, :8]  # warm MKL
     times = []
@@ -296,21 +341,29 @@ def cpu_baseline(ei, n, f, gpu_out, inputs, budget_s=20.0):
         if time.perf_counter() > t_end or len(times) >= 10:
             break
     ms = 1e3 * sum(times) / len(times)
+    del y, dx
+    return {"value": round(ms, 3), "unit": "ms per M@X fwd+bwd (FP32)", "cores": cores, "kind": "port",
+            "sample": f"full graph N={n} F={f}, {len(times)} steps of torch.sparse CSR M@X + M^T@G (MKL, FP32)"}
+
+
+def fp64_errors(m_csr, gpu_out, inputs):
+    """Max relative error (max-norm) of the GPU results against FP64 products
+    on the host: FP32 inputs, BF16-rounded inputs, and the BF16 results
+    against FP64 of the original FP32 inputs (end to end)."""
+    import torch
+    a64, at64 = _host_csr(m_csr[0], torch.float64), _host_csr(m_csr[1], torch.float64)
+    x32, g32 = inputs[0].cpu(), inputs[1].cpu()
+
+    def rel(got, ref):
+        return float((got.double().cpu() - ref).abs().max() / ref.abs().max())
+
     errors = {}
-    a64, at64 = a.to(torch.float64), at.to(torch.float64)
     for tag, xx, gg in (("fp32", x32, g32), ("bf16", inputs[2].float().cpu(), inputs[3].float().cpu())):
         ry, rd = a64 @ xx.double(), at64 @ gg.double()
-        gy, gd = (t.double().cpu() for t in gpu_out[tag])
-        errors[tag] = {"forward": float((gy - ry).abs().max() / ry.abs().max()),
-                       "adjoint": float((gd - rd).abs().max() / rd.abs().max())}
-    # BF16 results against FP64 of the original FP32 inputs (end-to-end)
-    gy, gd = (t.double().cpu() for t in gpu_out["bf16"])
+        errors[tag] = {"forward": rel(gpu_out[tag][0], ry), "adjoint": rel(gpu_out[tag][1], rd)}
     ry, rd = a64 @ x32.double(), at64 @ g32.double()
-    errors["bf16_vs_fp32_inputs"] = {"forward": float((gy - ry).abs().max() / ry.abs().max()),
-                                     "adjoint": float((gd - rd).abs().max() / rd.abs().max())}
-    del y, dx
-    return {"value": round(ms, 3), "unit": "ms per A@X fwd+bwd (FP32)", "cores": cores, "kind": "port",
-            "sample": f"full graph N={n} F={f}, {len(times)} steps of torch.sparse CSR A@X + A^T@G (MKL, FP32)"}, errors
+    errors["bf16_vs_fp32_inputs"] = {"forward": rel(gpu_out["bf16"][0], ry), "adjoint": rel(gpu_out["bf16"][1], rd)}
+    return errors
 
 
 def main():
@@ -357,20 +410,22 @@ def main():
     if args.profile:
         profile_steps(w, m, args, dev, stream)
         return
-    rec, gpu_out, inputs = measure(args.workload, w, ei, m, args, dev, stream)
-    cpu, errors = None, None
+    rec, gpu_out, inputs, m_csr = measure(args.workload, w, ei, m, args, dev, stream)
+    cpu = None
     if not args.no_cpu:
         t0 = time.perf_counter()
-        cpu, errors = cpu_baseline(ei, w["n"], w["cols"], gpu_out, inputs)
-        rec["max_rel_error_vs_fp64"] = errors
+        cpu = cpu_baseline(m_csr, w["n"], w["cols"], inputs)
         log(f"CPU baseline {cpu['value']} ms on {cpu['cores']} cores ({time.perf_counter() - t0:.1f}s)")
-    del gpu_out, inputs, m
+        rec["max_rel_error_vs_fp64"] = fp64_errors(m_csr, gpu_out, inputs)
+    del gpu_out, inputs, m, m_csr
 
     secondary = None
     if args.secondary and args.secondary != args.workload:
         w2, ei2, m2 = build_workload(args.secondary, None, None, dev)
-        secondary, _, _ = measure(args.secondary, w2, ei2, m2, args, dev, stream)
-        del m2
+        secondary, out2, in2, m_csr2 = measure(args.secondary, w2, ei2, m2, args, dev, stream)
+        if not args.no_cpu:
+            secondary["max_rel_error_vs_fp64"] = fp64_errors(m_csr2, out2, in2)
+        del m2, out2, in2, m_csr2
 
     ms16 = rec["ms_step_bf16"]
     dom = rec["dominant_phase"]
@@ -378,7 +433,7 @@ def main():
     out = {
         "metric": "A@X fwd+bwd ms & HBM GB/s (% roofline), CBM-HIP vs CSR on-GPU, FP32 & BF16",
         "value": ms16,
-        "unit": "ms per A@X fwd+bwd (BF16 storage, FP32 accumulate)",
+        "unit": "ms per fwd+bwd step (BF16 storage, FP32 accumulate)",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,

@@ -55,13 +55,24 @@ class CbmDesc(ctypes.Structure):
         ("fck_gslot", c_vp),
         ("fred_gslot", c_vp),
         ("fwd_gslots", c_i32),
+        ("rev_nlong_host", c_vp),
+        ("rck_ptr_host", c_vp),
+        ("rred_ptr_host", c_vp),
+        ("rck_pos", c_vp),
+        ("rck_meta", c_vp),
+        ("rck_slot", c_vp),
+        ("rred_pos", c_vp),
+        ("rred_slot", c_vp),
+        ("rred_cnt", c_vp),
+        ("rev_slots", c_i32),
     ]
 
 # Device arrays of the plan that the descriptor points at.
 DEVICE_ARRAYS = ("lvl_node", "lvl_parent", "lvl_ppos", "fwd_ptr", "fwd_col", "rev_node", "ch_ptr", "ch_node", "ch_mid", "fck_pos", "fck_meta",
                  "fck_slot", "fred_pos", "fred_slot", "fred_cnt", "bwd_node", "bwd_ptr", "bwd_row", "bwd_mid", "bck_pos",
-                 "bck_meta", "bck_slot", "bred_pos", "bred_slot", "bred_cnt", "fck_gslot", "fred_gslot")
-HOST_ARRAYS = ("lvl_ptr", "lvl_nlong", "fck_ptr", "fred_ptr", "rev_ptr")
+                 "bck_meta", "bck_slot", "bred_pos", "bred_slot", "bred_cnt", "fck_gslot", "fred_gslot", "rck_pos", "rck_meta", "rck_slot", "rred_pos",
+                 "rred_slot", "rred_cnt")
+HOST_ARRAYS = ("lvl_ptr", "lvl_nlong", "fck_ptr", "fred_ptr", "rev_ptr", "rev_nlong", "rck_ptr", "rred_ptr")
 
 
 def declare(lib, sig):

@@ -122,7 +122,7 @@ class cusparseCBM:
         desc = CbmDesc(n_rows=self.num_rows, n_levels=self.depth, nnz=self.nnz, n_edges=self.n_edges,
                        mode=self.mode, fwd_slots=st["fwd_slots"], bwd_nlong=st["bwd_nlong"],
                        n_bck=int(self._dev["bck_pos"].numel()), n_bred=int(self._dev["bred_pos"].numel()),
-                       bwd_slots=st["bwd_slots"], fwd_gslots=st["fwd_gslots"], d=self._d.data_ptr() if self._d is not None else None, **ptrs)
+                       bwd_slots=st["bwd_slots"], fwd_gslots=st["fwd_gslots"], rev_slots=st["rev_slots"], d=self._d.data_ptr() if self._d is not None else None, **ptrs)
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             rc = self._lib.cbm_create(ctypes.byref(desc), current_stream_handle(self.device), ctypes.byref(h))

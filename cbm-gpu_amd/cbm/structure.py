@@ -14,8 +14,9 @@ _STAT_NAMES = ("n", "nnz_a", "nnz_delta", "candidates", "msa_weight", "tree_edge
 _PLAN_NAMES = ("lvl_ptr", "lvl_node", "lvl_parent", "fwd_ptr", "fwd_col", "bwd_ptr", "bwd_row", "ch_ptr",
                "ch_node", "lvl_nlong", "fck_ptr", "fck_pos", "fck_meta", "fck_slot", "fred_ptr", "fred_pos",
                "fred_slot", "fred_cnt", "bwd_node", "bck_pos", "bck_meta", "bck_slot", "bred_pos", "bred_slot",
-               "bred_cnt", "rev_ptr", "rev_node", "bwd_mid", "ch_mid", "lvl_ppos", "fck_gslot", "fred_gslot")
-_PLAN_STATS = ("depth", "fwd_slots", "bwd_slots", "bwd_nlong", "fwd_gslots")
+               "bred_cnt", "rev_ptr", "rev_node", "bwd_mid", "ch_mid", "lvl_ppos", "fck_gslot", "fred_gslot", "rev_nlong", "rck_ptr", "rck_pos", "rck_meta", "rck_slot",
+               "rred_ptr", "rred_pos", "rred_slot", "rred_cnt")
+_PLAN_STATS = ("depth", "fwd_slots", "bwd_slots", "bwd_nlong", "fwd_gslots", "rev_slots")
 
 
 def _copy_out(ptr, n, dtype):

@@ -16,7 +16,7 @@
 struct cbm_handle {
     cbm_desc d{};
     // host copies of the per-level tables
-    std::vector<int32_t> lvl_ptr, lvl_nlong, fck_ptr, fred_ptr, rev_ptr;
+    std::vector<int32_t> lvl_ptr, lvl_nlong, fck_ptr, fred_ptr, rev_ptr, rev_nlong, rck_ptr, rred_ptr;
     float *ps_pos = nullptr;  // D A D only: d[node] / d[parent] per level position
     float *scratch = nullptr; // partial sums of long-row chunks
     int64_t scratch_floats = 0;
@@ -110,7 +110,7 @@ int ensure_comp(cbm_handle *h, int32_t n_cols) {
 int ensure_scratch(cbm_handle *h, int32_t n_cols) {
     const int rc = ensure_comp(h, n_cols);
     if (rc != CBM_OK) return rc;
-    const int64_t slots = std::max<int64_t>({h->d.fwd_slots, h->d.bwd_slots, h->d.fwd_gslots});
+    const int64_t slots = std::max<int64_t>({h->d.fwd_slots, h->d.bwd_slots, h->d.fwd_gslots, h->d.rev_slots});
     const int64_t need = slots * int64_t(n_cols);
     if (need <= h->scratch_floats) return CBM_OK;
     if (h->scratch) {
@@ -295,7 +295,8 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
     }
     *out = nullptr;
     const cbm_desc &d = *desc;
-    const bool tables = d.lvl_ptr_host && d.lvl_nlong_host && d.fck_ptr_host && d.fred_ptr_host && d.rev_ptr_host;
+    const bool tables = d.lvl_ptr_host && d.lvl_nlong_host && d.fck_ptr_host && d.fred_ptr_host && d.rev_ptr_host &&
+                        d.rev_nlong_host && d.rck_ptr_host && d.rred_ptr_host;
     if (d.n_rows < 1 || d.n_levels < 1 || d.nnz < 0 || d.n_edges < 0 || d.n_edges >= d.n_rows || !tables ||
         !d.lvl_node || !d.lvl_parent || !d.lvl_ppos || !d.fwd_ptr || !d.bwd_ptr || !d.bwd_node || !d.bwd_mid || !d.ch_ptr ||
         (d.nnz > 0 && (!d.fwd_col || !d.bwd_row)) || (d.n_edges > 0 && (!d.ch_node || !d.rev_node || !d.ch_mid)) ||
@@ -317,7 +318,7 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
     }
     const int32_t L = d.n_levels;
     if (d.lvl_ptr_host[0] != 0 || d.lvl_ptr_host[L] != d.n_rows || d.fck_ptr_host[0] != 0 || d.fred_ptr_host[0] != 0 ||
-        d.rev_ptr_host[0] != 0 || d.rev_ptr_host[L] > d.n_rows) {
+        d.rev_ptr_host[0] != 0 || d.rev_ptr_host[L] > d.n_rows || d.rck_ptr_host[0] != 0 || d.rred_ptr_host[0] != 0) {
         set_error("cbm_create: level tables do not start at 0 / end at n_rows");
         return CBM_ERR_ARG;
     }
@@ -325,14 +326,18 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
         if (d.lvl_ptr_host[l + 1] <= d.lvl_ptr_host[l] || d.lvl_nlong_host[l] < 0 ||
             d.lvl_nlong_host[l] > d.lvl_ptr_host[l + 1] - d.lvl_ptr_host[l] ||
             d.fck_ptr_host[l + 1] < d.fck_ptr_host[l] || d.fred_ptr_host[l + 1] < d.fred_ptr_host[l] ||
-            d.rev_ptr_host[l + 1] < d.rev_ptr_host[l]) {
+            d.rev_ptr_host[l + 1] < d.rev_ptr_host[l] || d.rev_nlong_host[l] < 0 ||
+            d.rev_nlong_host[l] > d.rev_ptr_host[l + 1] - d.rev_ptr_host[l] ||
+            d.rck_ptr_host[l + 1] < d.rck_ptr_host[l] || d.rred_ptr_host[l + 1] < d.rred_ptr_host[l]) {
             set_error("cbm_create: inconsistent level tables at level %d", l);
             return CBM_ERR_ARG;
         }
     }
     if ((d.fck_ptr_host[L] > 0 && (!d.fck_pos || !d.fck_meta || !d.fck_slot || !d.fck_gslot)) ||
         (d.fred_ptr_host[L] > 0 && (!d.fred_pos || !d.fred_slot || !d.fred_cnt || !d.fred_gslot)) ||
-        d.fwd_gslots < 0) {
+        d.fwd_gslots < 0 || d.rev_slots < 0 ||
+        (d.rck_ptr_host[L] > 0 && (!d.rck_pos || !d.rck_meta || !d.rck_slot)) ||
+        (d.rred_ptr_host[L] > 0 && (!d.rred_pos || !d.rred_slot || !d.rred_cnt))) {
         set_error("cbm_create: missing forward chunk tables");
         return CBM_ERR_ARG;
     }
@@ -352,6 +357,12 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
     h->d.lvl_nlong_host = h->lvl_nlong.data();
     h->d.fck_ptr_host = h->fck_ptr.data();
     h->d.fred_ptr_host = h->fred_ptr.data();
+    h->rev_nlong.assign(d.rev_nlong_host, d.rev_nlong_host + L);
+    h->rck_ptr.assign(d.rck_ptr_host, d.rck_ptr_host + L + 1);
+    h->rred_ptr.assign(d.rred_ptr_host, d.rred_ptr_host + L + 1);
+    h->d.rev_nlong_host = h->rev_nlong.data();
+    h->d.rck_ptr_host = h->rck_ptr.data();
+    h->d.rred_ptr_host = h->rred_ptr.data();
     if (d.mode == CBM_MODE_DADX) {
         hipError_t e = hipMalloc(&h->ps_pos, sizeof(float) * size_t(d.n_rows));
         if (e == hipSuccess) e = cbmk::launch_dad_scales(d.n_rows, d.lvl_node, d.lvl_parent, d.d, h->ps_pos, stream);
@@ -449,13 +460,25 @@ int adjoint_reverse(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, fl
     rv.y = z;
     rv.ldy = ldz;
     rv.ncols = n_cols;
+    cbmk::ChunkArgs ck;
+    ck.pos = h->d.rck_pos;
+    ck.meta = h->d.rck_meta;
+    ck.slot = h->d.rck_slot;
+    ck.scratch = h->scratch;
+    cbmk::ReduceArgs rd;
+    rd.pos = h->d.rred_pos;
+    rd.slot = h->d.rred_slot;
+    rd.cnt = h->d.rred_cnt;
+    rd.scratch = h->scratch;
     for (int32_t l = h->d.n_levels - 1; l >= 0; --l) {
-        rv.p_begin = h->rev_ptr[size_t(l)];
-        rv.p_end = h->rev_ptr[size_t(l) + 1];
-        if (rv.p_end <= rv.p_begin) continue;
-        const hipError_t e = cbmk::launch_row_pass(rv, stream);
-        if (e != hipSuccess) return device_error(e, "adjoint reverse pass");
-        ++h->last_launches;
+        const int32_t b = h->rev_ptr[size_t(l)], e = h->rev_ptr[size_t(l) + 1];
+        if (e <= b) continue;
+        ck.first = h->rck_ptr[size_t(l)];
+        ck.count = h->rck_ptr[size_t(l) + 1] - ck.first;
+        rd.first = h->rred_ptr[size_t(l)];
+        rd.count = h->rred_ptr[size_t(l) + 1] - rd.first;
+        const int rc = run_range(h, rv, b, e, h->rev_nlong[size_t(l)], ck, rd, stream, "adjoint reverse pass");
+        if (rc != CBM_OK) return rc;
     }
     return CBM_OK;
 }
@@ -517,6 +540,7 @@ extern "C" int cbm_adjoint_reverse(cbm_handle *h, int32_t dtype, const void *g, 
                                    int32_t n_cols, hipStream_t stream) {
     // This half reads g and writes z only.
     int rc = check_dense(h, dtype, g, ldg, z, ldz, n_cols, "cbm_adjoint_reverse");
+    if (rc == CBM_OK) rc = ensure_scratch(h, n_cols);
     if (rc != CBM_OK) return rc;
     h->last_launches = 0;
     return adjoint_reverse(h, dtype, g, ldg, z, ldz, n_cols, stream);

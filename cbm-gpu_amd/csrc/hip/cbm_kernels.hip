@@ -140,10 +140,13 @@ struct Pass {
 // by the LPR lanes of one group (all lanes of the group call it with the same
 // kb, ke; lanes whose columns are past the end pass active = false but still
 // take part in the index broadcasts).  scale and val may be null (1).
-template <typename TS, int VE, int LPR>
-__device__ inline void gather_src(const RowPass &a, const TS *__restrict__ src, int64_t ld,
-                                  const float *__restrict__ scale, const float *__restrict__ val, int32_t kb,
-                                  int32_t ke, int l, int32_t c0, bool active, float (&acc)[VE]) {
+// SCALED: whether scale or val is given; without them an entry's factor is
+// +-1 from the sign bit of the broadcast index itself, so only the index
+// crosses lanes.
+template <typename TS, int VE, int LPR, bool SCALED>
+__device__ inline void gather_src_impl(const RowPass &a, const TS *__restrict__ src, int64_t ld,
+                                       const float *__restrict__ scale, const float *__restrict__ val, int32_t kb,
+                                       int32_t ke, int l, int32_t c0, bool active, float (&acc)[VE]) {
     if constexpr (LPR < 4) {
         // Groups narrower than four lanes fetch four entries per step, 4 / LPR
         // per lane (entry base + m * LPR + l), so that four gathers are still
@@ -159,9 +162,11 @@ __device__ inline void gather_src(const RowPass &a, const TS *__restrict__ src, 
                 mine_scale[m] = 0.0f;
                 if (e < ke) {
                     mine[m] = a.idx[e];
-                    float cs = scale ? scale[mine[m] & kIndexMask] : 1.0f;
-                    if (val) cs *= val[e];
-                    mine_scale[m] = signed_scale(cs, mine[m]);
+                    if constexpr (SCALED) {
+                        float cs = scale ? scale[mine[m] & kIndexMask] : 1.0f;
+                        if (val) cs *= val[e];
+                        mine_scale[m] = signed_scale(cs, mine[m]);
+                    }
                 }
             }
             const int32_t cnt = min(4, ke - base);
@@ -169,8 +174,13 @@ __device__ inline void gather_src(const RowPass &a, const TS *__restrict__ src, 
             float sc[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                j[u] = group_bcast<LPR>(mine[u / LPR], u % LPR) & kIndexMask;
-                sc[u] = group_bcast<LPR>(mine_scale[u / LPR], u % LPR);
+                const int32_t raw = group_bcast<LPR>(mine[u / LPR], u % LPR);
+                j[u] = raw & kIndexMask;
+                if constexpr (SCALED) {
+                    sc[u] = group_bcast<LPR>(mine_scale[u / LPR], u % LPR);
+                } else {
+                    sc[u] = signed_scale(1.0f, raw);
+                }
             }
             if (active) {
                 float v[4][VE];
@@ -196,9 +206,11 @@ __device__ inline void gather_src(const RowPass &a, const TS *__restrict__ src, 
         float mine_scale = 0.0f;
         if (base + l < ke) {
             mine = a.idx[base + l];
-            float cs = scale ? scale[mine & kIndexMask] : 1.0f;
-            if (val) cs *= val[base + l];
-            mine_scale = signed_scale(cs, mine);
+            if constexpr (SCALED) {
+                float cs = scale ? scale[mine & kIndexMask] : 1.0f;
+                if (val) cs *= val[base + l];
+                mine_scale = signed_scale(cs, mine);
+            }
         }
         const int32_t cnt = min(LPR, ke - base);
         int32_t k = 0;
@@ -207,8 +219,13 @@ __device__ inline void gather_src(const RowPass &a, const TS *__restrict__ src, 
             float sc[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                j[u] = group_bcast<LPR>(mine, k + u) & kIndexMask;
-                sc[u] = group_bcast<LPR>(mine_scale, k + u);
+                const int32_t raw = group_bcast<LPR>(mine, k + u);
+                j[u] = raw & kIndexMask;
+                if constexpr (SCALED) {
+                    sc[u] = group_bcast<LPR>(mine_scale, k + u);
+                } else {
+                    sc[u] = signed_scale(1.0f, raw);
+                }
             }
             if (active) {
                 float v[4][VE];
@@ -221,8 +238,14 @@ __device__ inline void gather_src(const RowPass &a, const TS *__restrict__ src, 
             }
         }
         for (; k < cnt; ++k) {
-            const int32_t j = group_bcast<LPR>(mine, k) & kIndexMask;
-            const float sc = group_bcast<LPR>(mine_scale, k);
+            const int32_t raw = group_bcast<LPR>(mine, k);
+            const int32_t j = raw & kIndexMask;
+            float sc;
+            if constexpr (SCALED) {
+                sc = group_bcast<LPR>(mine_scale, k);
+            } else {
+                sc = signed_scale(1.0f, raw);
+            }
             if (active) {
                 float v[VE];
                 Load<TS, VE>::run(src + int64_t(j) * ld + c0, v);
@@ -230,6 +253,17 @@ __device__ inline void gather_src(const RowPass &a, const TS *__restrict__ src, 
                 for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], sc, acc[q]);
             }
         }
+    }
+}
+
+template <typename TS, int VE, int LPR>
+__device__ inline void gather_src(const RowPass &a, const TS *__restrict__ src, int64_t ld,
+                                  const float *__restrict__ scale, const float *__restrict__ val, int32_t kb,
+                                  int32_t ke, int l, int32_t c0, bool active, float (&acc)[VE]) {
+    if (scale || val) {
+        gather_src_impl<TS, VE, LPR, true>(a, src, ld, scale, val, kb, ke, l, c0, active, acc);
+    } else {
+        gather_src_impl<TS, VE, LPR, false>(a, src, ld, scale, val, kb, ke, l, c0, active, acc);
     }
 }
 

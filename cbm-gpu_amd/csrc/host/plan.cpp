@@ -301,15 +301,24 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
             auto &cptr = plan->arr[CBM_P_CH_PTR];
             auto &cnode = plan->arr[CBM_P_CH_NODE];
             auto has_kids = [&](int32_t u) { return kptr[size_t(u) + 1] > kptr[size_t(u)]; };
-            rptr.assign(size_t(levels) + 1, 0);
+            auto many_kids = [&](int32_t u) { return kptr[size_t(u) + 1] - kptr[size_t(u)] > CBM_LONG_ROW; };
+            // Counting sort on (depth, short?): long child lists first in
+            // every level, node ids ascending within each part.
+            std::vector<int32_t> rkey(size_t(2 * levels) + 1, 0);
             for (int32_t v = 0; v < n; ++v)
-                if (has_kids(v)) ++rptr[size_t(depth[size_t(v)]) + 1];
-            std::partial_sum(rptr.begin(), rptr.end(), rptr.begin());
+                if (has_kids(v)) ++rkey[size_t(2 * depth[size_t(v)] + (many_kids(v) ? 0 : 1)) + 1];
+            std::partial_sum(rkey.begin(), rkey.end(), rkey.begin());
+            rptr.resize(size_t(levels) + 1);
+            for (int32_t l = 0; l <= levels; ++l) rptr[size_t(l)] = rkey[size_t(2 * l)];
+            auto &rnlong = plan->arr[CBM_P_REV_NLONG];
+            rnlong.resize(size_t(levels));
+            for (int32_t l = 0; l < levels; ++l) rnlong[size_t(l)] = rkey[size_t(2 * l + 1)] - rkey[size_t(2 * l)];
             rnode.resize(size_t(rptr[size_t(levels)]));
             {
-                std::vector<int32_t> fill(rptr.begin(), rptr.end() - 1);
+                std::vector<int32_t> fill(rkey.begin(), rkey.end() - 1);
                 for (int32_t v = 0; v < n; ++v)
-                    if (has_kids(v)) rnode[size_t(fill[size_t(depth[size_t(v)])]++)] = v;
+                    if (has_kids(v))
+                        rnode[size_t(fill[size_t(2 * depth[size_t(v)] + (many_kids(v) ? 0 : 1))]++)] = v;
             }
             auto &cmid = plan->arr[CBM_P_CH_MID];
             cptr.assign(rnode.size() + 1, 0);
@@ -331,6 +340,25 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
                     }
                 }
             }
+            // Chunk and reduction tables of the long child lists, per level.
+            Split r;
+            auto &rck_ptr = plan->arr[CBM_P_RCK_PTR];
+            auto &rred_ptr = plan->arr[CBM_P_RRED_PTR];
+            rck_ptr.assign(size_t(levels) + 1, 0);
+            rred_ptr.assign(size_t(levels) + 1, 0);
+            int32_t max_slots = 0;
+            for (int32_t l = 0; l < levels; ++l) {
+                max_slots = std::max(max_slots, split_long_rows(cptr, rptr[size_t(l)], rnlong[size_t(l)], r));
+                rck_ptr[size_t(l) + 1] = int32_t(r.ck_pos.size());
+                rred_ptr[size_t(l) + 1] = int32_t(r.red_pos.size());
+            }
+            plan->arr[CBM_P_RCK_POS].swap(r.ck_pos);
+            plan->arr[CBM_P_RCK_META].swap(r.ck_meta);
+            plan->arr[CBM_P_RCK_SLOT].swap(r.ck_slot);
+            plan->arr[CBM_P_RRED_POS].swap(r.red_pos);
+            plan->arr[CBM_P_RRED_SLOT].swap(r.red_slot);
+            plan->arr[CBM_P_RRED_CNT].swap(r.red_cnt);
+            plan->stat[CBM_PS_REV_SLOTS] = max_slots;
         }
         *out = plan.release();
         return CBM_OK;

@@ -110,6 +110,19 @@ typedef struct cbm_desc {
     const int32_t *fck_gslot;
     const int32_t *fred_gslot;
     int32_t fwd_gslots;
+    /* the adjoint's reverse pass: host per-level tables (long child lists per
+       level [L], chunk offsets [L+1], reduction-row offsets [L+1]), device
+       chunk and reduction tables, and the slots its largest level needs */
+    const int32_t *rev_nlong_host;
+    const int32_t *rck_ptr_host;
+    const int32_t *rred_ptr_host;
+    const int32_t *rck_pos;
+    const int32_t *rck_meta;
+    const int32_t *rck_slot;
+    const int32_t *rred_pos;
+    const int32_t *rred_slot;
+    const int32_t *rred_cnt;
+    int32_t rev_slots;
 } cbm_desc;
 
 typedef struct cbm_handle cbm_handle;
@@ -130,7 +143,7 @@ int cbm_destroy(cbm_handle *h);
 
 /*
  * Sizes the handle's scratch for products of up to n_cols columns: the
- * partial sums of long rows, max(fwd_slots, bwd_slots, fwd_gslots) x n_cols
+ * partial sums of long rows, max(fwd_slots, bwd_slots, fwd_gslots, rev_slots) x n_cols
  * FP32, and the compensation rows of the compensated levels.  A
  * product call with more columns than reserved grows the scratch itself
  * (one hipMalloc); reserve first when the calls must not allocate (graph

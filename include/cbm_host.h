@@ -153,14 +153,25 @@ enum {
     CBM_P_BRED_SLOT = 23,  /* int32 [Rb]   first slot                                                 */
     CBM_P_BRED_CNT = 24,   /* int32 [Rb]   number of slots                                            */
     CBM_P_REV_PTR = 25,    /* int32 [L+1]  offsets per level into REV_NODE                                 */
-    CBM_P_REV_NODE = 26,   /* int32 [I]    internal nodes (those with children) by level, id ascending     */
+    CBM_P_REV_NODE = 26,   /* int32 [I]    internal nodes (those with children) by level; within a level,
+                                           those with more than CBM_LONG_ROW children first, then id ascending */
     CBM_P_BWD_MID = 27,    /* int32 [N]    first internal-node entry of each backward row              */
     CBM_P_CH_MID = 28,     /* int32 [I]    first internal child of each internal-node position         */
     CBM_P_LVL_PPOS = 29,   /* int32 [N]    level position of the parent of each position, -1 for level 0 */
     CBM_P_FCK_GSLOT = 30,  /* int32 [C]    partial slot of each forward chunk numbered over all levels
                                            (one pass over every level at once), -1 for a one-chunk row */
     CBM_P_FRED_GSLOT = 31, /* int32 [R]    first such slot of each forward reduction row               */
-    CBM_P_N_ARRAYS = 32
+    CBM_P_REV_NLONG = 32,  /* int32 [L]    internal nodes with long child lists at the start of each level
+                                           of REV_NODE                                                  */
+    CBM_P_RCK_PTR = 33,    /* int32 [L+1]  reverse-pass chunk offsets per level                         */
+    CBM_P_RCK_POS = 34,    /* int32 [Cr]   internal-node position of each reverse chunk                 */
+    CBM_P_RCK_META = 35,   /* int32 [Cr]   chunk index / count                                           */
+    CBM_P_RCK_SLOT = 36,   /* int32 [Cr]   partial slot or -1                                            */
+    CBM_P_RRED_PTR = 37,   /* int32 [L+1]  offsets per level of the reverse rows that need a reduction   */
+    CBM_P_RRED_POS = 38,   /* int32 [Rr]   their internal-node positions                                 */
+    CBM_P_RRED_SLOT = 39,  /* int32 [Rr]   first slot                                                    */
+    CBM_P_RRED_CNT = 40,   /* int32 [Rr]   number of slots                                               */
+    CBM_P_N_ARRAYS = 41
 };
 
 /* Scalars of a cbm_plan, addressed by cbm_plan_stat(). */
@@ -170,7 +181,8 @@ enum {
     CBM_PS_BWD_SLOTS = 2,  /* partial slots needed by the backward pass              */
     CBM_PS_BWD_NLONG = 3,  /* long rows at the start of the backward order           */
     CBM_PS_FWD_GSLOTS = 4, /* partial slots of all forward levels together (FCK_GSLOT) */
-    CBM_PS_N_STATS = 5
+    CBM_PS_REV_SLOTS = 5,  /* partial slots needed by the largest reverse-pass level    */
+    CBM_PS_N_STATS = 6
 };
 
 #define CBM_LONG_ROW 256

@@ -122,3 +122,43 @@ def test_compensation_from_level_zero_equals_level_one(hub, cuda):
     h.set_kahan_depth(32)
     h.set_forward_split(None)
     assert all(torch.equal(out[0], o) for o in out[1:])
+
+
+@pytest.fixture(scope="module")
+def star(cuda):
+    from hubgraph import star_graph
+    ei, n = star_graph()
+    a = O.csr_of(ei.numpy(), n)
+    mats = {mode: cls(ei, torch.ones(ei.shape[1]), alpha=0, device=cuda) for mode, cls in CLASSES.items()}
+    assert mats["ax"].cusparse_extension.plan_stats["rev_slots"] >= 3  # a multi-chunk child list
+    return a, n, mats
+
+
+@pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
+@pytest.mark.parametrize("dtype,f", [(torch.float32, 1), (torch.float32, 64), (torch.float32, 260),
+                                     (torch.bfloat16, 16), (torch.bfloat16, 256)])
+def test_chunked_child_lists_in_the_reverse_pass(star, cuda, mode, dtype, f):
+    """The adjoint over a node with 2500 children (its child list split into
+    chunks and reduced in order) against the FP64 oracle, forward too."""
+    a, n, mats = star
+    m = mats[mode]
+    xt = torch.from_numpy(_x((n, f), 40 + f)).to(cuda).to(dtype)
+    xe = xt.float().cpu().numpy()
+    dx = torch.empty(n, f, device=cuda)
+    m.matmul_t(xt, dx)
+    assert O.max_rel_error(dx.cpu().numpy(), O.adjoint(a, xe, mode)) <= 1e-5
+    y = torch.empty(n, f, device=cuda)
+    m.matmul(xt, y)
+    assert O.max_rel_error(y.cpu().numpy(), O.forward(a, xe, mode)) <= 1e-5
+
+
+def test_chunked_child_lists_integer_exact(star, cuda):
+    a, n, mats = star
+    h = mats["ax"].cusparse_extension
+    x = np.random.default_rng(9).integers(-3, 4, size=(n, 24)).astype(np.float32)
+    d1 = torch.empty(n, 24, device=cuda)
+    d2 = torch.empty(n, 24, device=cuda)
+    h.matmul_t(torch.from_numpy(x).to(cuda), d1)
+    h.matmul_t(torch.from_numpy(x).to(cuda), d2)
+    assert np.array_equal(d1.cpu().numpy().astype(np.float64), O.adjoint(a, x, "ax"))
+    assert torch.equal(d1, d2)

@@ -275,3 +275,27 @@ def test_plan_chunk_tables_on_long_rows():
                  long_rows=all_long)
     assert st["fwd_gslots"] == int(pl.fred_cnt.numpy().sum())
     assert np.array_equal(pl.fck_gslot.numpy() < 0, pl.fck_slot.numpy() < 0)
+
+
+def test_reverse_chunk_tables_on_a_star():
+    """A node with STAR_K children: the reverse pass's internal-node order puts
+    long child lists first in each level, and their chunk tables tile them."""
+    from hubgraph import STAR_K, star_graph
+    ei, n = star_graph()
+    b, pl = _plan_for(ei.numpy(), n, 0)
+    parent = b.parent.numpy()
+    assert np.count_nonzero(parent == 20) == STAR_K  # every copy hangs off the prototype
+    st = pl.stats
+    assert st["rev_slots"] >= (STAR_K + 1023) // 1024
+    rp, rn = pl.rev_ptr.numpy(), pl.rev_nlong.numpy()
+    cp = pl.ch_ptr.numpy()
+    rck, rred = pl.rck_ptr.numpy(), pl.rred_ptr.numpy()
+    for l in range(pl.depth):
+        lens = np.diff(cp)[rp[l]:rp[l + 1]]
+        assert np.all(lens[:rn[l]] > 256) and np.all(lens[rn[l]:] <= 256)
+        c0, c1, r0, r1 = rck[l], rck[l + 1], rred[l], rred[l + 1]
+        _check_split(cp, rp[l], rn[l], pl.rck_pos.numpy()[c0:c1], pl.rck_meta.numpy()[c0:c1],
+                     pl.rck_slot.numpy()[c0:c1], pl.rred_pos.numpy()[r0:r1], pl.rred_slot.numpy()[r0:r1],
+                     pl.rred_cnt.numpy()[r0:r1], st["rev_slots"])
+    assert rn.sum() >= 1 and rred[-1] >= 1
+
