@@ -295,6 +295,12 @@ class cusparseCBM:
         m = -1 if mode is None else int(bool(mode))
         _lib.check_hip(self._lib.cbm_set_forward_split(self._h, m), "set_forward_split")
 
+    def set_wave_rows(self, mode):
+        """Row mapping of the short-row passes: True a whole wavefront per row,
+        False rows packed per wavefront, None by matrix size (the default)."""
+        m = -1 if mode is None else int(bool(mode))
+        _lib.check_hip(self._lib.cbm_set_wave_rows(self._h, m), "set_wave_rows")
+
     def set_slab_bytes(self, nbytes):
         """Column-slab size for cache residency of the gathers (0: whole rows)."""
         _lib.check_hip(self._lib.cbm_set_slab_bytes(self._h, int(nbytes)), "set_slab_bytes")

@@ -23,6 +23,7 @@ struct cbm_handle {
     int64_t slab_bytes = 0;   // column slab target (gathered bytes), 0 = whole rows
     int32_t kahan_level = 32; // levels from here on propagate with compensation
     int32_t fwd_split = -1;   // forward form: 0 fused per level, 1 split, -1 automatic
+    int32_t wave_rows = -1;   // short-row passes: whole wavefront per row (1), never (0), by size (-1)
     float *comp = nullptr;    // compensation rows of those levels
     int64_t comp_floats = 0;
     int last_launches = 0;
@@ -279,6 +280,8 @@ cbmk::RowPass forward_pass(const cbm_handle *h, int32_t dtype, const void *x, in
     rp.y = y;
     rp.ldy = ldy;
     rp.ncols = n_cols;
+    rp.rows_hint = h->d.n_rows;
+    rp.wave_rows = h->wave_rows;
     return rp;
 }
 
@@ -460,6 +463,8 @@ int adjoint_reverse(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, fl
     rv.y = z;
     rv.ldy = ldz;
     rv.ncols = n_cols;
+    rv.rows_hint = h->d.n_rows;
+    rv.wave_rows = h->wave_rows;
     cbmk::ChunkArgs ck;
     ck.pos = h->d.rck_pos;
     ck.meta = h->d.rck_meta;
@@ -502,6 +507,8 @@ int adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, co
     tp.y = dx;
     tp.ldy = lddx;
     tp.ncols = n_cols;
+    tp.rows_hint = h->d.n_rows;
+    tp.wave_rows = h->wave_rows;
     tp.rowscale = h->d.mode == CBM_MODE_AX ? nullptr : h->d.d;
     cbmk::ChunkArgs ck;
     ck.pos = h->d.bck_pos;
@@ -595,6 +602,15 @@ extern "C" int cbm_set_forward_split(cbm_handle *h, int32_t mode) {
         return CBM_ERR_ARG;
     }
     h->fwd_split = mode;
+    return CBM_OK;
+}
+
+extern "C" int cbm_set_wave_rows(cbm_handle *h, int32_t mode) {
+    if (!h || mode < -1 || mode > 1) {
+        set_error("cbm_set_wave_rows: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    h->wave_rows = mode;
     return CBM_OK;
 }
 

@@ -354,20 +354,29 @@ __device__ inline void finish_row(const RowPass &a, int32_t p, int32_t row, int3
     store_row<VE>(a.y + int64_t(row) * a.ldy + c0, acc);
 }
 
-// Short rows: one group of LPR lanes per row, 64 / LPR rows per wavefront.
-template <class P, int VE, int LPR>
+// Short rows: one group of LPR lanes per row, 64 / LPR rows per wavefront
+// (S = 1); or S = 64 / LPR sub-groups of LPR lanes per row, one row per
+// wavefront, each sub-group summing a contiguous slice of the row's entries
+// and the slices' sums added by a butterfly across the sub-groups (the same
+// order on every lane, so every sub-group holds the same bits).
+template <class P, int VE, int LPR, int S>
 __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
-    constexpr int kGroupsPerWave = 64 / LPR;
+    constexpr int kLanesPerRow = LPR * S;
+    constexpr int kRowsPerWave = 64 / kLanesPerRow;
     const int lane = int(threadIdx.x & 63u);
     const int l = lane & (LPR - 1);
+    const int sub = (lane / LPR) & (S - 1);
     const int64_t group =
-        (int64_t(blockIdx.x) * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kGroupsPerWave + lane / LPR;
+        (int64_t(blockIdx.x) * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kRowsPerWave +
+        lane / kLanesPerRow;
     const int64_t p64 = int64_t(a.p_begin) + group;
-    if (p64 >= int64_t(a.p_end)) return;  // whole groups leave together
+    if (p64 >= int64_t(a.p_end)) return;  // a row's lanes leave together
     const int32_t p = int32_t(p64);
     const int32_t kb = a.ptr[p], ke = a.ptr_end ? a.ptr_end[p] : a.ptr[p + 1];
     if (ke - kb > a.skip_above) return;  // a long row, done by the chunk kernel
     const int32_t row = a.node_of ? a.node_of[p] : p;
+    const int64_t len = int64_t(ke) - kb;
+    const int32_t sb = kb + int32_t(len * sub / S), se = kb + int32_t(len * (sub + 1) / S);
     constexpr int32_t kSpan = LPR * VE;
     const int32_t passes = (a.ncols + kSpan - 1) / kSpan;
     for (int32_t pass = 0; pass < passes; ++pass) {
@@ -376,8 +385,14 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
         float acc[VE];
 #pragma unroll
         for (int q = 0; q < VE; ++q) acc[q] = 0.0f;
-        gather_sum<P, VE, LPR>(a, p, kb, ke, l, c0, active, acc);
-        if (active) finish_row<P, VE>(a, p, row, c0, acc);
+        gather_sum<P, VE, LPR>(a, p, sb, se, l, c0, active, acc);
+        if constexpr (S > 1) {
+#pragma unroll
+            for (int off = LPR; off < kLanesPerRow; off <<= 1)
+#pragma unroll
+                for (int q = 0; q < VE; ++q) acc[q] += __shfl_xor(acc[q], off);
+        }
+        if (active && sub == 0) finish_row<P, VE>(a, p, row, c0, acc);
     }
 }
 
@@ -577,13 +592,28 @@ struct Launch {
     const ReduceArgs *reduce;
 };
 
+// Whether a short-row pass gives each row a whole wavefront (see RowPass).
+template <int LPR>
+bool wave_per_row(const RowPass &rp) {
+    if (LPR >= 64 || rp.wave_rows == 0) return false;
+    if (rp.wave_rows > 0) return true;
+    return rp.rows_hint > 0 && int64_t(rp.rows_hint) * LPR < 64 * kWaveSlots;
+}
+
 template <class P, int VE, int LPR>
 hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
     constexpr int64_t kRowsPerBlock = 4 * (64 / LPR);
     if (lk.kind == Kind::kPass) {
         const int64_t n = int64_t(rp.p_end) - rp.p_begin;
         if (n <= 0) return hipSuccess;
-        k_row_pass<P, VE, LPR><<<dim3(unsigned((n + kRowsPerBlock - 1) / kRowsPerBlock)), dim3(256), 0, stream>>>(rp);
+        if constexpr (LPR < 64) {
+            if (wave_per_row<LPR>(rp)) {
+                k_row_pass<P, VE, LPR, 64 / LPR><<<dim3(unsigned((n + 3) / 4)), dim3(256), 0, stream>>>(rp);
+                return hipGetLastError();
+            }
+        }
+        k_row_pass<P, VE, LPR, 1><<<dim3(unsigned((n + kRowsPerBlock - 1) / kRowsPerBlock)), dim3(256), 0, stream>>>(
+            rp);
     } else if (lk.kind == Kind::kChunk) {
         if (lk.chunk->count <= 0) return hipSuccess;
         constexpr int kBs = (16 * LPR > 256) ? 16 * LPR : 256;

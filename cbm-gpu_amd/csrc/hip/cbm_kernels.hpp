@@ -60,7 +60,19 @@ struct RowPass {
     // The short-row pass skips positions with more entries than this (rows
     // that the chunk kernel covers when one pass spans several levels).
     int32_t skip_above = INT32_MAX;
+    // Rows of the whole product (0: unknown).  When rows x LPR lanes cannot
+    // fill the device's wave slots, the short-row pass gives each row a whole
+    // wavefront: 64 / LPR sub-groups sum slices of its entries, which are
+    // then added across sub-groups.  This trades occupancy that would idle
+    // anyway for fewer dependent gather rounds per row.  wave_rows: -1 by
+    // that rule, 0 never, 1 always.
+    int32_t rows_hint = 0;
+    int32_t wave_rows = -1;
 };
+
+// Wave slots of the device (256 CUs x 4 SIMDs x 8 waves): a pass with fewer
+// rows x lanes than 64 x this leaves slots free.
+constexpr int64_t kWaveSlots = 8192;
 
 // Launches one row pass over positions [p_begin, p_end); returns the launch status.
 hipError_t launch_row_pass(const RowPass &rp, hipStream_t stream);

@@ -227,6 +227,17 @@ int cbm_set_slab_bytes(cbm_handle *h, int64_t bytes);
  */
 int cbm_set_forward_split(cbm_handle *h, int32_t mode);
 
+/*
+ * Row mapping of the short-row passes: 1 gives every row a whole wavefront
+ * (its entries split into slices summed by sub-groups of lanes, then added
+ * across them), 0 packs 64 / lanes-per-row rows into a wavefront, -1 (the
+ * default) chooses the first when N rows cannot fill the device's wave slots
+ * otherwise (small matrices, where per-row gather latency, not bandwidth,
+ * sets the time).  The two mappings add a row's entries in a different
+ * order; each is deterministic.
+ */
+int cbm_set_wave_rows(cbm_handle *h, int32_t mode);
+
 /* ------------------------------------------------------------------------- */
 /* CSR comparator on the same GPU: rocSPARSE SpMM, FP32.                      */
 /* Replaces class cusparseCSR (cbm/csr.cu:24-126).                            */

@@ -86,7 +86,8 @@ def test_split_and_fused_forward_are_bit_identical(hub, cuda, mode, dtype, f):
     a, mats = hub
     h = mats[mode].cusparse_extension
     xt = torch.from_numpy(_x((HUB_N, f), 11 + f)).to(cuda).to(dtype)
-    for depth in (None, 10, 1, 0):
+    for wave, depth in ((False, None), (False, 10), (True, None), (True, 10), (None, 1), (None, 0)):
+        h.set_wave_rows(wave)
         h.set_kahan_depth(depth)
         ys = []
         for split in (False, True):
@@ -102,6 +103,7 @@ def test_split_and_fused_forward_are_bit_identical(hub, cuda, mode, dtype, f):
         assert torch.equal(ys[0], y3), (depth, f)
         assert O.max_rel_error(ys[1].cpu().numpy(), O.forward(a, xt.float().cpu().numpy(), mode)) <= 1e-5
     h.set_kahan_depth(32)
+    h.set_wave_rows(None)
 
 
 def test_compensation_from_level_zero_equals_level_one(hub, cuda):
@@ -162,3 +164,32 @@ def test_chunked_child_lists_integer_exact(star, cuda):
     h.matmul_t(torch.from_numpy(x).to(cuda), d2)
     assert np.array_equal(d1.cpu().numpy().astype(np.float64), O.adjoint(a, x, "ax"))
     assert torch.equal(d1, d2)
+
+
+@pytest.mark.parametrize("wave", [False, True])
+@pytest.mark.parametrize("mode", ["ax", "dadx"])
+def test_row_mappings_against_the_oracle(hub, star, cuda, wave, mode):
+    """Both short-row mappings (rows packed per wavefront, a wavefront per row
+    with its entries in sub-group slices) on the hub and star graphs, forward
+    and adjoint, FP32 and BF16, with integer inputs bitwise exact."""
+    for a, n, m in ((hub[0], HUB_N, hub[1][mode]), (star[0], star[1], star[2][mode])):
+        h = m.cusparse_extension
+        h.set_wave_rows(wave)
+        for dtype, f in ((torch.float32, 8), (torch.float32, 64), (torch.bfloat16, 16), (torch.bfloat16, 128)):
+            xt = torch.from_numpy(_x((n, f), 60 + f)).to(cuda).to(dtype)
+            xe = xt.float().cpu().numpy()
+            y = torch.empty(n, f, device=cuda)
+            dx = torch.empty(n, f, device=cuda)
+            h.forward(xt, y)
+            h.matmul_t(xt, dx)
+            assert O.max_rel_error(y.cpu().numpy(), O.forward(a, xe, mode)) <= 1e-5, (wave, dtype, f)
+            assert O.max_rel_error(dx.cpu().numpy(), O.adjoint(a, xe, mode)) <= 1e-5, (wave, dtype, f)
+        if mode == "ax":
+            x = np.random.default_rng(5).integers(-3, 4, size=(n, 16)).astype(np.float32)
+            y = torch.empty(n, 16, device=cuda)
+            dx = torch.empty(n, 16, device=cuda)
+            h.forward(torch.from_numpy(x).to(cuda), y)
+            h.matmul_t(torch.from_numpy(x).to(cuda), dx)
+            assert np.array_equal(y.cpu().numpy().astype(np.float64), O.forward(a, x, "ax"))
+            assert np.array_equal(dx.cpu().numpy().astype(np.float64), O.adjoint(a, x, "ax"))
+        h.set_wave_rows(None)
