@@ -178,6 +178,15 @@ def measure(name, w, ei, m, args, dev, stream):
     h.set_forward_split(False)
     ph["forward_fused_per_level"] = time_loop(lambda: h.forward(x16, y), K, W, stream)
     h.set_forward_split(None)
+    # feature slabs spread over the 8 XCDs (same bits)
+    xcd = {}
+    for slabs in (2, 4, 8):
+        h.set_xcd_slabs(slabs)
+        xcd[slabs] = {"step_bf16": round(time_loop(step(x16, g16), K, W, stream), 4),
+                      "forward": round(time_loop(lambda: h.forward(x16, y), K, W, stream), 4),
+                      "adjoint_delta_t": round(time_loop(lambda: h.adjoint_delta_t(g16, z, dx), K, W, stream), 4)}
+    h.set_xcd_slabs(1)
+    log(f"{name}: XCD slabs {xcd}")
     # results the CPU leg compares against FP64 (fresh, from the BF16 and FP32 steps)
     out = {}
     for tag, (xx, gg) in (("bf16", (x16, g16)), ("fp32", (x32, g32))):
@@ -250,6 +259,7 @@ def measure(name, w, ei, m, args, dev, stream):
         "bytes_step_bf16": b16["step"], "bytes_phase_bf16": b16,
         "dominant_phase": dom, "dominant_gbs": round(dom_gbs, 1),
         "max_rel_error_bf16_vs_fp32_cbm": vs_fp32,
+        "ms_xcd_slabs_bf16": xcd,
     }
     return rec, out, (x32, g32, x16, g16), m_csr
 

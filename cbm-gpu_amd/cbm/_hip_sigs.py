@@ -93,6 +93,7 @@ def declare(lib, sig):
     sig(lib, "cbm_set_kahan_depth", c_i32, c_vp, c_i32)
     sig(lib, "cbm_set_forward_split", c_i32, c_vp, c_i32)
     sig(lib, "cbm_set_wave_rows", c_i32, c_vp, c_i32)
+    sig(lib, "cbm_set_xcd_slabs", c_i32, c_vp, c_i32)
     sig(lib, "csr_last_error", ctypes.c_char_p)
     sig(lib, "csr_create", c_i32, c_i32, c_i32, c_i64, c_vp, c_vp, c_vp, P_vp)
     sig(lib, "csr_destroy", c_i32, c_vp)
@@ -109,5 +110,5 @@ def declare(lib, sig):
 EXPORTED = ("cbm_last_error", "cbm_desc_size", "cbm_create", "cbm_destroy", "cbm_reserve", "cbm_matmul", "cbm_matmul_t", "cbm_adjoint_reverse",
             "cbm_adjoint_delta_t", "cbm_delta_spmm",
             "cbm_propagate", "cbm_last_launches", "cbm_set_slab_bytes", "cbm_set_kahan_depth",
-            "cbm_set_forward_split", "cbm_set_wave_rows", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
+            "cbm_set_forward_split", "cbm_set_wave_rows", "cbm_set_xcd_slabs", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
             "csr_last_error", "cbm_op_spmm", "cbm_op_update", "cbm_op_axpy", "cbm_op_last_error")

@@ -301,6 +301,10 @@ class cusparseCBM:
         m = -1 if mode is None else int(bool(mode))
         _lib.check_hip(self._lib.cbm_set_wave_rows(self._h, m), "set_wave_rows")
 
+    def set_xcd_slabs(self, slabs):
+        """Feature slabs spread over the XCDs (1 = off); same results."""
+        _lib.check_hip(self._lib.cbm_set_xcd_slabs(self._h, int(slabs)), "set_xcd_slabs")
+
     def set_slab_bytes(self, nbytes):
         """Column-slab size for cache residency of the gathers (0: whole rows)."""
         _lib.check_hip(self._lib.cbm_set_slab_bytes(self._h, int(nbytes)), "set_slab_bytes")

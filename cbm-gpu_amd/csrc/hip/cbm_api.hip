@@ -24,6 +24,7 @@ struct cbm_handle {
     int32_t kahan_level = 32; // levels from here on propagate with compensation
     int32_t fwd_split = -1;   // forward form: 0 fused per level, 1 split, -1 automatic
     int32_t wave_rows = -1;   // short-row passes: whole wavefront per row (1), never (0), by size (-1)
+    int32_t xcd_slabs = 1;    // feature slabs spread over the XCDs (1 = off)
     float *comp = nullptr;    // compensation rows of those levels
     int64_t comp_floats = 0;
     int last_launches = 0;
@@ -282,6 +283,7 @@ cbmk::RowPass forward_pass(const cbm_handle *h, int32_t dtype, const void *x, in
     rp.ncols = n_cols;
     rp.rows_hint = h->d.n_rows;
     rp.wave_rows = h->wave_rows;
+    rp.xcd_slabs = h->xcd_slabs;
     return rp;
 }
 
@@ -465,6 +467,7 @@ int adjoint_reverse(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, fl
     rv.ncols = n_cols;
     rv.rows_hint = h->d.n_rows;
     rv.wave_rows = h->wave_rows;
+    rv.xcd_slabs = h->xcd_slabs;
     cbmk::ChunkArgs ck;
     ck.pos = h->d.rck_pos;
     ck.meta = h->d.rck_meta;
@@ -509,6 +512,7 @@ int adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, co
     tp.ncols = n_cols;
     tp.rows_hint = h->d.n_rows;
     tp.wave_rows = h->wave_rows;
+    tp.xcd_slabs = h->xcd_slabs;
     tp.rowscale = h->d.mode == CBM_MODE_AX ? nullptr : h->d.d;
     cbmk::ChunkArgs ck;
     ck.pos = h->d.bck_pos;
@@ -611,6 +615,15 @@ extern "C" int cbm_set_wave_rows(cbm_handle *h, int32_t mode) {
         return CBM_ERR_ARG;
     }
     h->wave_rows = mode;
+    return CBM_OK;
+}
+
+extern "C" int cbm_set_xcd_slabs(cbm_handle *h, int32_t slabs) {
+    if (!h || slabs < 1 || slabs > 64) {
+        set_error("cbm_set_xcd_slabs: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    h->xcd_slabs = slabs;
     return CBM_OK;
 }
 

@@ -17,6 +17,8 @@
 // +-1 entry is added exactly as an add or a subtract.
 #include "cbm_kernels.hpp"
 
+#include <algorithm>
+
 namespace cbmk {
 namespace {
 
@@ -366,9 +368,10 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
     const int lane = int(threadIdx.x & 63u);
     const int l = lane & (LPR - 1);
     const int sub = (lane / LPR) & (S - 1);
-    const int64_t group =
-        (int64_t(blockIdx.x) * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kRowsPerWave +
-        lane / kLanesPerRow;
+    const int32_t slab = int32_t(blockIdx.x % uint32_t(a.xcd_slabs));
+    const int64_t blk = int64_t(blockIdx.x / uint32_t(a.xcd_slabs));
+    const int64_t group = (blk * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kRowsPerWave +
+                          lane / kLanesPerRow;
     const int64_t p64 = int64_t(a.p_begin) + group;
     if (p64 >= int64_t(a.p_end)) return;  // a row's lanes leave together
     const int32_t p = int32_t(p64);
@@ -377,11 +380,12 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
     const int32_t row = a.node_of ? a.node_of[p] : p;
     const int64_t len = int64_t(ke) - kb;
     const int32_t sb = kb + int32_t(len * sub / S), se = kb + int32_t(len * (sub + 1) / S);
+    const int32_t cb = slab * a.slab_w, ce = min(cb + a.slab_w, a.ncols);
     constexpr int32_t kSpan = LPR * VE;
-    const int32_t passes = (a.ncols + kSpan - 1) / kSpan;
+    const int32_t passes = (ce - cb + kSpan - 1) / kSpan;
     for (int32_t pass = 0; pass < passes; ++pass) {
-        const int32_t c0 = pass * kSpan + l * VE;
-        const bool active = c0 < a.ncols;
+        const int32_t c0 = cb + pass * kSpan + l * VE;
+        const bool active = c0 < ce;
         float acc[VE];
 #pragma unroll
         for (int q = 0; q < VE; ++q) acc[q] = 0.0f;
@@ -404,7 +408,8 @@ template <class P, int VE, int LPR, int BS>
 __global__ __launch_bounds__(BS) void k_row_chunk(const RowPass a, const ChunkArgs c) {
     constexpr int kGroups = BS / LPR;
     __shared__ float part[BS * VE];
-    const int32_t chunk = c.first + int32_t(blockIdx.x);
+    const int32_t slab = int32_t(blockIdx.x % uint32_t(a.xcd_slabs));
+    const int32_t chunk = c.first + int32_t(blockIdx.x / uint32_t(a.xcd_slabs));
     const int32_t p = c.pos[chunk];
     const uint32_t meta = uint32_t(c.meta[chunk]);
     const int64_t idx = meta & 0xFFFFu, cnt = meta >> 16;
@@ -416,11 +421,12 @@ __global__ __launch_bounds__(BS) void k_row_chunk(const RowPass a, const ChunkAr
     const int32_t ge = int32_t(ck0 + (ck1 - ck0) * (g + 1) / kGroups);
     const int32_t row = a.node_of ? a.node_of[p] : p;
     const int32_t slot = c.slot[chunk];
+    const int32_t cb = slab * a.slab_w, ce = min(cb + a.slab_w, a.ncols);
     constexpr int32_t kSpan = LPR * VE;
-    const int32_t passes = (a.ncols + kSpan - 1) / kSpan;
+    const int32_t passes = (ce - cb + kSpan - 1) / kSpan;
     for (int32_t pass = 0; pass < passes; ++pass) {
-        const int32_t c0 = pass * kSpan + l * VE;
-        const bool active = c0 < a.ncols;
+        const int32_t c0 = cb + pass * kSpan + l * VE;
+        const bool active = c0 < ce;
         float acc[VE];
 #pragma unroll
         for (int q = 0; q < VE; ++q) acc[q] = 0.0f;
@@ -595,7 +601,8 @@ struct Launch {
 // Whether a short-row pass gives each row a whole wavefront (see RowPass).
 template <int LPR>
 bool wave_per_row(const RowPass &rp) {
-    if (LPR >= 64 || rp.wave_rows == 0) return false;
+    // XCD slabs are for matrices that fill the chip; with them, rows stay packed.
+    if (LPR >= 64 || rp.wave_rows == 0 || rp.xcd_slabs > 1) return false;
     if (rp.wave_rows > 0) return true;
     return rp.rows_hint > 0 && int64_t(rp.rows_hint) * LPR < 64 * kWaveSlots;
 }
@@ -606,18 +613,20 @@ hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
     if (lk.kind == Kind::kPass) {
         const int64_t n = int64_t(rp.p_end) - rp.p_begin;
         if (n <= 0) return hipSuccess;
+        const int64_t ns = rp.xcd_slabs;
         if constexpr (LPR < 64) {
             if (wave_per_row<LPR>(rp)) {
-                k_row_pass<P, VE, LPR, 64 / LPR><<<dim3(unsigned((n + 3) / 4)), dim3(256), 0, stream>>>(rp);
+                k_row_pass<P, VE, LPR, 64 / LPR><<<dim3(unsigned((n + 3) / 4 * ns)), dim3(256), 0, stream>>>(rp);
                 return hipGetLastError();
             }
         }
-        k_row_pass<P, VE, LPR, 1><<<dim3(unsigned((n + kRowsPerBlock - 1) / kRowsPerBlock)), dim3(256), 0, stream>>>(
-            rp);
+        k_row_pass<P, VE, LPR, 1>
+            <<<dim3(unsigned((n + kRowsPerBlock - 1) / kRowsPerBlock * ns)), dim3(256), 0, stream>>>(rp);
     } else if (lk.kind == Kind::kChunk) {
         if (lk.chunk->count <= 0) return hipSuccess;
         constexpr int kBs = (16 * LPR > 256) ? 16 * LPR : 256;
-        k_row_chunk<P, VE, LPR, kBs><<<dim3(unsigned(lk.chunk->count)), dim3(kBs), 0, stream>>>(rp, *lk.chunk);
+        k_row_chunk<P, VE, LPR, kBs>
+            <<<dim3(unsigned(int64_t(lk.chunk->count) * rp.xcd_slabs)), dim3(kBs), 0, stream>>>(rp, *lk.chunk);
     } else {
         const int64_t n = lk.reduce->count;
         if (n <= 0) return hipSuccess;
@@ -655,8 +664,17 @@ hipError_t dispatch_ve(const RowPass &rp, const Launch &lk, hipStream_t stream) 
     if (P::kExtra != kExtraNone) vec = vec && aligned16(rp.e, rp.lde, int(sizeof(typename P::TE)));
     if (P::kDual) vec = vec && aligned16(rp.g2, rp.ldg2, int(sizeof(typename P::TG2)));
     if (P::kExtra == kExtraParentKahan) vec = vec && aligned16(rp.comp, rp.ldc, 4);
-    if (vec) return dispatch_lpr<P, kVe>(rp, lk, lanes_per_row(rp.ncols, kVe), stream);
-    return dispatch_lpr<P, 1>(rp, lk, lanes_per_row(rp.ncols, 1), stream);
+    const int ve = vec ? kVe : 1;
+    // Slab width: a multiple of the vector width; fewer slabs when the
+    // columns run out (every slab holds at least one column).
+    RowPass r = rp;
+    const int32_t want = std::max<int32_t>(1, rp.xcd_slabs);
+    int32_t w = (rp.ncols + want - 1) / want;
+    w = (w + ve - 1) / ve * ve;
+    r.xcd_slabs = (rp.ncols + w - 1) / w;
+    r.slab_w = w;
+    if (vec) return dispatch_lpr<P, kVe>(r, lk, lanes_per_row(w, kVe), stream);
+    return dispatch_lpr<P, 1>(r, lk, lanes_per_row(w, 1), stream);
 }
 
 // The passes the library runs (anything else is rejected):

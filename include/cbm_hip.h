@@ -238,6 +238,16 @@ int cbm_set_forward_split(cbm_handle *h, int32_t mode);
  */
 int cbm_set_wave_rows(cbm_handle *h, int32_t mode);
 
+/*
+ * XCD feature slabs (1..64, default 1 = off): the columns of the gathered
+ * matrix are cut into `slabs` slabs and workgroup b of a gather pass works
+ * on slab b % slabs.  MI355X dispatches workgroups round-robin over its 8
+ * XCDs, so with 8 slabs each XCD gathers one eighth of every row and its
+ * 4 MB L2 covers 8x more of the rows it re-reads.  Every row is still summed
+ * in one place, in the same order: results are identical to slabs = 1.
+ */
+int cbm_set_xcd_slabs(cbm_handle *h, int32_t slabs);
+
 /* ------------------------------------------------------------------------- */
 /* CSR comparator on the same GPU: rocSPARSE SpMM, FP32.                      */
 /* Replaces class cusparseCSR (cbm/csr.cu:24-126).                            */

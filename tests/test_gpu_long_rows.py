@@ -193,3 +193,35 @@ def test_row_mappings_against_the_oracle(hub, star, cuda, wave, mode):
             assert np.array_equal(y.cpu().numpy().astype(np.float64), O.forward(a, x, "ax"))
             assert np.array_equal(dx.cpu().numpy().astype(np.float64), O.adjoint(a, x, "ax"))
         h.set_wave_rows(None)
+
+
+@pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
+@pytest.mark.parametrize("dtype,f", [(torch.float32, 1), (torch.float32, 8), (torch.float32, 300),
+                                     (torch.bfloat16, 64), (torch.bfloat16, 256)])
+def test_xcd_slabs_give_the_same_bits(hub, cuda, mode, dtype, f):
+    """Feature slabs spread over the XCDs change which workgroup computes which
+    columns, not the order in which a row's entries are added: forward (both
+    forms) and adjoint are bit-identical to the unslabbed passes."""
+    a, mats = hub
+    h = mats[mode].cusparse_extension
+    h.set_wave_rows(False)
+    xt = torch.from_numpy(_x((HUB_N, f), 80 + f)).to(cuda).to(dtype)
+    res = {}
+    for slabs in (1, 8, 3):
+        h.set_xcd_slabs(slabs)
+        for split in (False, True):
+            h.set_forward_split(split)
+            y = torch.full((HUB_N, f), float("nan"), device=cuda)
+            h.forward(xt, y)
+            res[(slabs, split)] = y
+        dx = torch.full((HUB_N, f), float("nan"), device=cuda)
+        h.matmul_t(xt, dx)
+        res[(slabs, "t")] = dx
+    h.set_xcd_slabs(1)
+    h.set_forward_split(None)
+    h.set_wave_rows(None)
+    for k, v in res.items():
+        assert torch.equal(v, res[(1, k[1])]), k
+    xe = xt.float().cpu().numpy()
+    assert O.max_rel_error(res[(8, True)].cpu().numpy(), O.forward(a, xe, mode)) <= 1e-5
+    assert O.max_rel_error(res[(8, "t")].cpu().numpy(), O.adjoint(a, xe, mode)) <= 1e-5
