@@ -665,10 +665,14 @@ hipError_t dispatch_ve(const RowPass &rp, const Launch &lk, hipStream_t stream) 
     if (P::kDual) vec = vec && aligned16(rp.g2, rp.ldg2, int(sizeof(typename P::TG2)));
     if (P::kExtra == kExtraParentKahan) vec = vec && aligned16(rp.comp, rp.ldc, 4);
     const int ve = vec ? kVe : 1;
-    // Slab width: a multiple of the vector width; fewer slabs when the
-    // columns run out (every slab holds at least one column).
+    // Slab width: a multiple of the vector width and at least 16 vectors, so
+    // that every slab keeps 16 or more lanes per row.  Rows that wide (and
+    // the unslabbed rows, which are wider) give the chunk kernel 16 groups
+    // either way, so its entry slices, and with them the bits, do not
+    // change; narrower gathers than 16 vectors also measured slower.
     RowPass r = rp;
-    const int32_t want = std::max<int32_t>(1, rp.xcd_slabs);
+    const int32_t want = std::min<int32_t>(std::max<int32_t>(1, rp.xcd_slabs),
+                                           std::max<int32_t>(1, rp.ncols / (16 * ve)));
     int32_t w = (rp.ncols + want - 1) / want;
     w = (w + ve - 1) / ve * ve;
     r.xcd_slabs = (rp.ncols + w - 1) / w;

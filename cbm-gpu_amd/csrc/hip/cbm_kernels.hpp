@@ -69,11 +69,11 @@ struct RowPass {
     int32_t rows_hint = 0;
     int32_t wave_rows = -1;
     // XCD feature slabs: the columns are cut into xcd_slabs slabs of slab_w
-    // (set by the launcher) and workgroup b works on slab b % xcd_slabs.
-    // Workgroups are dispatched round-robin over the XCDs, so with 8 slabs
-    // each XCD gathers only its own slab of the gathered matrix, an eighth of
-    // it, and its L2 holds more of what it re-reads.  Every row is still
-    // summed in one place: results are the same bits as without slabs.
+    // (set by the launcher, at least 16 vectors wide) and workgroup b works
+    // on slab b % xcd_slabs.  Workgroups are dispatched round-robin over the
+    // XCDs, so each XCD gathers only its own slab of the gathered matrix and
+    // its L2 holds more of what it re-reads.  Every row is still summed in
+    // one place, in the same order: the same bits as without slabs.
     int32_t xcd_slabs = 1;
     int32_t slab_w = 0;
 };

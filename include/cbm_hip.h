@@ -240,11 +240,14 @@ int cbm_set_wave_rows(cbm_handle *h, int32_t mode);
 
 /*
  * XCD feature slabs (1..64, default 1 = off): the columns of the gathered
- * matrix are cut into `slabs` slabs and workgroup b of a gather pass works
- * on slab b % slabs.  MI355X dispatches workgroups round-robin over its 8
- * XCDs, so with 8 slabs each XCD gathers one eighth of every row and its
- * 4 MB L2 covers 8x more of the rows it re-reads.  Every row is still summed
- * in one place, in the same order: results are identical to slabs = 1.
+ * matrix are cut into up to `slabs` slabs and workgroup b of a gather pass
+ * works on slab b % slabs.  MI355X dispatches workgroups round-robin over
+ * its 8 XCDs, so with 2 slabs each XCD gathers half of every row and its
+ * 4 MB L2 covers twice as many of the rows it re-reads.  A slab is never
+ * narrower than 16 lanes of 16-byte vectors (256 bytes of the narrowest
+ * gathered row), so fewer slabs than asked are used on narrow rows (none
+ * below 512 bytes).  Every row is still summed in one place, in the same
+ * order: results are identical to slabs = 1 under the packed row mapping.
  */
 int cbm_set_xcd_slabs(cbm_handle *h, int32_t slabs);
 

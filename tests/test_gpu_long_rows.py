@@ -197,11 +197,14 @@ def test_row_mappings_against_the_oracle(hub, star, cuda, wave, mode):
 
 @pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
 @pytest.mark.parametrize("dtype,f", [(torch.float32, 1), (torch.float32, 8), (torch.float32, 300),
-                                     (torch.bfloat16, 64), (torch.bfloat16, 256)])
+                                     (torch.float32, 512), (torch.bfloat16, 64), (torch.bfloat16, 256),
+                                     (torch.bfloat16, 520)])
 def test_xcd_slabs_give_the_same_bits(hub, cuda, mode, dtype, f):
     """Feature slabs spread over the XCDs change which workgroup computes which
     columns, not the order in which a row's entries are added: forward (both
-    forms) and adjoint are bit-identical to the unslabbed passes."""
+    forms) and adjoint are bit-identical to the unslabbed passes.  Narrow rows
+    get fewer slabs than asked (F = 1, 8, 64 none; 256 BF16 two; 300 FP32,
+    520 BF16 four; 512 FP32 eight) and long rows take the chunk kernel."""
     a, mats = hub
     h = mats[mode].cusparse_extension
     h.set_wave_rows(False)
