@@ -13,6 +13,9 @@ D^-1/2 A D^-1/2 (cbm4dad: c2, c3).  Reported on the JSON line:
   roofline         the dominant phase of the step against 8 TB/s HBM
   cpu_baseline     torch.sparse CSR (MKL) FP32 M@X + M^T@G on the host cores
   secondary        the compressible workload (c4) with the same fields
+  build_stages     the structure build by stage: adjacency CSR, candidate
+                   scan on the GPU (HIP kernels) vs on the host cores,
+                   arborescence + Delta
 
     python bench.py [--steps K] [--warmup W] [--workload c5] [--alpha 8] [--cols 128]
 """
@@ -115,10 +118,47 @@ def build_workload(name, alpha, cols, dev):
     log(f"graph {name}: N={w['n']} nnz={ei.shape[1]} ({time.perf_counter() - t0:.1f}s)")
     t0 = time.perf_counter()
     m = _class(CBM_CLASSES, w["product"])(ei, torch.ones(ei.shape[1]), alpha=w["alpha"], device=dev)
+    w["build_s"] = round(time.perf_counter() - t0, 2)
     h = m.cusparse_extension
     log(f"CBM built ({CBM_CLASSES[w['product']][1]}): nnz(Delta)={h.nnz} tree edges={h.n_edges} depth={h.depth} "
-        f"({time.perf_counter() - t0:.1f}s)")
+        f"({w['build_s']:.1f}s)")
     return w, ei, m
+
+
+def build_stages(ei, n, alpha, dev, stream):
+    """Times the stages of the structure build: the adjacency CSR (host), the
+    candidate-edge scan on the GPU (HIP events around count + emit, inputs
+    already on the device) and on the host cores, and the arborescence plus
+    Delta (host).  Both scans must give the same list."""
+    import torch
+    from cbm import structure
+    cores = len(os.sched_getaffinity(0))
+    out = {"host_cores": cores}
+    t0 = time.perf_counter()
+    adj = structure.Adjacency(ei[0], ei[1], torch.ones(ei.shape[1]), n)
+    out["adjacency_s"] = round(time.perf_counter() - t0, 3)
+    structure.gpu_candidates(adj, alpha, dev)  # warm-up (first launch, allocator)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    gsrc, gdst, gw = structure.gpu_candidates(adj, alpha, dev)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    out["candidates_gpu_ms"] = round(start.elapsed_time(stop), 3)
+    out["candidates_gpu_wall_ms"] = round(1e3 * (time.perf_counter() - t0), 3)
+    t0 = time.perf_counter()
+    hsrc, hdst, hw = structure.host_candidates(adj, alpha)
+    out["candidates_host_s"] = round(time.perf_counter() - t0, 3)
+    out["n_candidates"] = int(hsrc.numel())
+    out["scans_identical"] = bool(torch.equal(gsrc.cpu(), hsrc) and torch.equal(gdst.cpu(), hdst)
+                                  and torch.equal(gw.cpu(), hw))
+    t0 = time.perf_counter()
+    structure.build_from_candidates(adj, hsrc, hdst, hw)
+    out["arborescence_delta_s"] = round(time.perf_counter() - t0, 3)
+    adj.close()
+    log(f"build stages: {out}")
+    return out
 
 
 def measure(name, w, ei, m, args, dev, stream):
@@ -421,6 +461,8 @@ def main():
         profile_steps(w, m, args, dev, stream)
         return
     rec, gpu_out, inputs, m_csr = measure(args.workload, w, ei, m, args, dev, stream)
+    rec["build_s"] = w["build_s"]
+    rec["build_stages"] = build_stages(ei, w["n"], w["alpha"], dev, stream)
     cpu = None
     if not args.no_cpu:
         t0 = time.perf_counter()
@@ -433,6 +475,8 @@ def main():
     if args.secondary and args.secondary != args.workload:
         w2, ei2, m2 = build_workload(args.secondary, None, None, dev)
         secondary, out2, in2, m_csr2 = measure(args.secondary, w2, ei2, m2, args, dev, stream)
+        secondary["build_s"] = w2["build_s"]
+        secondary["build_stages"] = build_stages(ei2, w2["n"], w2["alpha"], dev, stream)
         if not args.no_cpu:
             secondary["max_rel_error_vs_fp64"] = fp64_errors(m_csr2, out2, in2)
         del m2, out2, in2, m_csr2

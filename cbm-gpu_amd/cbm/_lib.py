@@ -65,6 +65,17 @@ def host():
             _sig(lib, "cbm_build_array", c_vp, c_vp, c_i32)
             _sig(lib, "cbm_build_stat", c_i64, c_vp, c_i32)
             _sig(lib, "cbm_build_free", None, c_vp)
+            _sig(lib, "cbm_adjacency_create", c_i32, c_vp, c_vp, c_vp, c_i64, c_i32, P_vp)
+            _sig(lib, "cbm_adjacency_rows", c_i32, c_vp)
+            _sig(lib, "cbm_adjacency_nnz", c_i64, c_vp)
+            _sig(lib, "cbm_adjacency_ptr", c_vp, c_vp)
+            _sig(lib, "cbm_adjacency_col", c_vp, c_vp)
+            _sig(lib, "cbm_adjacency_free", None, c_vp)
+            _sig(lib, "cbm_candidates_create", c_i32, c_vp, c_i32, c_i32, P_vp)
+            _sig(lib, "cbm_candidates_size", c_i64, c_vp)
+            _sig(lib, "cbm_candidates_array", c_vp, c_vp, c_i32)
+            _sig(lib, "cbm_candidates_free", None, c_vp)
+            _sig(lib, "cbm_build_from_candidates", c_i32, c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, P_vp)
             _sig(lib, "cbm_plan_create", c_i32, c_i32, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_i32, P_vp)
             _sig(lib, "cbm_plan_size", c_i64, c_vp, c_i32)
             _sig(lib, "cbm_plan_array", c_vp, c_vp, c_i32)
@@ -103,6 +114,13 @@ def check_hip(rc, what=""):
     if rc == CBM_OK:
         return
     msg = hip().cbm_last_error().decode(errors="replace")
+    _raise(rc, msg, what)
+
+
+def check_candidates(rc, what=""):
+    if rc == CBM_OK:
+        return
+    msg = hip().cbm_candidates_last_error().decode(errors="replace")
     _raise(rc, msg, what)
 
 

@@ -22,12 +22,19 @@ class cbm4mm:
     ``num_nodes``, ``mca_offset``, ``mca_src_idx``, ``mca_dst_idx`` and
     ``cusparse_extension``.  ``matmul(x, y)`` writes ``y = A x`` in place.
     Additions: ``x @``-style autograd product via :meth:`apply`, BF16 ``x``,
-    ``num_nodes=`` and the host build statistics in ``build_stats``.
+    ``num_nodes=``, the candidate-edge scan on the GPU (``candidates_on``) and
+    the build statistics in ``build_stats``.
     """
 
-    def __init__(self, edge_index, edge_values, alpha=0, *, num_nodes=None, device="cuda", n_threads=0):
+    def __init__(self, edge_index, edge_values, alpha=0, *, num_nodes=None, device="cuda", n_threads=0,
+                 candidates_on="device"):
         num_rows = _num_rows(edge_index, num_nodes)
-        built = cbm_mkl.build(edge_index[0], edge_index[1], edge_values, num_rows, alpha, n_threads=n_threads)
+        # The candidate-edge scan of the build runs on the target GPU
+        # ("device"), or on the host (None); the structure is the same.
+        if candidates_on == "device":
+            candidates_on = device if torch.device(device).type == "cuda" else None
+        built = cbm_mkl.build(edge_index[0], edge_index[1], edge_values, num_rows, alpha, n_threads=n_threads,
+                              candidates_on=candidates_on)
         self.build_stats = built.stats
         # Delta as CSR sorted by (row, col): what coalesce().to_sparse_csr()
         # gives in the reference (cbm/cbm4mm.py:26-42), built directly.

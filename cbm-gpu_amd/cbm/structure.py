@@ -73,28 +73,144 @@ class Build:
         return [self.delta_row, self.delta_col, self.delta_val, self.branches, self.tree_src, self.tree_dst]
 
 
-def build(row, col, val, n_rows, alpha, n_threads=0):
-    """Builds the CBM structure of the binary matrix given as COO."""
+def _read_build(lib, b):
+    """Copies a cbm_build result into a :class:`Build` (CPU tensors)."""
+    def arr(which, dtype):
+        return _copy_out(lib.cbm_build_array(b, which), lib.cbm_build_size(b, which), dtype)
+    out = Build(arr(_B_DELTA_ROW, torch.int32), arr(_B_DELTA_COL, torch.int32), arr(_B_DELTA_VAL, torch.float32),
+                arr(_B_BRANCHES, torch.int32), arr(_B_TREE_SRC, torch.int32), arr(_B_TREE_DST, torch.int32),
+                arr(_B_PARENT, torch.int32), arr(_B_CSR_PTR, torch.int32), arr(_B_CSR_COL, torch.int32),
+                arr(_B_CSR_VAL, torch.float32))
+    out.stats = {name: int(lib.cbm_build_stat(b, i)) for i, name in enumerate(_STAT_NAMES)}
+    return out
+
+
+def _coo(row, col, val):
     row = _cpu_contig(row, torch.int32, "src_row_idx")
     col = _cpu_contig(col, torch.int32, "src_col_idx")
     val = _cpu_contig(val, torch.float32, "src_values")
     nnz = row.numel()
     if col.numel() != nnz or val.numel() != nnz:
         raise ValueError(f"row/col/val lengths differ: {row.numel()}, {col.numel()}, {val.numel()}")
+    return row, col, val
+
+
+class Adjacency:
+    """Binary CSR of A on the host (``cbm_adjacency_create``): rows sorted,
+    duplicates merged, values all ones.  ``ptr`` int64 ``[n+1]``, ``col``
+    int32 ``[nnz]`` as CPU tensors; the native object is released by
+    :meth:`close` (or ``with``)."""
+
+    def __init__(self, row, col, val, n_rows):
+        row, col, val = _coo(row, col, val)
+        self._lib = _lib.host()
+        self._p = ctypes.c_void_p()
+        rc = self._lib.cbm_adjacency_create(row.data_ptr(), col.data_ptr(), val.data_ptr(), row.numel(),
+                                            int(n_rows), ctypes.byref(self._p))
+        _lib.check_host(rc, "adjacency")
+        self.n = int(self._lib.cbm_adjacency_rows(self._p))
+        self.nnz = int(self._lib.cbm_adjacency_nnz(self._p))
+        self.ptr = _copy_out(self._lib.cbm_adjacency_ptr(self._p), self.n + 1, torch.int64)
+        self.col = _copy_out(self._lib.cbm_adjacency_col(self._p), self.nnz, torch.int32)
+
+    def close(self):
+        if self._p:
+            self._lib.cbm_adjacency_free(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+
+def host_candidates(adj, alpha, n_threads=0):
+    """Candidate edges of the host scan (``cbm_candidates_create``) in the
+    reference creation order: ``(src, dst, w)`` int32 CPU tensors."""
+    lib = _lib.host()
+    c = ctypes.c_void_p()
+    _lib.check_host(lib.cbm_candidates_create(adj._p, int(alpha), int(n_threads), ctypes.byref(c)), "candidates")
+    try:
+        m = lib.cbm_candidates_size(c)
+        return tuple(_copy_out(lib.cbm_candidates_array(c, k), m, torch.int32) for k in range(3))
+    finally:
+        lib.cbm_candidates_free(c)
+
+
+def gpu_candidates(adj, alpha, device, n_workgroups=0):
+    """The same candidate edges from the GPU scan (``cbm_candidates_count`` /
+    ``_emit``, HIP kernels) on ``device``; returned as ``(src, dst, w)`` int32
+    tensors on that device."""
+    dev = torch.device(device)
+    if dev.type != "cuda":
+        raise RuntimeError("tensor is not a CUDA tensor")
+    hip = _lib.hip()
+    n = adj.n
+    ws_bytes = int(hip.cbm_candidates_workspace_bytes(n, int(n_workgroups)))
+    if ws_bytes < 0:
+        raise ValueError(f"no candidate workspace for {n} rows")
+    with torch.cuda.device(dev):
+        stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        ptr = adj.ptr.to(dev)
+        col = adj.col.to(dev)
+        if col.numel() == 0:
+            col = torch.zeros(1, dtype=torch.int32, device=dev)  # a valid pointer for an empty matrix
+        ws = torch.empty(ws_bytes + 256, dtype=torch.uint8, device=dev)
+        ws_ptr = (ws.data_ptr() + 255) // 256 * 256
+        counts = torch.empty(n, dtype=torch.int64, device=dev)
+        _lib.check_candidates(hip.cbm_candidates_count(n, ptr.data_ptr(), col.data_ptr(), int(alpha),
+                                                       counts.data_ptr(), ws_ptr, ws_bytes, stream),
+                              "candidates (count)")
+        offsets = torch.cumsum(counts, 0) - counts
+        total = int(counts.sum().item())
+        src, dst, w = (torch.empty(max(total, 1), dtype=torch.int32, device=dev) for _ in range(3))
+        _lib.check_candidates(hip.cbm_candidates_emit(n, ptr.data_ptr(), col.data_ptr(), int(alpha),
+                                                      offsets.data_ptr(), src.data_ptr(), dst.data_ptr(),
+                                                      w.data_ptr(), total, ws_ptr, ws_bytes, stream),
+                              "candidates (emit)")
+        del ws
+        return src[:total], dst[:total], w[:total]
+
+
+def build_from_candidates(adj, src, dst, w, n_threads=0):
+    """The structure of ``adj`` from a candidate list (``cbm_build_from_candidates``)."""
+    src, dst, w = (_cpu_contig(t, torch.int32, name) for t, name in ((src, "src"), (dst, "dst"), (w, "w")))
+    if not (src.numel() == dst.numel() == w.numel()):
+        raise ValueError("candidate arrays differ in length")
     lib = _lib.host()
     b = ctypes.c_void_p()
-    rc = lib.cbm_build_create(row.data_ptr(), col.data_ptr(), val.data_ptr(), nnz, int(n_rows), int(alpha),
+    rc = lib.cbm_build_from_candidates(adj._p, src.data_ptr(), dst.data_ptr(), w.data_ptr(), src.numel(),
+                                       int(n_threads), ctypes.byref(b))
+    _lib.check_host(rc, "init from candidates")
+    try:
+        return _read_build(lib, b)
+    finally:
+        lib.cbm_build_free(b)
+
+
+def build(row, col, val, n_rows, alpha, n_threads=0, candidates_on=None):
+    """Builds the CBM structure of the binary matrix given as COO.
+
+    ``candidates_on``: None scans the candidate edges on the host; a CUDA
+    device scans them there with the HIP kernels.  The arborescence and Delta
+    are built on the host either way, and both scans give the same list, so
+    the result is the same."""
+    if candidates_on is not None:
+        with Adjacency(row, col, val, n_rows) as adj:
+            src, dst, w = gpu_candidates(adj, alpha, candidates_on)
+            return build_from_candidates(adj, src, dst, w, n_threads)
+    row, col, val = _coo(row, col, val)
+    lib = _lib.host()
+    b = ctypes.c_void_p()
+    rc = lib.cbm_build_create(row.data_ptr(), col.data_ptr(), val.data_ptr(), row.numel(), int(n_rows), int(alpha),
                               int(n_threads), ctypes.byref(b))
     _lib.check_host(rc, "init")
     try:
-        def arr(which, dtype):
-            return _copy_out(lib.cbm_build_array(b, which), lib.cbm_build_size(b, which), dtype)
-        out = Build(arr(_B_DELTA_ROW, torch.int32), arr(_B_DELTA_COL, torch.int32), arr(_B_DELTA_VAL, torch.float32),
-                    arr(_B_BRANCHES, torch.int32), arr(_B_TREE_SRC, torch.int32), arr(_B_TREE_DST, torch.int32),
-                    arr(_B_PARENT, torch.int32), arr(_B_CSR_PTR, torch.int32), arr(_B_CSR_COL, torch.int32),
-                    arr(_B_CSR_VAL, torch.float32))
-        out.stats = {name: int(lib.cbm_build_stat(b, i)) for i, name in enumerate(_STAT_NAMES)}
-        return out
+        return _read_build(lib, b)
     finally:
         lib.cbm_build_free(b)
 

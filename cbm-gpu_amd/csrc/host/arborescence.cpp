@@ -14,8 +14,7 @@ struct LeftistForest {
     std::vector<int32_t> key, lazy, left, right;
     std::vector<uint8_t> npl;  // null-path length of the right spine; 0 for "no node"
 
-    explicit LeftistForest(const std::vector<int32_t> &w)
-        : key(w), lazy(w.size(), 0), left(w.size(), -1), right(w.size(), -1), npl(w.size(), 1) {}
+    LeftistForest(const int32_t *w, size_t m) : key(w, w + m), lazy(m, 0), left(m, -1), right(m, -1), npl(m, 1) {}
 
     int npl_of(int32_t x) const { return x < 0 ? 0 : npl[size_t(x)]; }
 
@@ -101,11 +100,11 @@ struct Contraction {
 
 }  // namespace
 
-ArborescenceResult min_arborescence(int32_t n, int32_t root, const std::vector<int32_t> &src,
-                                    const std::vector<int32_t> &dst, const std::vector<int32_t> &w) {
+ArborescenceResult min_arborescence(int32_t n, int32_t root, const int32_t *src, const int32_t *dst,
+                                    const int32_t *w, int64_t n_edges) {
     ArborescenceResult res;
-    const size_t m = src.size();
-    LeftistForest heaps(w);
+    const size_t m = size_t(n_edges);
+    LeftistForest heaps(w, m);
     std::vector<int32_t> heap(size_t(n), -1);
     for (size_t e = 0; e < m; ++e) heap[size_t(dst[e])] = heaps.meld(heap[size_t(dst[e])], int32_t(e));
 

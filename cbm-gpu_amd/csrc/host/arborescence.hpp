@@ -27,7 +27,7 @@ struct ArborescenceResult {
 };
 
 // n vertices 0..n-1, root `root`; edge e goes src[e] -> dst[e] with weight w[e] >= 0.
-ArborescenceResult min_arborescence(int32_t n, int32_t root, const std::vector<int32_t> &src,
-                                    const std::vector<int32_t> &dst, const std::vector<int32_t> &w);
+ArborescenceResult min_arborescence(int32_t n, int32_t root, const int32_t *src, const int32_t *dst,
+                                    const int32_t *w, int64_t n_edges);
 
 }  // namespace cbmh

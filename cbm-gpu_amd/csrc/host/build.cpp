@@ -12,7 +12,11 @@
 //   * the arborescence is our own (arborescence.cpp), because arbok is not
 //     available; its tie-break rule is documented there;
 //   * Delta row lengths are counted first, so the rows are written in parallel
-//     straight into their final construction-order slots.
+//     straight into their final construction-order slots;
+//   * the three stages are also exported one by one (adjacency CSR, candidate
+//     edges, structure from a candidate list), so that the candidate scan can
+//     run elsewhere (the GPU kernel of cand_api.hip) and feed the same
+//     arborescence and Delta code.
 #include "arborescence.hpp"
 #include "common.hpp"
 #include "../../../include/cbm_host.h"
@@ -89,6 +93,18 @@ int build_csr(const int32_t *row, const int32_t *col, const float *val, int64_t 
 struct Candidates {
     std::vector<int32_t> src, dst, w;
 };
+
+}  // namespace
+
+struct cbm_adjacency {
+    Csr a;
+};
+
+struct cbm_candidates {
+    Candidates c;
+};
+
+namespace {
 
 void scan_candidates(const Csr &a, int32_t alpha, Candidates &out) {
     const int32_t n = a.n;
@@ -215,52 +231,48 @@ void write_delta_row(const Csr &a, int32_t v, int32_t u, int32_t *col, float *va
     }
 }
 
-}  // namespace
-
-extern "C" int cbm_build_create(const int32_t *row, const int32_t *col, const float *val, int64_t nnz,
-                                int32_t n_rows, int32_t alpha, int32_t n_threads, cbm_build **out) {
-    if (!out || (nnz > 0 && (!row || !col)) || nnz < 0 || n_rows < 1 || n_rows >= INT32_MAX) {
-        set_error("init: invalid arguments (n_rows=%d nnz=%lld)", n_rows, (long long)nnz);
-        return CBM_ERR_ARG;
+// Threads for one call (n_threads <= 0: the OpenMP default), restored on exit.
+struct ThreadScope {
+    int saved;
+    explicit ThreadScope(int32_t n_threads) : saved(omp_get_max_threads()) {
+        if (n_threads > 0) omp_set_num_threads(n_threads);
     }
-    *out = nullptr;
-    const int saved_threads = omp_get_max_threads();
-    if (n_threads > 0) omp_set_num_threads(n_threads);
-    struct Restore {
-        int t;
-        ~Restore() { omp_set_num_threads(t); }
-    } restore{saved_threads};
+    ~ThreadScope() { omp_set_num_threads(saved); }
+};
+
+// The structure of A from its candidate edges (cbm_extensions.cpp:188-299):
+// minimum arborescence rooted at N, parents, branches in BFS order, Delta.
+// `owned`: the library's own candidate list, released once the parents are
+// known; null when the caller supplied the list (then inconsistent weights
+// are the caller's error).
+int build_from(const Csr &a, const int32_t *csrc, const int32_t *cdst, const int32_t *cw, int64_t m,
+               Candidates *owned, cbm_build **out) {
+    const int32_t n = a.n;
     try {
-        Csr a;
-        int rc = build_csr(row, col, val, nnz, n_rows, a);
-        if (rc != CBM_OK) return rc;
-        const int32_t n = n_rows;
-
-        Candidates cand;
-        scan_candidates(a, alpha, cand);
-
-        ArborescenceResult msa = min_arborescence(n + 1, n, cand.src, cand.dst, cand.w);
+        ArborescenceResult msa = min_arborescence(n + 1, n, csrc, cdst, cw, m);
         if (!msa.spanning || !msa.consistent) {
             set_error("init: arborescence construction failed (spanning=%d consistent=%d)", int(msa.spanning),
                       int(msa.consistent));
-            return CBM_ERR_INTERNAL;
+            return owned || msa.spanning ? CBM_ERR_INTERNAL : CBM_ERR_VALUE;
         }
         auto *b = new cbm_build;
         std::unique_ptr<cbm_build> guard(b);
         b->stats[CBM_S_N] = n;
         b->stats[CBM_S_NNZ_A] = int64_t(a.col.size());
-        b->stats[CBM_S_CANDIDATES] = int64_t(cand.src.size());
+        b->stats[CBM_S_CANDIDATES] = m;
         b->stats[CBM_S_MSA_WEIGHT] = msa.weight;
 
         std::vector<int32_t> &parent = b->arr_i32[CBM_B_PARENT];
         parent.resize(size_t(n));
         for (int32_t v = 0; v < n; ++v) {
-            const int32_t s = cand.src[size_t(msa.in_edge[size_t(v)])];
+            const int32_t s = csrc[size_t(msa.in_edge[size_t(v)])];
             parent[size_t(v)] = (s == n) ? -1 : s;
         }
-        std::vector<int32_t>().swap(cand.src);
-        std::vector<int32_t>().swap(cand.dst);
-        std::vector<int32_t>().swap(cand.w);
+        if (owned) {
+            std::vector<int32_t>().swap(owned->src);
+            std::vector<int32_t>().swap(owned->dst);
+            std::vector<int32_t>().swap(owned->w);
+        }
 
         // Children lists by parent, ascending child id; slot n holds the root's children.
         std::vector<int64_t> cptr(size_t(n) + 2, 0);
@@ -328,9 +340,9 @@ extern "C" int cbm_build_create(const int32_t *row, const int32_t *col, const fl
         std::partial_sum(dptr.begin(), dptr.end(), dptr.begin());
         const int64_t nd = dptr[size_t(n)];
         if (nd != msa.weight) {
-            set_error("init: nnz(Delta)=%lld differs from the arborescence weight %lld", (long long)nd,
-                      (long long)msa.weight);
-            return CBM_ERR_INTERNAL;
+            set_error("init: nnz(Delta)=%lld differs from the arborescence weight %lld%s", (long long)nd,
+                      (long long)msa.weight, owned ? "" : " (candidate weights are not the row distances)");
+            return owned ? CBM_ERR_INTERNAL : CBM_ERR_VALUE;
         }
         b->stats[CBM_S_NNZ_DELTA] = nd;
         std::vector<int32_t> &drow = b->arr_i32[CBM_B_DELTA_ROW];
@@ -370,6 +382,106 @@ extern "C" int cbm_build_create(const int32_t *row, const int32_t *col, const fl
         set_error("init: out of host memory");
         return CBM_ERR_NOMEM;
     }
+}
+
+}  // namespace
+
+extern "C" int cbm_build_create(const int32_t *row, const int32_t *col, const float *val, int64_t nnz,
+                                int32_t n_rows, int32_t alpha, int32_t n_threads, cbm_build **out) {
+    if (!out || (nnz > 0 && (!row || !col)) || nnz < 0 || n_rows < 1 || n_rows >= INT32_MAX) {
+        set_error("init: invalid arguments (n_rows=%d nnz=%lld)", n_rows, (long long)nnz);
+        return CBM_ERR_ARG;
+    }
+    *out = nullptr;
+    ThreadScope threads(n_threads);
+    try {
+        Csr a;
+        int rc = build_csr(row, col, val, nnz, n_rows, a);
+        if (rc != CBM_OK) return rc;
+        Candidates cand;
+        scan_candidates(a, alpha, cand);
+        return build_from(a, cand.src.data(), cand.dst.data(), cand.w.data(), int64_t(cand.src.size()), &cand, out);
+    } catch (const std::bad_alloc &) {
+        set_error("init: out of host memory");
+        return CBM_ERR_NOMEM;
+    }
+}
+
+extern "C" int cbm_adjacency_create(const int32_t *row, const int32_t *col, const float *val, int64_t nnz,
+                                    int32_t n_rows, cbm_adjacency **out) {
+    if (!out || (nnz > 0 && (!row || !col)) || nnz < 0 || n_rows < 1 || n_rows >= INT32_MAX) {
+        set_error("adjacency: invalid arguments (n_rows=%d nnz=%lld)", n_rows, (long long)nnz);
+        return CBM_ERR_ARG;
+    }
+    *out = nullptr;
+    try {
+        std::unique_ptr<cbm_adjacency> adj(new cbm_adjacency);
+        const int rc = build_csr(row, col, val, nnz, n_rows, adj->a);
+        if (rc != CBM_OK) return rc;
+        *out = adj.release();
+        return CBM_OK;
+    } catch (const std::bad_alloc &) {
+        set_error("adjacency: out of host memory");
+        return CBM_ERR_NOMEM;
+    }
+}
+
+extern "C" int32_t cbm_adjacency_rows(const cbm_adjacency *adj) { return adj ? adj->a.n : -1; }
+extern "C" int64_t cbm_adjacency_nnz(const cbm_adjacency *adj) { return adj ? int64_t(adj->a.col.size()) : -1; }
+extern "C" const int64_t *cbm_adjacency_ptr(const cbm_adjacency *adj) { return adj ? adj->a.ptr.data() : nullptr; }
+extern "C" const int32_t *cbm_adjacency_col(const cbm_adjacency *adj) { return adj ? adj->a.col.data() : nullptr; }
+extern "C" void cbm_adjacency_free(cbm_adjacency *adj) { delete adj; }
+
+extern "C" int cbm_candidates_create(const cbm_adjacency *adj, int32_t alpha, int32_t n_threads,
+                                     cbm_candidates **out) {
+    if (!adj || !out) {
+        set_error("candidates: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    *out = nullptr;
+    ThreadScope threads(n_threads);
+    try {
+        std::unique_ptr<cbm_candidates> c(new cbm_candidates);
+        scan_candidates(adj->a, alpha, c->c);
+        *out = c.release();
+        return CBM_OK;
+    } catch (const std::bad_alloc &) {
+        set_error("candidates: out of host memory");
+        return CBM_ERR_NOMEM;
+    }
+}
+
+extern "C" int64_t cbm_candidates_size(const cbm_candidates *c) { return c ? int64_t(c->c.src.size()) : -1; }
+
+extern "C" const int32_t *cbm_candidates_array(const cbm_candidates *c, int32_t which) {
+    if (!c) return nullptr;
+    switch (which) {
+        case CBM_C_SRC: return c->c.src.data();
+        case CBM_C_DST: return c->c.dst.data();
+        case CBM_C_WEIGHT: return c->c.w.data();
+        default: return nullptr;
+    }
+}
+
+extern "C" void cbm_candidates_free(cbm_candidates *c) { delete c; }
+
+extern "C" int cbm_build_from_candidates(const cbm_adjacency *adj, const int32_t *src, const int32_t *dst,
+                                         const int32_t *w, int64_t n_cand, int32_t n_threads, cbm_build **out) {
+    if (!adj || !out || n_cand < 0 || (n_cand > 0 && (!src || !dst || !w))) {
+        set_error("init from candidates: invalid arguments (n_cand=%lld)", (long long)n_cand);
+        return CBM_ERR_ARG;
+    }
+    *out = nullptr;
+    const int32_t n = adj->a.n;
+    for (int64_t e = 0; e < n_cand; ++e) {
+        if (src[e] < 0 || src[e] > n || dst[e] < 0 || dst[e] >= n || src[e] == dst[e] || w[e] < 0) {
+            set_error("init from candidates: edge %lld = (%d -> %d, w %d) is not an edge of the %d-row candidate graph",
+                      (long long)e, src[e], dst[e], w[e], n);
+            return CBM_ERR_VALUE;
+        }
+    }
+    ThreadScope threads(n_threads);
+    return build_from(adj->a, src, dst, w, n_cand, nullptr, out);
 }
 
 extern "C" int64_t cbm_build_size(const cbm_build *b, int32_t which) {

@@ -252,6 +252,51 @@ int cbm_set_wave_rows(cbm_handle *h, int32_t mode);
 int cbm_set_xcd_slabs(cbm_handle *h, int32_t slabs);
 
 /* ------------------------------------------------------------------------- */
+/* Candidate edges of the structure build on the GPU.                         */
+/* Replace the MKL syrk of A A^T and the two candidate scans of cbm_init_     */
+/* (cbm/cbm_extensions.cpp:100-182).  The result feeds                        */
+/* cbm_build_from_candidates (cbm_host.h), which runs the arborescence and    */
+/* builds Delta on the host.                                                  */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * Bytes of device workspace for an n_rows matrix run by n_workgroups
+ * persistent workgroups (<= 0: the library's choice, about 4 GiB of counters
+ * at most): 256 bytes plus n_rows uint32 counters per workgroup.  -1 for
+ * n_rows < 1.
+ */
+int64_t cbm_candidates_workspace_bytes(int32_t n_rows, int32_t n_workgroups);
+
+/*
+ * Pass 1.  For the binary CSR (ptr int64 [n_rows + 1], col int32 [nnz],
+ * columns sorted and unique per row, all in [0, n_rows) -- what
+ * cbm_adjacency_create gives), row_count[r] (int64 [n_rows], device) receives
+ * the number of candidate edges created for row r: the root edge N -> r, plus
+ * for every c > r sharing a column with r, c -> r when h < deg(r) - alpha and
+ * h != deg(r), and r -> c when h < deg(c) - alpha and h != deg(c), where
+ * h = |A_r xor A_c|.  workspace: device, 256-byte aligned, workspace_bytes as
+ * given by cbm_candidates_workspace_bytes (the number of workgroups follows
+ * from its size).  Asynchronous on `stream`; allocates nothing.
+ */
+int cbm_candidates_count(int32_t n_rows, const int64_t *ptr, const int32_t *col, int32_t alpha,
+                         int64_t *row_count, void *workspace, int64_t workspace_bytes, hipStream_t stream);
+
+/*
+ * Pass 2.  Writes the edges of row r to src / dst / w (int32, device) from
+ * position row_offset[r] on (the exclusive prefix sum of pass 1's counts), in
+ * the reference creation order: the root edge (src = n_rows, w = deg(r)),
+ * then for c ascending, c -> r before r -> c (w = h).  n_cand is the length of
+ * the three arrays; no write goes past it.  The concatenation over r is the
+ * host scan's list (cbm_candidates_create) entry for entry.
+ */
+int cbm_candidates_emit(int32_t n_rows, const int64_t *ptr, const int32_t *col, int32_t alpha,
+                        const int64_t *row_offset, int32_t *src, int32_t *dst, int32_t *w, int64_t n_cand,
+                        void *workspace, int64_t workspace_bytes, hipStream_t stream);
+
+/* Message describing the last cbm_candidates_* failure on the calling thread. */
+const char *cbm_candidates_last_error(void);
+
+/* ------------------------------------------------------------------------- */
 /* CSR comparator on the same GPU: rocSPARSE SpMM, FP32.                      */
 /* Replaces class cusparseCSR (cbm/csr.cu:24-126).                            */
 /* ------------------------------------------------------------------------- */

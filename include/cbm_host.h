@@ -113,6 +113,49 @@ const void *cbm_build_array(const cbm_build *b, int32_t which);
 int64_t cbm_build_stat(const cbm_build *b, int32_t which);
 void cbm_build_free(cbm_build *b);
 
+/*
+ * The three stages of cbm_build_create, exported one by one so that the
+ * candidate scan can run on the GPU (cbm_candidates_count / _emit in
+ * cbm_hip.h) between the host stages.  No reference counterpart as separate
+ * calls: together they are cbm_init_ (cbm/cbm_extensions.cpp:60-345).
+ *
+ * Stage 1, the binary CSR of A (MKL COO->CSR in the reference, :66-99): rows
+ * sorted, duplicate (row, col) pairs merged, values must be all ones.
+ * ptr is int64 [n_rows + 1], col int32 [nnz].
+ */
+typedef struct cbm_adjacency cbm_adjacency;
+int cbm_adjacency_create(const int32_t *row, const int32_t *col, const float *val, int64_t nnz,
+                         int32_t n_rows, cbm_adjacency **out);
+int32_t cbm_adjacency_rows(const cbm_adjacency *adj);
+int64_t cbm_adjacency_nnz(const cbm_adjacency *adj);
+const int64_t *cbm_adjacency_ptr(const cbm_adjacency *adj);
+const int32_t *cbm_adjacency_col(const cbm_adjacency *adj);
+void cbm_adjacency_free(cbm_adjacency *adj);
+
+/*
+ * Stage 2, the candidate edges in the reference creation order
+ * (cbm/cbm_extensions.cpp:122-182, the rule given at cbm_build_create):
+ * arrays src, dst, weight of cbm_candidates_size() entries.
+ */
+enum { CBM_C_SRC = 0, CBM_C_DST = 1, CBM_C_WEIGHT = 2 };
+typedef struct cbm_candidates cbm_candidates;
+int cbm_candidates_create(const cbm_adjacency *adj, int32_t alpha, int32_t n_threads, cbm_candidates **out);
+int64_t cbm_candidates_size(const cbm_candidates *c);
+const int32_t *cbm_candidates_array(const cbm_candidates *c, int32_t which);
+void cbm_candidates_free(cbm_candidates *c);
+
+/*
+ * Stage 3, the structure from a candidate list (arborescence, parents,
+ * branches, Delta; cbm/cbm_extensions.cpp:188-299).  The list must be the
+ * candidate list of `adj` in creation order (as stage 2 or the GPU scan gives
+ * it) for the result to equal cbm_build_create's.  An edge outside the
+ * candidate graph, a list that does not span the rows, or a chosen tree whose
+ * weight is not nnz(Delta) (weights that are not the row distances) fails
+ * with CBM_ERR_VALUE.
+ */
+int cbm_build_from_candidates(const cbm_adjacency *adj, const int32_t *src, const int32_t *dst, const int32_t *w,
+                              int64_t n_cand, int32_t n_threads, cbm_build **out);
+
 /* ------------------------------------------------------------------------- */
 /* Device plan: the layout the HIP kernels read, built on the host.           */
 /* ------------------------------------------------------------------------- */
