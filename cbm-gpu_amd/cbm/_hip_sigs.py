@@ -3,9 +3,10 @@ import ctypes
 
 c_i32, c_i64, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
 
-# enum cbm_mode / cbm_dtype
+# enum cbm_mode / cbm_dtype / candidate paths
 MODE_AX, MODE_ADX, MODE_DADX = 0, 1, 2
 F32, BF16 = 0, 1
+CAND_AUTO, CAND_TILES, CAND_LISTS = 0, 1, 2
 
 
 class CbmDesc(ctypes.Structure):
@@ -105,10 +106,10 @@ def declare(lib, sig):
     sig(lib, "cbm_op_update", c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_vp)
     sig(lib, "cbm_op_axpy", c_i32, c_i32, c_i32, ctypes.c_float, c_vp, c_i64, c_vp, c_i64, c_vp)
     sig(lib, "cbm_candidates_last_error", ctypes.c_char_p)
-    sig(lib, "cbm_candidates_workspace_bytes", c_i64, c_i32, c_i32)
-    sig(lib, "cbm_candidates_count", c_i32, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_i64, c_vp)
-    sig(lib, "cbm_candidates_emit", c_i32, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64,
-        c_vp)
+    sig(lib, "cbm_candidates_workspace_bytes", c_i64, c_i32, c_i32, c_i32)
+    sig(lib, "cbm_candidates_count", c_i32, c_i32, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i64, c_vp)
+    sig(lib, "cbm_candidates_emit", c_i32, c_i32, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_i64,
+        c_vp, c_i64, c_vp)
 
 
 # Every symbol include/cbm_hip.h declares (checked by the CPU test suite).

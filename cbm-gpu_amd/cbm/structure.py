@@ -141,19 +141,24 @@ def host_candidates(adj, alpha, n_threads=0):
         lib.cbm_candidates_free(c)
 
 
-def gpu_candidates(adj, alpha, device, n_workgroups=0):
+CANDIDATE_PATHS = {"auto": 0, "tiles": 1, "lists": 2}
+
+
+def gpu_candidates(adj, alpha, device, path="auto", n_workgroups=0):
     """The same candidate edges from the GPU scan (``cbm_candidates_count`` /
     ``_emit``, HIP kernels) on ``device``; returned as ``(src, dst, w)`` int32
-    tensors on that device."""
+    tensors on that device.  ``path``: "auto", "tiles" (LDS counters) or
+    "lists" (global counters and touched lists); same list either way."""
     dev = torch.device(device)
     if dev.type != "cuda":
         raise RuntimeError("tensor is not a CUDA tensor")
     hip = _lib.hip()
     n = adj.n
-    ws_bytes = int(hip.cbm_candidates_workspace_bytes(n, int(n_workgroups)))
-    if ws_bytes < 0:
-        raise ValueError(f"no candidate workspace for {n} rows")
+    pcode = CANDIDATE_PATHS[path]
     with torch.cuda.device(dev):
+        ws_bytes = int(hip.cbm_candidates_workspace_bytes(n, pcode, int(n_workgroups)))
+        if ws_bytes < 0:
+            raise ValueError(f"no candidate workspace for {n} rows ({path}, {n_workgroups} workgroups)")
         stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
         ptr = adj.ptr.to(dev)
         col = adj.col.to(dev)
@@ -162,16 +167,16 @@ def gpu_candidates(adj, alpha, device, n_workgroups=0):
         ws = torch.empty(ws_bytes + 256, dtype=torch.uint8, device=dev)
         ws_ptr = (ws.data_ptr() + 255) // 256 * 256
         counts = torch.empty(n, dtype=torch.int64, device=dev)
-        _lib.check_candidates(hip.cbm_candidates_count(n, ptr.data_ptr(), col.data_ptr(), int(alpha),
-                                                       counts.data_ptr(), ws_ptr, ws_bytes, stream),
-                              "candidates (count)")
+        _lib.check_candidates(hip.cbm_candidates_count(n, ptr.data_ptr(), col.data_ptr(), int(alpha), pcode,
+                                                       int(n_workgroups), counts.data_ptr(), ws_ptr, ws_bytes,
+                                                       stream), "candidates (count)")
         offsets = torch.cumsum(counts, 0) - counts
         total = int(counts.sum().item())
         src, dst, w = (torch.empty(max(total, 1), dtype=torch.int32, device=dev) for _ in range(3))
-        _lib.check_candidates(hip.cbm_candidates_emit(n, ptr.data_ptr(), col.data_ptr(), int(alpha),
-                                                      offsets.data_ptr(), src.data_ptr(), dst.data_ptr(),
-                                                      w.data_ptr(), total, ws_ptr, ws_bytes, stream),
-                              "candidates (emit)")
+        _lib.check_candidates(hip.cbm_candidates_emit(n, ptr.data_ptr(), col.data_ptr(), int(alpha), pcode,
+                                                      int(n_workgroups), offsets.data_ptr(), src.data_ptr(),
+                                                      dst.data_ptr(), w.data_ptr(), total, ws_ptr, ws_bytes,
+                                                      stream), "candidates (emit)")
         del ws
         return src[:total], dst[:total], w[:total]
 

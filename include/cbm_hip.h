@@ -260,12 +260,24 @@ int cbm_set_xcd_slabs(cbm_handle *h, int32_t slabs);
 /* ------------------------------------------------------------------------- */
 
 /*
- * Bytes of device workspace for an n_rows matrix run by n_workgroups
- * persistent workgroups (<= 0: the library's choice, about 4 GiB of counters
- * at most): 256 bytes plus n_rows uint32 counters per workgroup.  -1 for
- * n_rows < 1.
+ * How the overlap counters are held (see cand_api.hip):
+ *   CBM_CAND_TILES  in LDS, column tile by column tile; one 1024-thread
+ *                   workgroup per CU; cost per row ~ sum of deg(k) + (n - r),
+ *                   for matrices up to a few 100k rows
+ *   CBM_CAND_LISTS  n global counters and an n-entry touched list per
+ *                   workgroup in the workspace; cost per row ~ the columns it
+ *                   touches, for any n
+ *   CBM_CAND_AUTO   tiles up to 262144 rows, lists above
  */
-int64_t cbm_candidates_workspace_bytes(int32_t n_rows, int32_t n_workgroups);
+enum { CBM_CAND_AUTO = 0, CBM_CAND_TILES = 1, CBM_CAND_LISTS = 2 };
+
+/*
+ * Bytes of device workspace a call with these arguments needs: 256 bytes,
+ * plus for the lists 2 n_rows uint32 words per workgroup (n_workgroups <= 0:
+ * the library's choice, about 8 GiB of slabs at most).  -1 for invalid
+ * arguments.
+ */
+int64_t cbm_candidates_workspace_bytes(int32_t n_rows, int32_t path, int32_t n_workgroups);
 
 /*
  * Pass 1.  For the binary CSR (ptr int64 [n_rows + 1], col int32 [nnz],
@@ -274,12 +286,13 @@ int64_t cbm_candidates_workspace_bytes(int32_t n_rows, int32_t n_workgroups);
  * the number of candidate edges created for row r: the root edge N -> r, plus
  * for every c > r sharing a column with r, c -> r when h < deg(r) - alpha and
  * h != deg(r), and r -> c when h < deg(c) - alpha and h != deg(c), where
- * h = |A_r xor A_c|.  workspace: device, 256-byte aligned, workspace_bytes as
- * given by cbm_candidates_workspace_bytes (the number of workgroups follows
- * from its size).  Asynchronous on `stream`; allocates nothing.
+ * h = |A_r xor A_c|.  path and n_workgroups as for the workspace, which is
+ * device memory, 256-byte aligned, of at least cbm_candidates_workspace_bytes
+ * bytes.  Asynchronous on `stream`; allocates nothing.
  */
-int cbm_candidates_count(int32_t n_rows, const int64_t *ptr, const int32_t *col, int32_t alpha,
-                         int64_t *row_count, void *workspace, int64_t workspace_bytes, hipStream_t stream);
+int cbm_candidates_count(int32_t n_rows, const int64_t *ptr, const int32_t *col, int32_t alpha, int32_t path,
+                         int32_t n_workgroups, int64_t *row_count, void *workspace, int64_t workspace_bytes,
+                         hipStream_t stream);
 
 /*
  * Pass 2.  Writes the edges of row r to src / dst / w (int32, device) from
@@ -287,11 +300,11 @@ int cbm_candidates_count(int32_t n_rows, const int64_t *ptr, const int32_t *col,
  * the reference creation order: the root edge (src = n_rows, w = deg(r)),
  * then for c ascending, c -> r before r -> c (w = h).  n_cand is the length of
  * the three arrays; no write goes past it.  The concatenation over r is the
- * host scan's list (cbm_candidates_create) entry for entry.
+ * host scan's list (cbm_candidates_create) entry for entry, whatever the path.
  */
-int cbm_candidates_emit(int32_t n_rows, const int64_t *ptr, const int32_t *col, int32_t alpha,
-                        const int64_t *row_offset, int32_t *src, int32_t *dst, int32_t *w, int64_t n_cand,
-                        void *workspace, int64_t workspace_bytes, hipStream_t stream);
+int cbm_candidates_emit(int32_t n_rows, const int64_t *ptr, const int32_t *col, int32_t alpha, int32_t path,
+                        int32_t n_workgroups, const int64_t *row_offset, int32_t *src, int32_t *dst, int32_t *w,
+                        int64_t n_cand, void *workspace, int64_t workspace_bytes, hipStream_t stream);
 
 /* Message describing the last cbm_candidates_* failure on the calling thread. */
 const char *cbm_candidates_last_error(void);

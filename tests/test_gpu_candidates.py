@@ -1,8 +1,9 @@
 """The GPU candidate-edge scan (cand_api.hip, through the C ABI) against the
-host scan and the oracle: the same list entry for entry, on sparse graphs
-(sorted path), dense ones (rows touching more than 4096 columns take the
-counter-walk path), with 1 to many persistent workgroups; the full build fed
-by it equals the host build; writes never pass the caller's array length."""
+host scan and the oracle: the same list entry for entry, by both counter
+paths (LDS tiles, global touched lists), on sparse graphs, dense ones (rows
+sharing columns with thousands of later rows, several LDS tiles per row),
+with 1 to many persistent workgroups; the full build fed by it equals the
+host build; writes never pass the caller's array length."""
 import ctypes
 
 import numpy as np
@@ -52,22 +53,36 @@ def _check_same(got, want):
         assert np.array_equal(g, w)
 
 
+@pytest.mark.parametrize("path", ["tiles", "lists"])
 @pytest.mark.parametrize("alpha", [0, 2, 8])
 @pytest.mark.parametrize("name", ["random40", "random300", "c1", "sparse"])
-def test_gpu_scan_equals_host_scan_and_oracle(graphs, cuda, name, alpha):
+def test_gpu_scan_equals_host_scan_and_oracle(graphs, cuda, name, alpha, path):
     ei, n = graphs[name]
-    print(f"candidates: {name} n={n} alpha={alpha}", flush=True)
+    print(f"candidates: {name} n={n} alpha={alpha} {path}", flush=True)
     with structure.Adjacency(ei[0], ei[1], torch.ones(ei.shape[1]), n) as adj:
         host = structure.host_candidates(adj, alpha)
         for wgs in (1, 7, 0):
-            _check_same(structure.gpu_candidates(adj, alpha, cuda, n_workgroups=wgs), host)
+            _check_same(structure.gpu_candidates(adj, alpha, cuda, path=path, n_workgroups=wgs), host)
     _check_same(host, O.candidates(O.csr_of(ei.numpy(), n), alpha))
 
 
+def _max_passing_columns(src, dst, n):
+    """Most distinct columns c with an edge in one row's part of a list."""
+    s, d = src.numpy(), dst.numpy()
+    root = np.nonzero(s == n)[0]
+    ends = np.append(root[1:], len(s))
+    best = 0
+    for a, b in zip(root, ends):
+        r = d[a]
+        best = max(best, len(np.unique(np.where(s[a + 1:b] == r, d[a + 1:b], s[a + 1:b]))))
+    return best
+
+
 @pytest.mark.parametrize("alpha", [0, 32])
-def test_dense_rows_take_the_counter_walk(graphs, cuda, alpha):
-    """The hub graph's early rows share columns with nearly every later row
-    (far more than the 4096 the sorted path holds); its late rows few."""
+def test_dense_rows(graphs, cuda, alpha):
+    """The hub graph's early rows share columns with nearly every later row:
+    thousands of touched columns per row, and rows with more than the 4096
+    passing columns the lists path sorts in LDS (it walks their counters)."""
     ei, n = graphs["hub"]
     print(f"candidates: hub n={n} alpha={alpha}", flush=True)
     with structure.Adjacency(ei[0], ei[1], torch.ones(ei.shape[1]), n) as adj:
@@ -76,8 +91,42 @@ def test_dense_rows_take_the_counter_walk(graphs, cuda, alpha):
         touched = [int((s[r].indices > r).sum()) for r in range(50)]
         assert max(touched) > 4096
         host = structure.host_candidates(adj, alpha)
-        for wgs in (5, 0):
-            _check_same(structure.gpu_candidates(adj, alpha, cuda, n_workgroups=wgs), host)
+        assert _max_passing_columns(host[0], host[1], n) > 4096
+        for path, wgs in (("tiles", 5), ("tiles", 0), ("lists", 5), ("lists", 0)):
+            _check_same(structure.gpu_candidates(adj, alpha, cuda, path=path, n_workgroups=wgs), host)
+
+
+def test_rows_spanning_many_tiles(cuda):
+    """A 100,000-row graph: rows cross up to four 32768-column LDS tiles, with
+    hub columns in every tile."""
+    n = 100_000
+    ei = structure.copying_model(n, 4, 0.8, 1.0, 0, 31).long()
+    hubs = torch.tensor([5, 40_000, 70_000, 99_990])
+    nb = torch.arange(0, n, 37)
+    extra = [torch.stack([torch.full_like(nb, h), nb]) for h in hubs]
+    extra += [torch.stack([nb, torch.full_like(nb, h)]) for h in hubs]
+    allr = torch.cat([ei] + extra, dim=1)
+    allr = allr[:, allr[0] != allr[1]]
+    key = torch.unique(allr[0] * n + allr[1])
+    ei = torch.stack([key // n, key % n]).to(torch.int32)
+    print(f"candidates: tiles n={n} nnz={ei.shape[1]}", flush=True)
+    with structure.Adjacency(ei[0], ei[1], torch.ones(ei.shape[1]), n) as adj:
+        for alpha in (0, 3):
+            host = structure.host_candidates(adj, alpha)
+            for path in ("tiles", "lists"):
+                _check_same(structure.gpu_candidates(adj, alpha, cuda, path=path), host)
+
+
+def test_large_sparse_graph_takes_the_lists(cuda):
+    """Above 262144 rows the automatic choice is the touched-list path."""
+    n = 300_000
+    ei = structure.copying_model(n, 6, 0.7, 1.0, 0, 17)
+    print(f"candidates: lists n={n} nnz={ei.shape[1]}", flush=True)
+    hip = _lib.hip()
+    assert hip.cbm_candidates_workspace_bytes(n, 0, 3) == hip.cbm_candidates_workspace_bytes(n, 2, 3) > 256
+    with structure.Adjacency(ei[0], ei[1], torch.ones(ei.shape[1]), n) as adj:
+        host = structure.host_candidates(adj, 1)
+        _check_same(structure.gpu_candidates(adj, 1, cuda), host)
 
 
 @pytest.mark.parametrize("name,alpha", [("c1", 0), ("c1", 4), ("hub", 0), ("random300", 1)])
@@ -111,38 +160,44 @@ def test_emit_stays_inside_the_callers_arrays(graphs, cuda):
         total = host[0].numel()
         cap = total // 2
         ptr, col = adj.ptr.to(cuda), adj.col.to(cuda)
-        ws_bytes = int(hip.cbm_candidates_workspace_bytes(n, 3))
-        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=cuda)
-        assert ws.data_ptr() % 256 == 0
-        counts = torch.empty(n, dtype=torch.int64, device=cuda)
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-        _lib.check_candidates(hip.cbm_candidates_count(n, ptr.data_ptr(), col.data_ptr(), 0, counts.data_ptr(),
-                                                       ws.data_ptr(), ws_bytes, stream))
-        offsets = torch.cumsum(counts, 0) - counts
-        assert int(counts.sum()) == total
-        out = [torch.full((total + 64,), -7, dtype=torch.int32, device=cuda) for _ in range(3)]
-        _lib.check_candidates(hip.cbm_candidates_emit(n, ptr.data_ptr(), col.data_ptr(), 0, offsets.data_ptr(),
-                                                      out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(), cap,
-                                                      ws.data_ptr(), ws_bytes, stream))
-        torch.cuda.synchronize()
-        for o, h in zip(out, host):
-            o = o.cpu()
-            assert torch.equal(o[:cap], h[:cap])
-            assert bool((o[cap:] == -7).all())
+        for path in (1, 2):
+            ws_bytes = int(hip.cbm_candidates_workspace_bytes(n, path, 3))
+            ws = torch.empty(ws_bytes, dtype=torch.uint8, device=cuda)
+            assert ws.data_ptr() % 256 == 0
+            counts = torch.empty(n, dtype=torch.int64, device=cuda)
+            _lib.check_candidates(hip.cbm_candidates_count(n, ptr.data_ptr(), col.data_ptr(), 0, path, 3,
+                                                           counts.data_ptr(), ws.data_ptr(), ws_bytes, stream))
+            offsets = torch.cumsum(counts, 0) - counts
+            assert int(counts.sum()) == total
+            out = [torch.full((total + 64,), -7, dtype=torch.int32, device=cuda) for _ in range(3)]
+            _lib.check_candidates(hip.cbm_candidates_emit(n, ptr.data_ptr(), col.data_ptr(), 0, path, 3,
+                                                          offsets.data_ptr(), out[0].data_ptr(), out[1].data_ptr(),
+                                                          out[2].data_ptr(), cap, ws.data_ptr(), ws_bytes, stream))
+            torch.cuda.synchronize()
+            for o, h in zip(out, host):
+                o = o.cpu()
+                assert torch.equal(o[:cap], h[:cap])
+                assert bool((o[cap:] == -7).all())
 
 
-def test_bad_workspaces_are_rejected(cuda):
+def test_bad_arguments_are_rejected(cuda):
     hip = _lib.hip()
-    assert hip.cbm_candidates_workspace_bytes(0, 0) == -1
-    assert hip.cbm_candidates_workspace_bytes(100, 4) == 256 + 4 * 100 * 4
+    assert hip.cbm_candidates_workspace_bytes(0, 0, 0) == -1
+    assert hip.cbm_candidates_workspace_bytes(100, 3, 0) == -1
+    assert hip.cbm_candidates_workspace_bytes(100, 2, 4) == 256 + 4 * 100 * 8
+    assert hip.cbm_candidates_workspace_bytes(100, 1, 4) == 256
     ptr = torch.zeros(3, dtype=torch.int64, device=cuda)
     col = torch.zeros(1, dtype=torch.int32, device=cuda)
     counts = torch.empty(2, dtype=torch.int64, device=cuda)
     ws = torch.empty(4096, dtype=torch.uint8, device=cuda)
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     with pytest.raises(ValueError, match="aligned"):
-        _lib.check_candidates(hip.cbm_candidates_count(2, ptr.data_ptr(), col.data_ptr(), 0, counts.data_ptr(),
-                                                       ws.data_ptr() + 4, 1024, stream))
-    with pytest.raises(ValueError, match="holds no counter slab"):
-        _lib.check_candidates(hip.cbm_candidates_count(2, ptr.data_ptr(), col.data_ptr(), 0, counts.data_ptr(),
-                                                       ws.data_ptr(), 260, stream))
+        _lib.check_candidates(hip.cbm_candidates_count(2, ptr.data_ptr(), col.data_ptr(), 0, 2, 1,
+                                                       counts.data_ptr(), ws.data_ptr() + 4, 1024, stream))
+    with pytest.raises(ValueError, match="needed"):
+        _lib.check_candidates(hip.cbm_candidates_count(2, ptr.data_ptr(), col.data_ptr(), 0, 2, 1,
+                                                       counts.data_ptr(), ws.data_ptr(), 260, stream))
+    with pytest.raises(ValueError, match="invalid"):
+        _lib.check_candidates(hip.cbm_candidates_count(2, ptr.data_ptr(), col.data_ptr(), 0, 7, 1,
+                                                       counts.data_ptr(), ws.data_ptr(), 4096, stream))
