@@ -10,8 +10,7 @@ class cbm4ad(cbm4mm):
     kernels apply the scale to each gathered row of X instead of storing
     scaled values.  ``matmul(x, y)`` writes ``y = A D^-1/2 x``."""
 
-    def __init__(self, edge_index, edge_values, alpha=0, **kwargs):
-        super().__init__(edge_index, edge_values, alpha, **kwargs)
+    def _scale(self):
         self.D = self._degree_scale()
         column_indices = self.deltas.col_indices()
         new_values = self.deltas.values() * self.D[column_indices]

@@ -12,8 +12,7 @@ class cbm4dad(cbm4mm):
     scaling every row; this builds the same values in one vectorised step.
     ``matmul(x, y)`` writes ``y = D^-1/2 A D^-1/2 x``."""
 
-    def __init__(self, edge_index, edge_values, alpha=0, **kwargs):
-        super().__init__(edge_index, edge_values, alpha, **kwargs)
+    def _scale(self):
         self.D = self._degree_scale()
         crow = self.deltas.crow_indices()
         column_indices = self.deltas.col_indices()

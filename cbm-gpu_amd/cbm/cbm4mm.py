@@ -3,6 +3,7 @@ import torch
 
 from . import cbm_cusparse as cusparse
 from . import cbm_mkl_cpp as cbm_mkl
+from . import storage, structure
 from ._hip_sigs import MODE_AX
 from .autograd import cbm_apply
 
@@ -22,8 +23,9 @@ class cbm4mm:
     ``num_nodes``, ``mca_offset``, ``mca_src_idx``, ``mca_dst_idx`` and
     ``cusparse_extension``.  ``matmul(x, y)`` writes ``y = A x`` in place.
     Additions: ``x @``-style autograd product via :meth:`apply`, BF16 ``x``,
-    ``num_nodes=``, the candidate-edge scan on the GPU (``candidates_on``) and
-    the build statistics in ``build_stats``.
+    ``num_nodes=``, the candidate-edge scan on the GPU (``candidates_on``),
+    the build statistics in ``build_stats``, and :meth:`save` / :meth:`load`
+    of the built structure.
     """
 
     def __init__(self, edge_index, edge_values, alpha=0, *, num_nodes=None, device="cuda", n_threads=0,
@@ -35,18 +37,50 @@ class cbm4mm:
             candidates_on = device if torch.device(device).type == "cuda" else None
         built = cbm_mkl.build(edge_index[0], edge_index[1], edge_values, num_rows, alpha, n_threads=n_threads,
                               candidates_on=candidates_on)
-        self.build_stats = built.stats
+        self.alpha = int(alpha)
+        self._assemble(num_rows, built.csr_ptr, built.csr_col, built.csr_val, built.branches, built.tree_src,
+                       built.tree_dst, built.stats, device, n_threads)
+
+    def _assemble(self, num_rows, crow, col, val, branches, tree_src, tree_dst, stats, device, n_threads, plan=None):
+        self.build_stats = stats
         # Delta as CSR sorted by (row, col): what coalesce().to_sparse_csr()
         # gives in the reference (cbm/cbm4mm.py:26-42), built directly.
-        self.deltas = torch.sparse_csr_tensor(built.csr_ptr, built.csr_col, built.csr_val,
-                                              (num_rows, num_rows)).to(device=device)
+        self.deltas = torch.sparse_csr_tensor(crow, col, val, (num_rows, num_rows)).to(device=device)
         self.num_nodes = num_rows
-        self.mca_offset = built.branches.to(dtype=torch.int32, device=device)
-        self.mca_src_idx = built.tree_src.to(dtype=torch.int32, device=device)
-        self.mca_dst_idx = built.tree_dst.to(dtype=torch.int32, device=device)
+        self.mca_offset = branches.to(dtype=torch.int32, device=device)
+        self.mca_src_idx = tree_src.to(dtype=torch.int32, device=device)
+        self.mca_dst_idx = tree_dst.to(dtype=torch.int32, device=device)
         dummy = torch.empty(0, device=device)
         self.cusparse_extension = cusparse.cusparseCBM(self.deltas, self.mca_offset, self.mca_src_idx,
-                                                       self.mca_dst_idx, dummy, mode=MODE_AX, n_threads=n_threads)
+                                                       self.mca_dst_idx, dummy, mode=MODE_AX, n_threads=n_threads,
+                                                       plan=plan)
+        self._scale()
+
+    def _scale(self):
+        """The product's scalings (none for A; see cbm4ad / cbm4dad)."""
+
+    def save(self, path):
+        """Writes the structure (Delta, tree, device layout) to a file that
+        :meth:`load` reads back in place of a rebuild (cbm/storage.py)."""
+        ext = self.cusparse_extension
+        plan = structure.Plan(n=self.num_nodes, depth=ext.depth,
+                              arrays={**{k: v.cpu() for k, v in ext._dev.items()}, **ext._host},
+                              stats=dict(ext.plan_stats))
+        storage.save(path, n=self.num_nodes, alpha=self.alpha, delta_crow=self.deltas.crow_indices(),
+                     delta_col=self.deltas.col_indices(), delta_sign=torch.sign(self.deltas.values()),
+                     mca_offset=self.mca_offset, mca_src_idx=self.mca_src_idx, mca_dst_idx=self.mca_dst_idx,
+                     plan=plan, build_stats=self.build_stats, saved_by=type(self).__name__)
+
+    @classmethod
+    def load(cls, path, *, device="cuda"):
+        """A matrix of this class from a structure file (any ``cbm4*`` class's
+        file: the structure does not depend on the product)."""
+        st = storage.load(path)
+        obj = cls.__new__(cls)
+        obj.alpha = st.alpha
+        obj._assemble(st.n, st.delta_crow, st.delta_col, st.delta_values(), st.mca_offset, st.mca_src_idx,
+                      st.mca_dst_idx, st.build_stats, device, 0, plan=st.plan)
+        return obj
 
     def matmul(self, x, y):
         self.cusparse_extension.matmul_AX_ADX(x, y)

@@ -73,12 +73,14 @@ class cusparseCBM:
     and ``multipliers`` is D^-1/2 for the D A D product or an empty tensor.
 
     Keyword-only ``mode`` (0: A, 1: A D, 2: D A D) and ``scale`` (D^-1/2) skip
-    the inference of the product from the values.  The handle owns device
-    copies of its layout; nothing is allocated per call.
+    the inference of the product from the values; ``plan`` (a
+    :class:`cbm.structure.Plan` of this matrix, e.g. from a structure file)
+    skips the layout build.  The handle owns device copies of its layout;
+    nothing is allocated per call.
     """
 
     def __init__(self, deltas, mca_branches, mca_src_node_idx, mca_dst_node_idx, multipliers, *, mode=None,
-                 scale=None, n_threads=0):
+                 scale=None, n_threads=0, plan=None):
         for t in (deltas, mca_branches, mca_src_node_idx, mca_dst_node_idx):
             _require_cuda(t)
         if multipliers is not None and multipliers.numel():
@@ -99,8 +101,12 @@ class cusparseCBM:
         elif mode != MODE_AX and scale is None:
             raise ValueError(f"mode {mode} needs the D^-1/2 vector as `scale`")
         self.device = deltas.device
-        plan = structure.plan(n, crow.cpu(), col.cpu(), val.cpu(), mca_src_node_idx.cpu(), mca_dst_node_idx.cpu(),
-                              n_threads=n_threads)
+        if plan is None:
+            plan = structure.plan(n, crow.cpu(), col.cpu(), val.cpu(), mca_src_node_idx.cpu(),
+                                  mca_dst_node_idx.cpu(), n_threads=n_threads)
+        elif (plan.n != n or int(plan.lvl_node.numel()) != n or int(plan.fwd_col.numel()) != int(col.numel())
+              or int(plan.ch_node.numel()) != int(mca_src_node_idx.numel())):
+            raise ValueError("the given layout does not belong to this matrix")
         self.num_rows = n
         self.depth = plan.depth
         self.nnz = int(col.numel())

@@ -1,17 +1,22 @@
-"""A two-layer GCN over the normalised adjacency D^-1/2 A D^-1/2 whose sparse
-products run on the CBM kernels (forward) and the hand-written adjoint
-(backward).  The reference's harness imports ``GCNConv`` but never builds a
-model (benchmark/utilities.py:11); this is the training step its products
-serve.
+"""GNN layers whose sparse products run on the CBM kernels (forward) and the
+hand-written adjoint (backward), and a two-layer GCN built from them.  The
+reference's harness imports ``GCNConv`` and ``GINConv`` but never builds a
+model (benchmark/utilities.py:11); its operator surface holds the GIN-style
+product ``A x + (1 + eps) x`` (s_spmm_add_csr_int32 and
+s_spmm_update_add_csr_int32, cbm/cbm_extensions.cpp:446-591).
 
-Layer order: features are multiplied by the weight first and then propagated,
-``H = relu(Â (X W1))``, ``out = Â (H W2)``, so both sparse products run at the
-hidden width.
+* :class:`GCNConv`  ``out = Â (x W) + b`` with Â the adjacency object's product
+  (``cbm4dad``: D^-1/2 A D^-1/2), weight first so the product runs at the
+  output width.
+* :class:`GINConv`  ``out = mlp(A x + (1 + eps) x)`` with A the adjacency
+  object's product (``cbm4mm``); eps fixed or trained.
+* :class:`GCN`      two GCNConv layers, ``out = Â relu(Â (X W1)) W2``.
 """
 import torch
 
+from . import _lib
 from .autograd import cbm_apply
-from .cbm_cusparse import cusparseCBM
+from .cbm_cusparse import current_stream_handle, cusparseCBM
 
 
 class CsrProduct(torch.autograd.Function):
@@ -35,8 +40,89 @@ def csr_apply(csr, x):
     return CsrProduct.apply(x, csr)
 
 
+def propagate(adjacency, h):
+    """``M h`` with autograd, on the CBM kernels (``cbm4*``) or rocSPARSE (``csr4*``)."""
+    ext = adjacency.cusparse_extension
+    if isinstance(ext, cusparseCBM):
+        return cbm_apply(ext, h)
+    return csr_apply(ext, h)
+
+
+def _axpy(coef, x, y):
+    """y += coef * x (FP32, the library's row kernel)."""
+    rc = _lib.hip().cbm_op_axpy(int(y.size(0)), int(y.size(1)), float(coef), x.data_ptr(), x.stride(0),
+                                y.data_ptr(), y.stride(0), current_stream_handle(y.device))
+    if rc != _lib.CBM_OK:
+        _lib._raise(rc, _lib.hip().cbm_op_last_error().decode(errors="replace"), "axpy")
+
+
+class CbmAddProduct(torch.autograd.Function):
+    """``y = M x + (1 + eps) x`` (the GIN aggregation) and its adjoint
+    ``dx = M^T g + (1 + eps) g``, ``d eps = <x, g>``.  Computed in FP32 (a
+    BF16 ``x`` is widened; its gradient is returned in BF16)."""
+
+    @staticmethod
+    def forward(ctx, x, eps, handle):
+        ctx.in_dtype = x.dtype
+        x = x.contiguous().float()
+        y = torch.empty((handle.num_rows, x.size(1)), dtype=torch.float32, device=x.device)
+        handle.forward(x, y)
+        coef = 1.0 + float(eps)
+        _axpy(coef, x, y)
+        ctx.handle = handle
+        ctx.coef = coef
+        ctx.save_for_backward(x, eps)
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        x, eps = ctx.saved_tensors
+        g = gy.contiguous().float()
+        dx = torch.empty((ctx.handle.num_rows, g.size(1)), dtype=torch.float32, device=g.device)
+        ctx.handle.matmul_t(g, dx)
+        _axpy(ctx.coef, g, dx)
+        deps = (x * g).sum().reshape(eps.shape).to(eps.dtype) if ctx.needs_input_grad[1] else None
+        return dx.to(ctx.in_dtype), deps, None
+
+
+class GCNConv(torch.nn.Module):
+    """``out = Â (x W) + b``; ``adjacency`` is a ``cbm4dad`` (CBM kernels) or a
+    ``csr4dad`` (rocSPARSE), or any ``cbm4*`` / ``csr4*`` for another Â."""
+
+    def __init__(self, adjacency, in_dim, out_dim, *, bias=True, seed=0, device="cuda"):
+        super().__init__()
+        gen = torch.Generator().manual_seed(seed)
+        self.weight = torch.nn.Parameter((torch.randn(in_dim, out_dim, generator=gen) / in_dim ** 0.5).to(device))
+        self.bias = torch.nn.Parameter(torch.zeros(out_dim, device=device)) if bias else None
+        self.adjacency = adjacency
+
+    def forward(self, x):
+        out = propagate(self.adjacency, x @ self.weight)
+        return out + self.bias if self.bias is not None else out
+
+
+class GINConv(torch.nn.Module):
+    """``out = mlp(A x + (1 + eps) x)``; ``adjacency`` is a ``cbm4mm`` (or any
+    ``cbm4*`` for another product).  ``mlp`` defaults to the identity."""
+
+    def __init__(self, adjacency, mlp=None, *, eps=0.0, train_eps=False, device="cuda"):
+        super().__init__()
+        ext = adjacency.cusparse_extension
+        if not isinstance(ext, cusparseCBM):
+            raise TypeError("GINConv runs on the CBM kernels: pass a cbm4mm / cbm4ad / cbm4dad adjacency")
+        self.adjacency = adjacency
+        self.mlp = mlp if mlp is not None else torch.nn.Identity()
+        e = torch.tensor([float(eps)], device=device)
+        self.eps = torch.nn.Parameter(e) if train_eps else e
+
+    def forward(self, x):
+        return self.mlp(CbmAddProduct.apply(x, self.eps, self.adjacency.cusparse_extension))
+
+
 class GCN(torch.nn.Module):
-    """``adjacency`` is a ``cbm4dad`` (CBM kernels) or ``csr4dad`` (rocSPARSE)."""
+    """Two GCNConv layers without bias: ``Â relu(Â (X W1)) W2``, the second
+    product at the output width.  ``adjacency`` is a ``cbm4dad`` (CBM kernels)
+    or ``csr4dad`` (rocSPARSE)."""
 
     def __init__(self, adjacency, in_dim, hidden, out_dim, *, seed=0, device="cuda"):
         super().__init__()
@@ -46,10 +132,7 @@ class GCN(torch.nn.Module):
         self.adjacency = adjacency
 
     def propagate(self, h):
-        ext = self.adjacency.cusparse_extension
-        if isinstance(ext, cusparseCBM):
-            return cbm_apply(ext, h)
-        return csr_apply(ext, h)
+        return propagate(self.adjacency, h)
 
     def forward(self, x):
         h = torch.relu(self.propagate(x @ self.w1))

@@ -17,6 +17,7 @@ _PLAN_NAMES = ("lvl_ptr", "lvl_node", "lvl_parent", "fwd_ptr", "fwd_col", "bwd_p
                "bred_cnt", "rev_ptr", "rev_node", "bwd_mid", "ch_mid", "lvl_ppos", "fck_gslot", "fred_gslot", "rev_nlong", "rck_ptr", "rck_pos", "rck_meta", "rck_slot",
                "rred_ptr", "rred_pos", "rred_slot", "rred_cnt")
 _PLAN_STATS = ("depth", "fwd_slots", "bwd_slots", "bwd_nlong", "fwd_gslots", "rev_slots")
+PLAN_NAMES = _PLAN_NAMES
 
 
 def _copy_out(ptr, n, dtype):

@@ -1,0 +1,127 @@
+"""The on-disk structure format (cbm/storage.py): a saved structure reads
+back identical, the file holds no pickles, and damaged or foreign files are
+refused (CPU only)."""
+import io
+import json
+import zipfile
+
+import numpy as np
+import pytest
+import torch
+
+from cbm import storage, structure
+
+
+@pytest.fixture(scope="module")
+def built():
+    ei = structure.copying_model(3000, 5, 0.85, 0.5, 0, 77)
+    b = structure.build(ei[0], ei[1], torch.ones(ei.shape[1]), 3000, 2)
+    pl = structure.plan(3000, b.csr_ptr, b.csr_col, b.csr_val, b.tree_src, b.tree_dst)
+    return b, pl
+
+
+def _save(path, b, pl, **over):
+    kw = dict(n=3000, alpha=2, delta_crow=b.csr_ptr, delta_col=b.csr_col, delta_sign=torch.sign(b.csr_val),
+              mca_offset=b.branches, mca_src_idx=b.tree_src, mca_dst_idx=b.tree_dst, plan=pl,
+              build_stats=b.stats, saved_by="cbm4mm")
+    kw.update(over)
+    storage.save(path, **kw)
+
+
+def test_round_trip(built, tmp_path):
+    b, pl = built
+    path = tmp_path / "m.cbm.npz"
+    _save(path, b, pl)
+    st = storage.load(path)
+    assert st.n == 3000 and st.alpha == 2 and st.saved_by == "cbm4mm"
+    assert st.build_stats == b.stats
+    assert torch.equal(st.delta_crow, b.csr_ptr) and torch.equal(st.delta_col, b.csr_col)
+    assert torch.equal(st.delta_values(), b.csr_val)
+    assert torch.equal(st.mca_offset, b.branches)
+    assert torch.equal(st.mca_src_idx, b.tree_src) and torch.equal(st.mca_dst_idx, b.tree_dst)
+    assert st.plan.depth == pl.depth and st.plan.stats == pl.stats
+    for k in structure.PLAN_NAMES:
+        assert torch.equal(st.plan.arrays[k], pl.arrays[k]), k
+    # nothing in the file needs unpickling
+    with np.load(path, allow_pickle=False) as z:
+        for name in z.files:
+            assert z[name].dtype != object
+
+
+def _rewrite(src, dst, edit):
+    """Copies a structure file, passing each member's array through edit(name, array)."""
+    with np.load(src, allow_pickle=False) as z:
+        arrays = {k: z[k] for k in z.files}
+    arrays = {k: edit(k, v) for k, v in arrays.items()}
+    arrays = {k: v for k, v in arrays.items() if v is not None}
+    with open(dst, "wb") as fh:
+        np.savez(fh, **arrays)
+
+
+def test_damaged_files_are_refused(built, tmp_path):
+    b, pl = built
+    good = tmp_path / "good.npz"
+    _save(good, b, pl)
+
+    def flip(name, a):
+        if name == "delta_col":
+            a = a.copy()
+            a[17] ^= 1
+        return a
+    _rewrite(good, tmp_path / "flip.npz", flip)
+    with pytest.raises(storage.StructureFileError, match="checksum"):
+        storage.load(tmp_path / "flip.npz")
+
+    _rewrite(good, tmp_path / "missing.npz", lambda k, a: None if k == "plan/bwd_row" else a)
+    with pytest.raises(storage.StructureFileError, match="missing"):
+        storage.load(tmp_path / "missing.npz")
+
+    def version(name, a):
+        if name == "meta":
+            m = json.loads(bytes(a).decode())
+            m["version"] = 99
+            return np.frombuffer(json.dumps(m).encode(), dtype=np.uint8)
+        return a
+    _rewrite(good, tmp_path / "version.npz", version)
+    with pytest.raises(storage.StructureFileError, match="version"):
+        storage.load(tmp_path / "version.npz")
+
+    _rewrite(good, tmp_path / "dtype.npz", lambda k, a: a.astype(np.int64) if k == "mca_src_idx" else a)
+    with pytest.raises(storage.StructureFileError, match="int32"):
+        storage.load(tmp_path / "dtype.npz")
+
+    (tmp_path / "junk.npz").write_bytes(b"not a zip")
+    with pytest.raises(storage.StructureFileError, match="not a structure file"):
+        storage.load(tmp_path / "junk.npz")
+
+    buf = io.BytesIO()
+    with zipfile.ZipFile(buf, "w") as zf:
+        zf.writestr("x.npy", b"")
+    (tmp_path / "nometa.npz").write_bytes(buf.getvalue())
+    with pytest.raises(storage.StructureFileError):
+        storage.load(tmp_path / "nometa.npz")
+
+
+def test_inconsistent_sizes_are_refused(built, tmp_path):
+    """A file whose arrays checksum but do not fit together (n disagrees)."""
+    b, pl = built
+    path = tmp_path / "n.npz"
+    _save(path, b, pl)
+
+    def bad_n(name, a):
+        if name == "meta":
+            m = json.loads(bytes(a).decode())
+            m["n"] = 2999
+            return np.frombuffer(json.dumps(m).encode(), dtype=np.uint8)
+        return a
+    _rewrite(path, tmp_path / "n2.npz", bad_n)
+    with pytest.raises(storage.StructureFileError, match="sizes"):
+        storage.load(tmp_path / "n2.npz")
+
+
+def test_save_checks_its_input(built, tmp_path):
+    b, pl = built
+    with pytest.raises(ValueError, match="n x n"):
+        _save(tmp_path / "x.npz", b, pl, n=2999)
+    with pytest.raises(ValueError, match="delta_sign"):
+        _save(tmp_path / "y.npz", b, pl, delta_sign=torch.zeros_like(b.csr_val))
