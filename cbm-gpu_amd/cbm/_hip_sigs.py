@@ -106,10 +106,14 @@ def declare(lib, sig):
     sig(lib, "cbm_op_update", c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_vp)
     sig(lib, "cbm_op_axpy", c_i32, c_i32, c_i32, ctypes.c_float, c_vp, c_i64, c_vp, c_i64, c_vp)
     sig(lib, "cbm_candidates_last_error", ctypes.c_char_p)
+    sig(lib, "cbm_cluster_last_error", ctypes.c_char_p)
+    sig(lib, "cbm_cluster_column_counts", c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp)
+    sig(lib, "cbm_cluster_fingerprints", c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_vp, c_vp)
     sig(lib, "cbm_candidates_workspace_bytes", c_i64, c_i32, c_i32, c_i32)
-    sig(lib, "cbm_candidates_count", c_i32, c_i32, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i64, c_vp)
-    sig(lib, "cbm_candidates_emit", c_i32, c_i32, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_i64,
-        c_vp, c_i64, c_vp)
+    sig(lib, "cbm_candidates_count", c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_i64,
+        c_vp)
+    sig(lib, "cbm_candidates_emit", c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp,
+        c_vp, c_i64, c_vp, c_i64, c_vp)
 
 
 # Every symbol include/cbm_hip.h declares (checked by the CPU test suite).
@@ -119,4 +123,5 @@ EXPORTED = ("cbm_last_error", "cbm_desc_size", "cbm_create", "cbm_destroy", "cbm
             "cbm_set_forward_split", "cbm_set_wave_rows", "cbm_set_xcd_slabs", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
             "csr_last_error", "cbm_op_spmm", "cbm_op_update", "cbm_op_axpy", "cbm_op_last_error",
             "cbm_candidates_workspace_bytes", "cbm_candidates_count", "cbm_candidates_emit",
-            "cbm_candidates_last_error")
+            "cbm_candidates_last_error", "cbm_cluster_column_counts", "cbm_cluster_fingerprints",
+            "cbm_cluster_last_error")

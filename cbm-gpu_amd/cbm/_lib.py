@@ -70,6 +70,9 @@ def host():
             _sig(lib, "cbm_adjacency_nnz", c_i64, c_vp)
             _sig(lib, "cbm_adjacency_ptr", c_vp, c_vp)
             _sig(lib, "cbm_adjacency_col", c_vp, c_vp)
+            _sig(lib, "cbm_adjacency_symmetric", c_i32, c_vp)
+            _sig(lib, "cbm_adjacency_t_ptr", c_vp, c_vp)
+            _sig(lib, "cbm_adjacency_t_col", c_vp, c_vp)
             _sig(lib, "cbm_adjacency_free", None, c_vp)
             _sig(lib, "cbm_candidates_create", c_i32, c_vp, c_i32, c_i32, P_vp)
             _sig(lib, "cbm_candidates_size", c_i64, c_vp)

@@ -99,7 +99,8 @@ def _coo(row, col, val):
 class Adjacency:
     """Binary CSR of A on the host (``cbm_adjacency_create``): rows sorted,
     duplicates merged, values all ones.  ``ptr`` int64 ``[n+1]``, ``col``
-    int32 ``[nnz]`` as CPU tensors; the native object is released by
+    int32 ``[nnz]`` as CPU tensors, and ``t_ptr``, ``t_col`` for A^T (the
+    same tensors when ``symmetric``); the native object is released by
     :meth:`close` (or ``with``)."""
 
     def __init__(self, row, col, val, n_rows):
@@ -113,6 +114,12 @@ class Adjacency:
         self.nnz = int(self._lib.cbm_adjacency_nnz(self._p))
         self.ptr = _copy_out(self._lib.cbm_adjacency_ptr(self._p), self.n + 1, torch.int64)
         self.col = _copy_out(self._lib.cbm_adjacency_col(self._p), self.nnz, torch.int32)
+        self.symmetric = bool(self._lib.cbm_adjacency_symmetric(self._p))
+        if self.symmetric:
+            self.t_ptr, self.t_col = self.ptr, self.col
+        else:
+            self.t_ptr = _copy_out(self._lib.cbm_adjacency_t_ptr(self._p), self.n + 1, torch.int64)
+            self.t_col = _copy_out(self._lib.cbm_adjacency_t_col(self._p), self.nnz, torch.int32)
 
     def close(self):
         if self._p:
@@ -165,16 +172,23 @@ def gpu_candidates(adj, alpha, device, path="auto", n_workgroups=0):
         col = adj.col.to(dev)
         if col.numel() == 0:
             col = torch.zeros(1, dtype=torch.int32, device=dev)  # a valid pointer for an empty matrix
+        if adj.symmetric:
+            t_ptr, t_col = ptr, col
+        else:
+            t_ptr = adj.t_ptr.to(dev)
+            t_col = adj.t_col.to(dev) if adj.nnz else col
         ws = torch.empty(ws_bytes + 256, dtype=torch.uint8, device=dev)
         ws_ptr = (ws.data_ptr() + 255) // 256 * 256
         counts = torch.empty(n, dtype=torch.int64, device=dev)
-        _lib.check_candidates(hip.cbm_candidates_count(n, ptr.data_ptr(), col.data_ptr(), int(alpha), pcode,
+        _lib.check_candidates(hip.cbm_candidates_count(n, ptr.data_ptr(), col.data_ptr(), t_ptr.data_ptr(),
+                                                       t_col.data_ptr(), int(alpha), pcode,
                                                        int(n_workgroups), counts.data_ptr(), ws_ptr, ws_bytes,
                                                        stream), "candidates (count)")
         offsets = torch.cumsum(counts, 0) - counts
         total = int(counts.sum().item())
         src, dst, w = (torch.empty(max(total, 1), dtype=torch.int32, device=dev) for _ in range(3))
-        _lib.check_candidates(hip.cbm_candidates_emit(n, ptr.data_ptr(), col.data_ptr(), int(alpha), pcode,
+        _lib.check_candidates(hip.cbm_candidates_emit(n, ptr.data_ptr(), col.data_ptr(), t_ptr.data_ptr(),
+                                                      t_col.data_ptr(), int(alpha), pcode,
                                                       int(n_workgroups), offsets.data_ptr(), src.data_ptr(),
                                                       dst.data_ptr(), w.data_ptr(), total, ws_ptr, ws_bytes,
                                                       stream), "candidates (emit)")

@@ -4,20 +4,21 @@
 // stay on the host (cbm_build_from_candidates in cbm_host.h).
 //
 // Work: for every row r, the overlaps |A_r & A_c| with every c > r that
-// shares a column with r, i.e. for each k in A_r the entries of A_k above r
-// -- sum_k deg(k)^2 / 2 counter increments in all.  Integer work bound by
+// shares a column with r, i.e. for each column k of A_r the rows above r
+// that hold k (row k of A^T, which is A itself for a symmetric A) -- the
+// upper triangle of A A^T, sum_k colcount(k)^2 / 2 counter increments.  Integer work bound by
 // the rate of scattered counter updates and by list-search latency, not a
 // dense contraction.  A persistent grid takes rows from a global work
 // counter, one row per workgroup at a time; the wavefronts of a workgroup
-// take the rows k of A_r in turn, find the first entry of A_k in range with
-// a 64-way wavefront search, and count the entries after it.  Edges that
+// take the columns k of A_r in turn, find the first row of column k in range
+// with a 64-way wavefront search, and count the rows after it.  Edges that
 // pass the rule are placed by workgroup prefix sums, so each row's output is
 // the reference's creation order (root edge, then for c ascending: c -> r,
 // then r -> c).  The count pass writes the number of edges per row, the emit
 // pass repeats the work and writes the edges at the caller's offsets.
 //
 // Two ways to hold the counters:
-//   tiles  (n <= kTileMaxRows): the columns above r are cut into tiles of
+//   tiles  (n <= kTileMaxRows): the rows above r are cut into tiles of
 //          kTileCols; a tile's counters live in LDS (128 KB, one 1024-thread
 //          workgroup per CU), are counted with LDS atomics and then walked in
 //          column order.  Cost per row ~ sum deg(k) + (n - r): for small n.
@@ -62,8 +63,10 @@ void set_error(const char *fmt, ...) {
 struct CandArgs {
     int32_t n = 0;
     int32_t alpha = 0;
-    const int64_t *ptr = nullptr;
+    const int64_t *ptr = nullptr;      // A: row offsets (degrees) and columns
     const int32_t *col = nullptr;
+    const int64_t *t_ptr = nullptr;    // A^T: for each column, the rows holding it
+    const int32_t *t_col = nullptr;
     int64_t *row_count = nullptr;      // count pass: edges per row
     const int64_t *row_off = nullptr;  // emit pass: first edge of each row
     int32_t *src = nullptr, *dst = nullptr, *w = nullptr;
@@ -200,9 +203,9 @@ __global__ __launch_bounds__(kTileThreads) void k_cand_tiles(const CandArgs a) {
             for (int64_t i = rb + wave; i < re; i += kWaves) {
                 const int32_t k = a.col[i];
                 if (uint32_t(k) >= uint32_t(a.n)) continue;
-                const int64_t ke = a.ptr[k + 1];
-                for (int64_t q = wave_lower_bound(a.col, a.ptr[k], ke, t0, lane) + lane; q < ke; q += 64) {
-                    const int64_t off = int64_t(a.col[q]) - t0;  // in [0, t1 - t0) for sorted columns
+                const int64_t ke = a.t_ptr[k + 1];
+                for (int64_t q = wave_lower_bound(a.t_col, a.t_ptr[k], ke, t0, lane) + lane; q < ke; q += 64) {
+                    const int64_t off = int64_t(a.t_col[q]) - t0;  // in [0, t1 - t0) for sorted rows
                     if (uint64_t(off) >= uint64_t(t1 - t0)) break;
                     atomicAdd(&cnt[off], 1u);
                 }
@@ -251,9 +254,9 @@ __global__ __launch_bounds__(kListThreads) void k_cand_lists(const CandArgs a) {
         for (int64_t i = rb + wave; i < re; i += kWaves) {
             const int32_t k = a.col[i];
             if (uint32_t(k) >= uint32_t(a.n)) continue;
-            const int64_t ke = a.ptr[k + 1];
-            for (int64_t q = wave_lower_bound(a.col, a.ptr[k], ke, r + 1, lane) + lane; q < ke; q += 64) {
-                const int32_t c = a.col[q];
+            const int64_t ke = a.t_ptr[k + 1];
+            for (int64_t q = wave_lower_bound(a.t_col, a.t_ptr[k], ke, r + 1, lane) + lane; q < ke; q += 64) {
+                const int32_t c = a.t_col[q];
                 if (uint32_t(c) >= uint32_t(a.n)) continue;
                 if (atomicAdd(&cnt[c], 1u) == 0u) touched[atomicAdd(&s_touched, 1)] = c;
             }
@@ -363,11 +366,13 @@ bool resolve(int32_t n_rows, int32_t path, int32_t n_workgroups, Shape &s) {
     return true;
 }
 
-int run_pass(bool emit, int32_t n_rows, const int64_t *ptr, const int32_t *col, int32_t alpha, int32_t path,
+int run_pass(bool emit, int32_t n_rows, const int64_t *ptr, const int32_t *col, const int64_t *t_ptr,
+             const int32_t *t_col, int32_t alpha, int32_t path,
              int32_t n_workgroups, int64_t *row_count, const int64_t *row_off, int32_t *src, int32_t *dst, int32_t *w,
              int64_t cap, void *workspace, int64_t workspace_bytes, hipStream_t stream, const char *what) {
     Shape s;
-    if (!resolve(n_rows, path, n_workgroups, s) || !ptr || !col || !workspace || (!emit && !row_count) ||
+    if (!resolve(n_rows, path, n_workgroups, s) || !ptr || !col || !t_ptr || !t_col || !workspace ||
+        (!emit && !row_count) ||
         (emit && (!row_off || cap < 0 || (cap > 0 && (!src || !dst || !w))))) {
         set_error("%s: invalid arguments (n_rows=%d path=%d n_workgroups=%d)", what, n_rows, path, n_workgroups);
         return CBM_ERR_ARG;
@@ -385,6 +390,8 @@ int run_pass(bool emit, int32_t n_rows, const int64_t *ptr, const int32_t *col, 
     a.alpha = alpha;
     a.ptr = ptr;
     a.col = col;
+    a.t_ptr = t_ptr;
+    a.t_col = t_col;
     a.row_count = row_count;
     a.row_off = row_off;
     a.src = src;
@@ -429,17 +436,18 @@ extern "C" int64_t cbm_candidates_workspace_bytes(int32_t n_rows, int32_t path, 
     return resolve(n_rows, path, n_workgroups, s) ? s.bytes : -1;
 }
 
-extern "C" int cbm_candidates_count(int32_t n_rows, const int64_t *ptr, const int32_t *col, int32_t alpha,
-                                    int32_t path, int32_t n_workgroups, int64_t *row_count, void *workspace,
-                                    int64_t workspace_bytes, hipStream_t stream) {
-    return run_pass(false, n_rows, ptr, col, alpha, path, n_workgroups, row_count, nullptr, nullptr, nullptr, nullptr,
-                    0, workspace, workspace_bytes, stream, "cbm_candidates_count");
+extern "C" int cbm_candidates_count(int32_t n_rows, const int64_t *ptr, const int32_t *col, const int64_t *t_ptr,
+                                    const int32_t *t_col, int32_t alpha, int32_t path, int32_t n_workgroups,
+                                    int64_t *row_count, void *workspace, int64_t workspace_bytes,
+                                    hipStream_t stream) {
+    return run_pass(false, n_rows, ptr, col, t_ptr, t_col, alpha, path, n_workgroups, row_count, nullptr, nullptr,
+                    nullptr, nullptr, 0, workspace, workspace_bytes, stream, "cbm_candidates_count");
 }
 
-extern "C" int cbm_candidates_emit(int32_t n_rows, const int64_t *ptr, const int32_t *col, int32_t alpha,
-                                   int32_t path, int32_t n_workgroups, const int64_t *row_offset, int32_t *src,
-                                   int32_t *dst, int32_t *w, int64_t n_cand, void *workspace,
-                                   int64_t workspace_bytes, hipStream_t stream) {
-    return run_pass(true, n_rows, ptr, col, alpha, path, n_workgroups, nullptr, row_offset, src, dst, w, n_cand,
-                    workspace, workspace_bytes, stream, "cbm_candidates_emit");
+extern "C" int cbm_candidates_emit(int32_t n_rows, const int64_t *ptr, const int32_t *col, const int64_t *t_ptr,
+                                   const int32_t *t_col, int32_t alpha, int32_t path, int32_t n_workgroups,
+                                   const int64_t *row_offset, int32_t *src, int32_t *dst, int32_t *w, int64_t n_cand,
+                                   void *workspace, int64_t workspace_bytes, hipStream_t stream) {
+    return run_pass(true, n_rows, ptr, col, t_ptr, t_col, alpha, path, n_workgroups, nullptr, row_offset, src, dst, w,
+                    n_cand, workspace, workspace_bytes, stream, "cbm_candidates_emit");
 }

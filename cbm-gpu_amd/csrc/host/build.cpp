@@ -4,8 +4,9 @@
 //
 // Differences in method, not in result:
 //   * A*A^T is never materialised: each row's overlap counts are accumulated
-//     in a dense per-thread counter and consumed at once (the reference
-//     exports the full upper triangle from mkl_sparse_syrk, :101-120);
+//     in a dense per-thread counter (walking A^T, which is A itself when A is
+//     symmetric) and consumed at once (the reference exports the full upper
+//     triangle from mkl_sparse_syrk, :101-120);
 //   * the candidate list is built once, in parallel over row chunks, and
 //     concatenated in row order, which is exactly the creation order of the
 //     reference's two serial scans (:122-151 and :158-182);
@@ -98,6 +99,9 @@ struct Candidates {
 
 struct cbm_adjacency {
     Csr a;
+    Csr at;                 // A^T (the rows holding each column); empty when A is symmetric
+    bool symmetric = false;
+    const Csr &columns() const { return symmetric ? a : at; }
 };
 
 struct cbm_candidates {
@@ -106,7 +110,34 @@ struct cbm_candidates {
 
 namespace {
 
-void scan_candidates(const Csr &a, int32_t alpha, Candidates &out) {
+// A^T by counting sort: rows of A^T (columns of A) ascending, each with its
+// row ids ascending.
+void transpose(const Csr &a, Csr &at) {
+    const int32_t n = a.n;
+    at.n = n;
+    at.ptr.assign(size_t(n) + 1, 0);
+    for (const int32_t c : a.col) ++at.ptr[size_t(c) + 1];
+    std::partial_sum(at.ptr.begin(), at.ptr.end(), at.ptr.begin());
+    at.col.resize(a.col.size());
+    std::vector<int64_t> fill(at.ptr.begin(), at.ptr.end() - 1);
+    for (int32_t r = 0; r < n; ++r)
+        for (int64_t i = a.ptr[size_t(r)]; i < a.ptr[size_t(r) + 1]; ++i) at.col[size_t(fill[size_t(a.col[size_t(i)])]++)] = r;
+}
+
+// Builds A^T and keeps it only if A is not symmetric.
+void attach_columns(cbm_adjacency &adj) {
+    transpose(adj.a, adj.at);
+    adj.symmetric = adj.at.ptr == adj.a.ptr && adj.at.col == adj.a.col;
+    if (adj.symmetric) {
+        std::vector<int64_t>().swap(adj.at.ptr);
+        std::vector<int32_t>().swap(adj.at.col);
+    }
+}
+
+// Overlaps |A_r & A_c| = sum over columns k of A_r of [c holds k]: for each
+// k in A_r, the rows c > r of column k (row k of `at` = A^T), which is the
+// upper triangle of A A^T (the reference's mkl_sparse_syrk, :101-120).
+void scan_candidates(const Csr &a, const Csr &at, int32_t alpha, Candidates &out) {
     const int32_t n = a.n;
     const int32_t chunk_rows = 512;
     const int64_t n_chunks = (int64_t(n) + chunk_rows - 1) / chunk_rows;
@@ -129,9 +160,9 @@ void scan_candidates(const Csr &a, int32_t alpha, Candidates &out) {
                 touched.clear();
                 for (int64_t i = a.ptr[size_t(r)]; i < a.ptr[size_t(r) + 1]; ++i) {
                     const int32_t k = a.col[size_t(i)];
-                    // Columns of row k are sorted: skip to those above r.
-                    const int32_t *kb = a.col.data() + a.ptr[size_t(k)];
-                    const int32_t *ke = a.col.data() + a.ptr[size_t(k) + 1];
+                    // Rows holding column k are sorted: skip to those above r.
+                    const int32_t *kb = at.col.data() + at.ptr[size_t(k)];
+                    const int32_t *ke = at.col.data() + at.ptr[size_t(k) + 1];
                     for (const int32_t *q = std::upper_bound(kb, ke, r); q < ke; ++q)
                         if (overlap[size_t(*q)]++ == 0) touched.push_back(*q);
                 }
@@ -395,12 +426,16 @@ extern "C" int cbm_build_create(const int32_t *row, const int32_t *col, const fl
     *out = nullptr;
     ThreadScope threads(n_threads);
     try {
-        Csr a;
-        int rc = build_csr(row, col, val, nnz, n_rows, a);
+        cbm_adjacency adj;
+        int rc = build_csr(row, col, val, nnz, n_rows, adj.a);
         if (rc != CBM_OK) return rc;
+        attach_columns(adj);
         Candidates cand;
-        scan_candidates(a, alpha, cand);
-        return build_from(a, cand.src.data(), cand.dst.data(), cand.w.data(), int64_t(cand.src.size()), &cand, out);
+        scan_candidates(adj.a, adj.columns(), alpha, cand);
+        std::vector<int64_t>().swap(adj.at.ptr);
+        std::vector<int32_t>().swap(adj.at.col);
+        return build_from(adj.a, cand.src.data(), cand.dst.data(), cand.w.data(), int64_t(cand.src.size()), &cand,
+                          out);
     } catch (const std::bad_alloc &) {
         set_error("init: out of host memory");
         return CBM_ERR_NOMEM;
@@ -418,6 +453,7 @@ extern "C" int cbm_adjacency_create(const int32_t *row, const int32_t *col, cons
         std::unique_ptr<cbm_adjacency> adj(new cbm_adjacency);
         const int rc = build_csr(row, col, val, nnz, n_rows, adj->a);
         if (rc != CBM_OK) return rc;
+        attach_columns(*adj);
         *out = adj.release();
         return CBM_OK;
     } catch (const std::bad_alloc &) {
@@ -430,6 +466,13 @@ extern "C" int32_t cbm_adjacency_rows(const cbm_adjacency *adj) { return adj ? a
 extern "C" int64_t cbm_adjacency_nnz(const cbm_adjacency *adj) { return adj ? int64_t(adj->a.col.size()) : -1; }
 extern "C" const int64_t *cbm_adjacency_ptr(const cbm_adjacency *adj) { return adj ? adj->a.ptr.data() : nullptr; }
 extern "C" const int32_t *cbm_adjacency_col(const cbm_adjacency *adj) { return adj ? adj->a.col.data() : nullptr; }
+extern "C" int32_t cbm_adjacency_symmetric(const cbm_adjacency *adj) { return adj ? int32_t(adj->symmetric) : -1; }
+extern "C" const int64_t *cbm_adjacency_t_ptr(const cbm_adjacency *adj) {
+    return adj ? adj->columns().ptr.data() : nullptr;
+}
+extern "C" const int32_t *cbm_adjacency_t_col(const cbm_adjacency *adj) {
+    return adj ? adj->columns().col.data() : nullptr;
+}
 extern "C" void cbm_adjacency_free(cbm_adjacency *adj) { delete adj; }
 
 extern "C" int cbm_candidates_create(const cbm_adjacency *adj, int32_t alpha, int32_t n_threads,
@@ -442,7 +485,7 @@ extern "C" int cbm_candidates_create(const cbm_adjacency *adj, int32_t alpha, in
     ThreadScope threads(n_threads);
     try {
         std::unique_ptr<cbm_candidates> c(new cbm_candidates);
-        scan_candidates(adj->a, alpha, c->c);
+        scan_candidates(adj->a, adj->columns(), alpha, c->c);
         *out = c.release();
         return CBM_OK;
     } catch (const std::bad_alloc &) {

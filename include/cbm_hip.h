@@ -281,8 +281,9 @@ int64_t cbm_candidates_workspace_bytes(int32_t n_rows, int32_t path, int32_t n_w
 
 /*
  * Pass 1.  For the binary CSR (ptr int64 [n_rows + 1], col int32 [nnz],
- * columns sorted and unique per row, all in [0, n_rows) -- what
- * cbm_adjacency_create gives), row_count[r] (int64 [n_rows], device) receives
+ * columns sorted and unique per row, all in [0, n_rows)) and the CSR of its
+ * transpose (t_ptr, t_col; the same arrays when A is symmetric) -- what
+ * cbm_adjacency_create gives -- row_count[r] (int64 [n_rows], device) receives
  * the number of candidate edges created for row r: the root edge N -> r, plus
  * for every c > r sharing a column with r, c -> r when h < deg(r) - alpha and
  * h != deg(r), and r -> c when h < deg(c) - alpha and h != deg(c), where
@@ -290,9 +291,9 @@ int64_t cbm_candidates_workspace_bytes(int32_t n_rows, int32_t path, int32_t n_w
  * device memory, 256-byte aligned, of at least cbm_candidates_workspace_bytes
  * bytes.  Asynchronous on `stream`; allocates nothing.
  */
-int cbm_candidates_count(int32_t n_rows, const int64_t *ptr, const int32_t *col, int32_t alpha, int32_t path,
-                         int32_t n_workgroups, int64_t *row_count, void *workspace, int64_t workspace_bytes,
-                         hipStream_t stream);
+int cbm_candidates_count(int32_t n_rows, const int64_t *ptr, const int32_t *col, const int64_t *t_ptr,
+                         const int32_t *t_col, int32_t alpha, int32_t path, int32_t n_workgroups, int64_t *row_count,
+                         void *workspace, int64_t workspace_bytes, hipStream_t stream);
 
 /*
  * Pass 2.  Writes the edges of row r to src / dst / w (int32, device) from
@@ -302,12 +303,39 @@ int cbm_candidates_count(int32_t n_rows, const int64_t *ptr, const int32_t *col,
  * the three arrays; no write goes past it.  The concatenation over r is the
  * host scan's list (cbm_candidates_create) entry for entry, whatever the path.
  */
-int cbm_candidates_emit(int32_t n_rows, const int64_t *ptr, const int32_t *col, int32_t alpha, int32_t path,
-                        int32_t n_workgroups, const int64_t *row_offset, int32_t *src, int32_t *dst, int32_t *w,
-                        int64_t n_cand, void *workspace, int64_t workspace_bytes, hipStream_t stream);
+int cbm_candidates_emit(int32_t n_rows, const int64_t *ptr, const int32_t *col, const int64_t *t_ptr,
+                        const int32_t *t_col, int32_t alpha, int32_t path, int32_t n_workgroups,
+                        const int64_t *row_offset, int32_t *src, int32_t *dst, int32_t *w, int64_t n_cand,
+                        void *workspace, int64_t workspace_bytes, hipStream_t stream);
 
 /* Message describing the last cbm_candidates_* failure on the calling thread. */
 const char *cbm_candidates_last_error(void);
+
+/* ------------------------------------------------------------------------- */
+/* Row clustering: the data passes of the fingerprint clustering of           */
+/* benchmark/clustering_gpu.py:86-128 (the recursion, grouping and merging    */
+/* run on the host, cbm/clustering.py).                                       */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * counts[c] (int32 [n_cols], device) = number of the n_sel selected rows
+ * rows[] that hold column c in the binary CSR (ptr int64 [n_rows + 1], col
+ * int32): the in-degrees within a cluster (clustering_gpu.py:99-100).
+ * Rows or columns out of range are skipped.
+ */
+int cbm_cluster_column_counts(int32_t n_rows, int32_t n_cols, const int64_t *ptr, const int32_t *col,
+                              const int32_t *rows, int32_t n_sel, int32_t *counts, hipStream_t stream);
+
+/*
+ * fp[i] (int32 [n_sel], device) = sum over j < n_hash of 2^j if row rows[i]
+ * holds column hash_cols[j] (clustering_gpu.py:106-110).  1 <= n_hash <= 30.
+ */
+int cbm_cluster_fingerprints(int32_t n_rows, const int64_t *ptr, const int32_t *col, const int32_t *rows,
+                             int32_t n_sel, const int32_t *hash_cols, int32_t n_hash, int32_t *fp,
+                             hipStream_t stream);
+
+/* Message describing the last cbm_cluster_* failure on the calling thread. */
+const char *cbm_cluster_last_error(void);
 
 /* ------------------------------------------------------------------------- */
 /* CSR comparator on the same GPU: rocSPARSE SpMM, FP32.                      */

@@ -121,7 +121,10 @@ void cbm_build_free(cbm_build *b);
  *
  * Stage 1, the binary CSR of A (MKL COO->CSR in the reference, :66-99): rows
  * sorted, duplicate (row, col) pairs merged, values must be all ones.
- * ptr is int64 [n_rows + 1], col int32 [nnz].
+ * ptr is int64 [n_rows + 1], col int32 [nnz].  t_ptr / t_col are the same
+ * for A^T (for each column, the rows holding it, ascending); they are the
+ * arrays of A itself when A is symmetric (cbm_adjacency_symmetric() == 1).
+ * The candidate scans walk A^T to count row overlaps |A_r & A_c|.
  */
 typedef struct cbm_adjacency cbm_adjacency;
 int cbm_adjacency_create(const int32_t *row, const int32_t *col, const float *val, int64_t nnz,
@@ -130,6 +133,9 @@ int32_t cbm_adjacency_rows(const cbm_adjacency *adj);
 int64_t cbm_adjacency_nnz(const cbm_adjacency *adj);
 const int64_t *cbm_adjacency_ptr(const cbm_adjacency *adj);
 const int32_t *cbm_adjacency_col(const cbm_adjacency *adj);
+int32_t cbm_adjacency_symmetric(const cbm_adjacency *adj);
+const int64_t *cbm_adjacency_t_ptr(const cbm_adjacency *adj);
+const int32_t *cbm_adjacency_t_col(const cbm_adjacency *adj);
 void cbm_adjacency_free(cbm_adjacency *adj);
 
 /*

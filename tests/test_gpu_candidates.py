@@ -43,6 +43,15 @@ def graphs():
     }
     # isolated trailing rows and a single edge
     out["sparse"] = (torch.tensor([[0, 3], [3, 0]], dtype=torch.int32), 7)
+    # not symmetric: a row subset of c1 (the matrices the row clustering
+    # builds) and a random directed graph, where row overlaps need A^T
+    c1, _ = out["c1"]
+    keep = (c1[0] % 3) == 0
+    out["c1_rows"] = (c1[:, keep].contiguous(), 2708)
+    rng = np.random.default_rng(5)
+    d = (rng.random((400, 400)) < 0.04) & ~np.eye(400, dtype=bool)
+    r, c = np.nonzero(d)
+    out["directed"] = (torch.from_numpy(np.stack([r, c]).astype(np.int32)), 400)
     return out
 
 
@@ -55,7 +64,7 @@ def _check_same(got, want):
 
 @pytest.mark.parametrize("path", ["tiles", "lists"])
 @pytest.mark.parametrize("alpha", [0, 2, 8])
-@pytest.mark.parametrize("name", ["random40", "random300", "c1", "sparse"])
+@pytest.mark.parametrize("name", ["random40", "random300", "c1", "sparse", "c1_rows", "directed"])
 def test_gpu_scan_equals_host_scan_and_oracle(graphs, cuda, name, alpha, path):
     ei, n = graphs[name]
     print(f"candidates: {name} n={n} alpha={alpha} {path}", flush=True)
@@ -129,7 +138,8 @@ def test_large_sparse_graph_takes_the_lists(cuda):
         _check_same(structure.gpu_candidates(adj, 1, cuda), host)
 
 
-@pytest.mark.parametrize("name,alpha", [("c1", 0), ("c1", 4), ("hub", 0), ("random300", 1)])
+@pytest.mark.parametrize("name,alpha", [("c1", 0), ("c1", 4), ("hub", 0), ("random300", 1), ("c1_rows", 0),
+                                        ("directed", 2)])
 def test_build_with_gpu_candidates_equals_host_build(graphs, cuda, name, alpha):
     ei, n = graphs[name]
     print(f"build: {name} n={n} alpha={alpha}", flush=True)
@@ -156,6 +166,7 @@ def test_emit_stays_inside_the_callers_arrays(graphs, cuda):
     ei, n = graphs["random300"]
     hip = _lib.hip()
     with structure.Adjacency(ei[0], ei[1], torch.ones(ei.shape[1]), n) as adj:
+        assert adj.symmetric  # A^T is A itself
         host = structure.host_candidates(adj, 0)
         total = host[0].numel()
         cap = total // 2
@@ -166,12 +177,14 @@ def test_emit_stays_inside_the_callers_arrays(graphs, cuda):
             ws = torch.empty(ws_bytes, dtype=torch.uint8, device=cuda)
             assert ws.data_ptr() % 256 == 0
             counts = torch.empty(n, dtype=torch.int64, device=cuda)
-            _lib.check_candidates(hip.cbm_candidates_count(n, ptr.data_ptr(), col.data_ptr(), 0, path, 3,
+            _lib.check_candidates(hip.cbm_candidates_count(n, ptr.data_ptr(), col.data_ptr(), ptr.data_ptr(),
+                                                           col.data_ptr(), 0, path, 3,
                                                            counts.data_ptr(), ws.data_ptr(), ws_bytes, stream))
             offsets = torch.cumsum(counts, 0) - counts
             assert int(counts.sum()) == total
             out = [torch.full((total + 64,), -7, dtype=torch.int32, device=cuda) for _ in range(3)]
-            _lib.check_candidates(hip.cbm_candidates_emit(n, ptr.data_ptr(), col.data_ptr(), 0, path, 3,
+            _lib.check_candidates(hip.cbm_candidates_emit(n, ptr.data_ptr(), col.data_ptr(), ptr.data_ptr(),
+                                                          col.data_ptr(), 0, path, 3,
                                                           offsets.data_ptr(), out[0].data_ptr(), out[1].data_ptr(),
                                                           out[2].data_ptr(), cap, ws.data_ptr(), ws_bytes, stream))
             torch.cuda.synchronize()
@@ -193,11 +206,11 @@ def test_bad_arguments_are_rejected(cuda):
     ws = torch.empty(4096, dtype=torch.uint8, device=cuda)
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     with pytest.raises(ValueError, match="aligned"):
-        _lib.check_candidates(hip.cbm_candidates_count(2, ptr.data_ptr(), col.data_ptr(), 0, 2, 1,
+        _lib.check_candidates(hip.cbm_candidates_count(2, ptr.data_ptr(), col.data_ptr(), ptr.data_ptr(), col.data_ptr(), 0, 2, 1,
                                                        counts.data_ptr(), ws.data_ptr() + 4, 1024, stream))
     with pytest.raises(ValueError, match="needed"):
-        _lib.check_candidates(hip.cbm_candidates_count(2, ptr.data_ptr(), col.data_ptr(), 0, 2, 1,
+        _lib.check_candidates(hip.cbm_candidates_count(2, ptr.data_ptr(), col.data_ptr(), ptr.data_ptr(), col.data_ptr(), 0, 2, 1,
                                                        counts.data_ptr(), ws.data_ptr(), 260, stream))
     with pytest.raises(ValueError, match="invalid"):
-        _lib.check_candidates(hip.cbm_candidates_count(2, ptr.data_ptr(), col.data_ptr(), 0, 7, 1,
+        _lib.check_candidates(hip.cbm_candidates_count(2, ptr.data_ptr(), col.data_ptr(), ptr.data_ptr(), col.data_ptr(), 0, 7, 1,
                                                        counts.data_ptr(), ws.data_ptr(), 4096, stream))

@@ -78,6 +78,34 @@ def test_random_graphs_match_oracle(alpha):
         assert np.array_equal(((sp.identity(n_eff) - P) @ a).toarray(), delta.toarray())
 
 
+def test_directed_graphs_match_oracle():
+    """Row overlaps are |A_r & A_c| (A A^T, the reference's syrk) for a matrix
+    that is not symmetric too: candidates, weight, Delta, A = (I - P)^-1 Delta."""
+    rng = np.random.default_rng(11)
+    for trial in range(12):
+        n = int(rng.integers(5, 90))
+        a = (rng.random((n, n)) < float(rng.uniform(0.03, 0.3))) & ~np.eye(n, dtype=bool)
+        if trial % 2:
+            for i in range(1, n):
+                j = int(rng.integers(0, i))
+                a[i] = np.where(rng.random(n) < 0.85, a[j], a[i])
+        r, c = np.nonzero(a)
+        if len(r) == 0:
+            continue
+        ei = np.stack([r, c])
+        alpha = int(trial % 3)
+        b = _build(ei, n, alpha)
+        ref = O.cbm_init(ei, n, alpha)
+        s_, d_, w_ = ref["candidates"]
+        assert b.stats["candidates"] == len(s_)
+        assert b.stats["msa_weight"] == ref["weight"] == b.stats["nnz_delta"]
+        am = O.csr_of(ei, n)
+        _same_structure(b, O.delta_for_parent(am, b.parent.numpy()))
+        delta = sp.csr_matrix((b.csr_val.numpy().astype(float), b.csr_col.numpy(), b.csr_ptr.numpy()), shape=(n, n))
+        P = O.parent_matrix(b.parent.numpy(), n)
+        assert np.array_equal(((sp.identity(n) - P) @ am).toarray(), delta.toarray())
+
+
 def test_workload_graph_matches_oracle():
     """The smallest workload graph: candidates, weight, parents and Delta."""
     n, nb, p, lam, sd, seed, _ = WORKLOAD_GRAPHS[0]

@@ -30,7 +30,13 @@ def _graphs():
     rng = np.random.default_rng(123)
     out = [(_random_graph(rng, n, d, cp), n) for n, d, cp in
            ((40, 0.2, False), (90, 0.1, True), (150, 0.05, True), (64, 0.5, False))]
-    out.append((structure.copying_model(2708, 7, 0.8, 0.58, 0, 1001), 2708))
+    c1 = structure.copying_model(2708, 7, 0.8, 0.58, 0, 1001)
+    out.append((c1, 2708))
+    # not symmetric: a row subset (as the row clustering builds) and a directed graph
+    out.append((c1[:, (c1[0] % 3) == 0].contiguous(), 2708))
+    d = (rng.random((150, 150)) < 0.06) & ~np.eye(150, dtype=bool)
+    r, c = np.nonzero(d)
+    out.append((torch.from_numpy(np.stack([r, c]).astype(np.int32)), 150))
     return out
 
 
@@ -63,6 +69,20 @@ def test_adjacency_is_the_sorted_binary_csr():
         assert np.array_equal(adj.col.numpy(), a.indices)
     with pytest.raises(ValueError, match="binary"):
         structure.Adjacency(ei[0], ei[1], torch.full((ei.shape[1],), 2.0), 500)
+
+
+def test_adjacency_transpose():
+    """A^T for a directed matrix; A itself for a symmetric one."""
+    ei = structure.copying_model(500, 2, 0.8, 1.0, 0, 5)
+    with structure.Adjacency(ei[0], ei[1], torch.ones(ei.shape[1]), 500) as adj:
+        assert adj.symmetric and adj.t_col is adj.col
+    half = ei[:, ei[0] < ei[1]]
+    with structure.Adjacency(half[0], half[1], torch.ones(half.shape[1]), 500) as adj:
+        assert not adj.symmetric
+        at = O.csr_of(half.numpy(), 500).T.tocsr()
+        at.sort_indices()
+        assert np.array_equal(adj.t_ptr.numpy(), at.indptr)
+        assert np.array_equal(adj.t_col.numpy(), at.indices)
 
 
 @pytest.mark.parametrize("alpha", [0, 4])
