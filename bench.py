@@ -430,6 +430,8 @@ def main():
                     help="time the 2-layer GCN training step (c3 graph, hidden 256) instead; CBM vs rocSPARSE CSR")
     ap.add_argument("--profile", action="store_true",
                     help="run only the CBM steps (BF16, then FP32) for a profiler; prints their times")
+    ap.add_argument("--build-only", action="store_true",
+                    help="build the workload's structure and time the build stages only (GPU vs host candidate scan)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (the calibration kernel of counter runs)")
     args = ap.parse_args()
@@ -455,6 +457,16 @@ def main():
 
     if args.gcn:
         gcn_step(args, dev, stream)
+        return
+    if args.build_only:
+        from cbm import structure
+        w = dict(WORKLOADS[args.workload])
+        if args.alpha is not None:
+            w["alpha"] = args.alpha
+        ei = structure.copying_model(w["n"], w["blocks"], w["p"], w["lam"], w["seed_deg"], w["seed"])
+        stages = build_stages(ei, w["n"], w["alpha"], dev, stream)
+        print(json.dumps({"workload": args.workload, "n": w["n"], "nnz": int(ei.shape[1]), "alpha": w["alpha"],
+                          "build_stages": stages}), flush=True)
         return
     w, ei, m = build_workload(args.workload, args.alpha, args.cols, dev)
     if args.profile:

@@ -22,8 +22,8 @@
 //          kTileCols; a tile's counters live in LDS (128 KB, one 1024-thread
 //          workgroup per CU), are counted with LDS atomics and then walked in
 //          column order.  Cost per row ~ sum deg(k) + (n - r): for small n.
-//   lists  (any n): each workgroup owns n global counters and a touched list
-//          of n entries in the workspace.  Counting appends a column to the
+//   lists  (any n): each 1024-thread workgroup owns n global counters and a
+//          touched list of n entries in the workspace.  Counting appends a column to the
 //          list when its counter leaves 0; the list is filtered by the rule
 //          into a short LDS list that alone is sorted (the passing edges are
 //          few), then the touched counters are reset.  Cost per row ~ the
@@ -44,7 +44,7 @@ constexpr int64_t kHeaderBytes = 256;                  // work counter, padded t
 constexpr int kTileThreads = 1024;
 constexpr int32_t kTileCols = 32768;                   // uint32 LDS counters per tile (128 KB)
 constexpr int32_t kTileMaxRows = 262144;               // automatic choice: tiles up to this size
-constexpr int kListThreads = 256;
+constexpr int kListThreads = 1024;                     // 16 wavefronts: the walk is latency-bound
 constexpr int32_t kPassCap = 4096;                     // passing columns sorted in LDS
 constexpr int32_t kMaxListWorkgroups = 2048;
 constexpr int64_t kDefaultListBytes = int64_t(8) << 30;  // counter + list slabs of the default workspace
