@@ -227,6 +227,18 @@ def measure(name, w, ei, m, args, dev, stream):
                       "adjoint_delta_t": round(time_loop(lambda: h.adjoint_delta_t(g16, z, dx), K, W, stream), 4)}
     h.set_xcd_slabs(1)
     log(f"{name}: XCD slabs {xcd}")
+    # gathers by XCD column range (different rounding order), off and on
+    xr = {}
+    if h.plan_stats.get("fwd_pieces", 0) > 0:
+        for on in (False, True):
+            h.set_xcd_ranges(on)
+            xr["on" if on else "off"] = {
+                "step_bf16": round(time_loop(step(x16, g16), K, W, stream), 4),
+                "step_fp32": round(time_loop(step(x32, g32), K, W, stream), 4),
+                "forward": round(time_loop(lambda: h.forward(x16, y), K, W, stream), 4),
+                "adjoint_delta_t": round(time_loop(lambda: h.adjoint_delta_t(g16, z, dx), K, W, stream), 4)}
+        h.set_xcd_ranges(None)
+        log(f"{name}: XCD ranges {xr}")
     # results the CPU leg compares against FP64 (fresh, from the BF16 and FP32 steps)
     out = {}
     for tag, (xx, gg) in (("bf16", (x16, g16)), ("fp32", (x32, g32))):
@@ -300,6 +312,8 @@ def measure(name, w, ei, m, args, dev, stream):
         "dominant_phase": dom, "dominant_gbs": round(dom_gbs, 1),
         "max_rel_error_bf16_vs_fp32_cbm": vs_fp32,
         "ms_xcd_slabs_bf16": xcd,
+        "ms_xcd_ranges": xr,
+        "xcd_ranges_default": bool(xr) and (n * f * 4 <= (256 << 20)),
     }
     return rec, out, (x32, g32, x16, g16), m_csr
 

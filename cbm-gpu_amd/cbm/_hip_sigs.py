@@ -66,14 +66,32 @@ class CbmDesc(ctypes.Structure):
         ("rred_slot", c_vp),
         ("rred_cnt", c_vp),
         ("rev_slots", c_i32),
+        ("fpc_range_host", c_vp),
+        ("fpc_b", c_vp),
+        ("fpc_e", c_vp),
+        ("fpc_slot", c_vp),
+        ("frs_slot", c_vp),
+        ("frs_cnt", c_vp),
+        ("fwd_pieces", c_i32),
+        ("bpc_range_host", c_vp),
+        ("bpc_b", c_vp),
+        ("bpc_e", c_vp),
+        ("bpc_mid", c_vp),
+        ("bpc_slot", c_vp),
+        ("brs_slot", c_vp),
+        ("brs_cnt", c_vp),
+        ("bwd_pieces", c_i32),
+        ("bpc_row", c_vp),
     ]
 
 # Device arrays of the plan that the descriptor points at.
 DEVICE_ARRAYS = ("lvl_node", "lvl_parent", "lvl_ppos", "fwd_ptr", "fwd_col", "rev_node", "ch_ptr", "ch_node", "ch_mid", "fck_pos", "fck_meta",
                  "fck_slot", "fred_pos", "fred_slot", "fred_cnt", "bwd_node", "bwd_ptr", "bwd_row", "bwd_mid", "bck_pos",
                  "bck_meta", "bck_slot", "bred_pos", "bred_slot", "bred_cnt", "fck_gslot", "fred_gslot", "rck_pos", "rck_meta", "rck_slot", "rred_pos",
-                 "rred_slot", "rred_cnt")
-HOST_ARRAYS = ("lvl_ptr", "lvl_nlong", "fck_ptr", "fred_ptr", "rev_ptr", "rev_nlong", "rck_ptr", "rred_ptr")
+                 "rred_slot", "rred_cnt", "fpc_b", "fpc_e", "fpc_slot", "frs_slot", "frs_cnt", "bpc_b", "bpc_e", "bpc_mid",
+                 "bpc_slot", "brs_slot", "brs_cnt", "bpc_row")
+HOST_ARRAYS = ("lvl_ptr", "lvl_nlong", "fck_ptr", "fred_ptr", "rev_ptr", "rev_nlong", "rck_ptr", "rred_ptr", "fpc_range",
+               "bpc_range")
 
 
 def declare(lib, sig):
@@ -95,6 +113,7 @@ def declare(lib, sig):
     sig(lib, "cbm_set_forward_split", c_i32, c_vp, c_i32)
     sig(lib, "cbm_set_wave_rows", c_i32, c_vp, c_i32)
     sig(lib, "cbm_set_xcd_slabs", c_i32, c_vp, c_i32)
+    sig(lib, "cbm_set_xcd_ranges", c_i32, c_vp, c_i32)
     sig(lib, "csr_last_error", ctypes.c_char_p)
     sig(lib, "csr_create", c_i32, c_i32, c_i32, c_i64, c_vp, c_vp, c_vp, P_vp)
     sig(lib, "csr_destroy", c_i32, c_vp)
@@ -120,7 +139,7 @@ def declare(lib, sig):
 EXPORTED = ("cbm_last_error", "cbm_desc_size", "cbm_create", "cbm_destroy", "cbm_reserve", "cbm_matmul", "cbm_matmul_t", "cbm_adjoint_reverse",
             "cbm_adjoint_delta_t", "cbm_delta_spmm",
             "cbm_propagate", "cbm_last_launches", "cbm_set_slab_bytes", "cbm_set_kahan_depth",
-            "cbm_set_forward_split", "cbm_set_wave_rows", "cbm_set_xcd_slabs", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
+            "cbm_set_forward_split", "cbm_set_wave_rows", "cbm_set_xcd_slabs", "cbm_set_xcd_ranges", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
             "csr_last_error", "cbm_op_spmm", "cbm_op_update", "cbm_op_axpy", "cbm_op_last_error",
             "cbm_candidates_workspace_bytes", "cbm_candidates_count", "cbm_candidates_emit",
             "cbm_candidates_last_error", "cbm_cluster_column_counts", "cbm_cluster_fingerprints",

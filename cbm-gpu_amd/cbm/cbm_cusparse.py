@@ -128,7 +128,9 @@ class cusparseCBM:
         desc = CbmDesc(n_rows=self.num_rows, n_levels=self.depth, nnz=self.nnz, n_edges=self.n_edges,
                        mode=self.mode, fwd_slots=st["fwd_slots"], bwd_nlong=st["bwd_nlong"],
                        n_bck=int(self._dev["bck_pos"].numel()), n_bred=int(self._dev["bred_pos"].numel()),
-                       bwd_slots=st["bwd_slots"], fwd_gslots=st["fwd_gslots"], rev_slots=st["rev_slots"], d=self._d.data_ptr() if self._d is not None else None, **ptrs)
+                       bwd_slots=st["bwd_slots"], fwd_gslots=st["fwd_gslots"], rev_slots=st["rev_slots"],
+                       fwd_pieces=st.get("fwd_pieces", 0), bwd_pieces=st.get("bwd_pieces", 0),
+                       d=self._d.data_ptr() if self._d is not None else None, **ptrs)
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             rc = self._lib.cbm_create(ctypes.byref(desc), current_stream_handle(self.device), ctypes.byref(h))
@@ -306,6 +308,12 @@ class cusparseCBM:
         False rows packed per wavefront, None by matrix size (the default)."""
         m = -1 if mode is None else int(bool(mode))
         _lib.check_hip(self._lib.cbm_set_wave_rows(self._h, m), "set_wave_rows")
+
+    def set_xcd_ranges(self, mode):
+        """XCD column ranges for the gather passes: None automatic (by size),
+        False off, True on (when the layout has range pieces)."""
+        code = -1 if mode is None else int(bool(mode))
+        _lib.check_hip(self._lib.cbm_set_xcd_ranges(self._h, code), "set_xcd_ranges")
 
     def set_xcd_slabs(self, slabs):
         """Feature slabs spread over the XCDs (1 = off); same results."""

@@ -15,8 +15,11 @@ _PLAN_NAMES = ("lvl_ptr", "lvl_node", "lvl_parent", "fwd_ptr", "fwd_col", "bwd_p
                "ch_node", "lvl_nlong", "fck_ptr", "fck_pos", "fck_meta", "fck_slot", "fred_ptr", "fred_pos",
                "fred_slot", "fred_cnt", "bwd_node", "bck_pos", "bck_meta", "bck_slot", "bred_pos", "bred_slot",
                "bred_cnt", "rev_ptr", "rev_node", "bwd_mid", "ch_mid", "lvl_ppos", "fck_gslot", "fred_gslot", "rev_nlong", "rck_ptr", "rck_pos", "rck_meta", "rck_slot",
-               "rred_ptr", "rred_pos", "rred_slot", "rred_cnt")
-_PLAN_STATS = ("depth", "fwd_slots", "bwd_slots", "bwd_nlong", "fwd_gslots", "rev_slots")
+               "rred_ptr", "rred_pos", "rred_slot", "rred_cnt", "fpc_b", "fpc_e", "fpc_slot", "fpc_range", "frs_slot",
+               "frs_cnt", "bpc_b", "bpc_e", "bpc_mid", "bpc_slot", "bpc_range", "brs_slot", "brs_cnt",
+               "bpc_row")
+_PLAN_STATS = ("depth", "fwd_slots", "bwd_slots", "bwd_nlong", "fwd_gslots", "rev_slots", "fwd_pieces", "bwd_pieces")
+XCD_RANGES, PIECE, RANGE_MAX_ROWS = 8, 256, 524288  # CBM_XCD_RANGES, CBM_PIECE, CBM_RANGE_MAX_ROWS
 PLAN_NAMES = _PLAN_NAMES
 
 

@@ -25,6 +25,10 @@ struct cbm_handle {
     int32_t fwd_split = -1;   // forward form: 0 fused per level, 1 split, -1 automatic
     int32_t wave_rows = -1;   // short-row passes: whole wavefront per row (1), never (0), by size (-1)
     int32_t xcd_slabs = 1;    // feature slabs spread over the XCDs (1 = off)
+    int32_t xcd_ranges = -1;  // column ranges per XCD: 1 on, 0 off, -1 by size
+    int32_t fpc_range[9] = {}, bpc_range[9] = {};  // host copies of the piece offsets per range
+    float *partial = nullptr; // one partial row per range piece
+    int64_t partial_floats = 0;
     float *comp = nullptr;    // compensation rows of those levels
     int64_t comp_floats = 0;
     int last_launches = 0;
@@ -109,8 +113,40 @@ int ensure_comp(cbm_handle *h, int32_t n_cols) {
     return CBM_OK;
 }
 
+// Whether the gather passes of an n_cols-wide call run by XCD column range.
+// Automatic: rows of 64 entries or more on average (so that a row's pieces
+// are not mostly partial-row overhead) and a gathered matrix of 16 to
+// 256 MiB at FP32 (smaller ones are cache-resident anyway, larger ones leave
+// each XCD's range far beyond its L2).
+bool use_ranges(const cbm_handle *h, int32_t n_cols) {
+    if (h->d.fwd_pieces <= 0 || h->d.bwd_pieces <= 0 || h->xcd_ranges == 0) return false;
+    if (h->xcd_ranges > 0) return true;
+    const int64_t bytes = int64_t(h->d.n_rows) * n_cols * 4;
+    return h->d.nnz >= 64 * int64_t(h->d.n_rows) && bytes >= (int64_t(16) << 20) && bytes <= (int64_t(256) << 20);
+}
+
+int ensure_partial(cbm_handle *h, int32_t n_cols) {
+    if (!use_ranges(h, n_cols)) return CBM_OK;
+    const int64_t need = int64_t(std::max(h->d.fwd_pieces, h->d.bwd_pieces)) * n_cols;
+    if (need <= h->partial_floats) return CBM_OK;
+    if (h->partial) {
+        const hipError_t e = hipFree(h->partial);
+        h->partial = nullptr;
+        h->partial_floats = 0;
+        if (e != hipSuccess) return device_error(e, "cbm range partials");
+    }
+    const hipError_t e = hipMalloc(&h->partial, sizeof(float) * size_t(need));
+    if (e != hipSuccess) {
+        h->partial = nullptr;
+        return device_error(e, "cbm range partials");
+    }
+    h->partial_floats = need;
+    return CBM_OK;
+}
+
 int ensure_scratch(cbm_handle *h, int32_t n_cols) {
-    const int rc = ensure_comp(h, n_cols);
+    int rc = ensure_comp(h, n_cols);
+    if (rc == CBM_OK) rc = ensure_partial(h, n_cols);
     if (rc != CBM_OK) return rc;
     const int64_t slots = std::max<int64_t>({h->d.fwd_slots, h->d.bwd_slots, h->d.fwd_gslots, h->d.rev_slots});
     const int64_t need = slots * int64_t(n_cols);
@@ -253,6 +289,64 @@ int propagate_levels(cbm_handle *h, float *y, int64_t ldy, int32_t n_cols, hipSt
     return CBM_OK;
 }
 
+// Copies the 9 piece offsets of a range table into a pass.
+void set_ranges(cbmk::RowPass &rp, const int32_t *first) {
+    rp.ranges = true;
+    for (int k = 0; k < 9; ++k) rp.range_first[k] = first[k];
+    rp.xcd_slabs = 1;
+    rp.wave_rows = 0;
+    rp.skip_above = INT32_MAX;
+}
+
+// The forward by XCD column range: one pass over all forward pieces (each
+// gathers one range of X into its partial row; range k on XCD k), then a
+// reduction that adds each row's pieces in order and applies the row scale --
+// per tree level with the parent term (compensated from the Kahan level on)
+// for the product, or over all rows at once without it for the Delta half
+// alone (the level propagation then adds the same parent terms, so the two
+// halves give the product's bits).
+int forward_ranges(cbm_handle *h, const cbmk::RowPass &base, bool with_parent, hipStream_t stream,
+                   const char *what) {
+    cbmk::RowPass pc = base;
+    pc.node_of = h->d.fpc_slot;
+    pc.parent_of = nullptr;
+    pc.ptr = h->d.fpc_b;
+    pc.ptr_end = h->d.fpc_e;
+    pc.rowscale = nullptr;
+    pc.escale = nullptr;
+    pc.extra = cbmk::kExtraNone;
+    pc.y = h->partial;
+    pc.ldy = base.ncols;
+    set_ranges(pc, h->fpc_range);
+    hipError_t e = cbmk::launch_row_pass(pc, stream);
+    if (e != hipSuccess) return device_error(e, what);
+    ++h->last_launches;
+    cbmk::RowPass rr = base;
+    rr.g = h->partial;
+    rr.ldg = base.ncols;
+    rr.g_bf16 = false;
+    rr.parent_pos = h->d.lvl_ppos;
+    rr.comp = h->comp;
+    rr.comp_base = comp_base(h);
+    rr.ldc = base.ncols;
+    cbmk::ReduceArgs rd;
+    rd.pos = nullptr;
+    rd.slot = h->d.frs_slot;
+    rd.cnt = h->d.frs_cnt;
+    rd.scratch = h->partial;
+    const int32_t levels = with_parent ? h->d.n_levels : 1;
+    for (int32_t l = 0; l < levels; ++l) {
+        rr.extra = (with_parent && l > 0) ? (kahan_at(h, l) ? cbmk::kExtraParentKahan : cbmk::kExtraParent)
+                                          : cbmk::kExtraNone;
+        rd.first = with_parent ? h->lvl_ptr[size_t(l)] : 0;
+        rd.count = with_parent ? h->lvl_ptr[size_t(l) + 1] - rd.first : h->d.n_rows;
+        e = cbmk::launch_row_reduce(rr, rd, stream);
+        if (e != hipSuccess) return device_error(e, what);
+        ++h->last_launches;
+    }
+    return CBM_OK;
+}
+
 // Which forward form a call uses.  The split form (one Delta pass over all
 // levels, then one light propagation launch per level) keeps the whole chip
 // busy on the gathers however thin the levels are, at the price of one more
@@ -346,12 +440,35 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
         set_error("cbm_create: missing forward chunk tables");
         return CBM_ERR_ARG;
     }
+    const bool fwd_pieces = d.fwd_pieces > 0, bwd_pieces = d.bwd_pieces > 0;
+    if (d.fwd_pieces < 0 || d.bwd_pieces < 0 ||
+        (fwd_pieces && (!d.fpc_range_host || !d.fpc_b || !d.fpc_e || !d.fpc_slot || !d.frs_slot || !d.frs_cnt)) ||
+        (bwd_pieces && (!d.bpc_range_host || !d.bpc_b || !d.bpc_e || !d.bpc_mid || !d.bpc_slot || !d.brs_slot ||
+                        !d.brs_cnt || !d.bpc_row))) {
+        set_error("cbm_create: incomplete range piece tables");
+        return CBM_ERR_ARG;
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        const int32_t *first = pass == 0 ? d.fpc_range_host : d.bpc_range_host;
+        const int32_t total = pass == 0 ? d.fwd_pieces : d.bwd_pieces;
+        if (total <= 0) continue;
+        bool ok = first[0] == 0 && first[8] == total;
+        for (int k = 0; k < 8; ++k) ok = ok && first[k + 1] >= first[k];
+        if (!ok) {
+            set_error("cbm_create: range piece offsets do not run from 0 to the piece count");
+            return CBM_ERR_ARG;
+        }
+    }
     cbm_handle *h = new (std::nothrow) cbm_handle;
     if (!h) {
         set_error("cbm_create: out of host memory");
         return CBM_ERR_NOMEM;
     }
     h->d = d;
+    if (fwd_pieces) std::copy(d.fpc_range_host, d.fpc_range_host + 9, h->fpc_range);
+    if (bwd_pieces) std::copy(d.bpc_range_host, d.bpc_range_host + 9, h->bpc_range);
+    h->d.fpc_range_host = fwd_pieces ? h->fpc_range : nullptr;
+    h->d.bpc_range_host = bwd_pieces ? h->bpc_range : nullptr;
     h->lvl_ptr.assign(d.lvl_ptr_host, d.lvl_ptr_host + L + 1);
     h->lvl_nlong.assign(d.lvl_nlong_host, d.lvl_nlong_host + L);
     h->fck_ptr.assign(d.fck_ptr_host, d.fck_ptr_host + L + 1);
@@ -384,7 +501,7 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
 extern "C" int cbm_destroy(cbm_handle *h) {
     if (!h) return CBM_OK;
     hipError_t e = hipSuccess;
-    for (float *p : {h->ps_pos, h->scratch, h->comp}) {
+    for (float *p : {h->ps_pos, h->scratch, h->comp, h->partial}) {
         if (!p) continue;
         const hipError_t ei = hipFree(p);
         if (e == hipSuccess) e = ei;
@@ -412,6 +529,13 @@ extern "C" int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t l
     const bool split = use_split(h);
     return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
         const cbmk::RowPass rp = forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w);
+        if (use_ranges(h, n_cols)) {
+            // Pieces and one reduction over all rows, then the light level
+            // propagation: the bits of the per-level reduction with the
+            // parent term (forward_ranges with_parent), in fewer, fuller launches.
+            const int r = forward_ranges(h, rp, false, stream, "cbm_matmul");
+            return r == CBM_OK ? propagate_levels(h, y + c0, ldy, w, stream, "cbm_matmul") : r;
+        }
         if (!split) return forward_levels(h, rp, true, stream, "cbm_matmul");
         const int r = delta_flat(h, rp, stream, "cbm_matmul");
         return r == CBM_OK ? propagate_levels(h, y + c0, ldy, w, stream, "cbm_matmul") : r;
@@ -528,6 +652,41 @@ int adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, co
     rd.first = 0;
     rd.count = h->d.n_bred;
     rd.scratch = h->scratch;
+    if (use_ranges(h, n_cols)) {
+        // By XCD column range: every backward piece (one range of G and Z,
+        // its leaf rows before its internal rows) into its partial row, then
+        // one reduction over all positions.
+        cbmk::RowPass pc = tp;
+        pc.node_of = h->d.bpc_slot;
+        pc.idx = h->d.bpc_row;
+        pc.ptr = h->d.bpc_b;
+        pc.ptr_end = h->d.bpc_e;
+        pc.mid = h->d.bpc_mid;
+        pc.rowscale = nullptr;
+        pc.y = h->partial;
+        pc.ldy = n_cols;
+        set_ranges(pc, h->bpc_range);
+        hipError_t e = cbmk::launch_row_pass(pc, stream);
+        if (e != hipSuccess) return device_error(e, "adjoint Delta^T pass");
+        ++h->last_launches;
+        cbmk::RowPass rr = tp;
+        rr.dual = false;
+        rr.g = h->partial;
+        rr.ldg = n_cols;
+        rr.g_bf16 = false;
+        rr.g2 = nullptr;
+        cbmk::ReduceArgs pr;
+        pr.pos = nullptr;
+        pr.slot = h->d.brs_slot;
+        pr.cnt = h->d.brs_cnt;
+        pr.first = 0;
+        pr.count = h->d.n_rows;
+        pr.scratch = h->partial;
+        e = cbmk::launch_row_reduce(rr, pr, stream);
+        if (e != hipSuccess) return device_error(e, "adjoint Delta^T pass");
+        ++h->last_launches;
+        return CBM_OK;
+    }
     return run_range(h, tp, 0, h->d.n_rows, h->d.bwd_nlong, ck, rd, stream, "adjoint Delta^T pass");
 }
 
@@ -573,8 +732,9 @@ extern "C" int cbm_delta_spmm(cbm_handle *h, int32_t dtype, const void *x, int64
     h->last_launches = 0;
     const int elem = dtype == CBM_BF16 ? 2 : 4;
     return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
-        return delta_flat(h, forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w), stream,
-                          "cbm_delta_spmm");
+        const cbmk::RowPass rp = forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w);
+        if (use_ranges(h, n_cols)) return forward_ranges(h, rp, false, stream, "cbm_delta_spmm");
+        return delta_flat(h, rp, stream, "cbm_delta_spmm");
     });
 }
 
@@ -624,6 +784,15 @@ extern "C" int cbm_set_xcd_slabs(cbm_handle *h, int32_t slabs) {
         return CBM_ERR_ARG;
     }
     h->xcd_slabs = slabs;
+    return CBM_OK;
+}
+
+extern "C" int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode) {
+    if (!h || mode < -1 || mode > 1) {
+        set_error("cbm_set_xcd_ranges: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    h->xcd_ranges = mode;
     return CBM_OK;
 }
 

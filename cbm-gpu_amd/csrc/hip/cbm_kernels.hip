@@ -368,12 +368,24 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
     const int lane = int(threadIdx.x & 63u);
     const int l = lane & (LPR - 1);
     const int sub = (lane / LPR) & (S - 1);
-    const int32_t slab = int32_t(blockIdx.x % uint32_t(a.xcd_slabs));
-    const int64_t blk = int64_t(blockIdx.x / uint32_t(a.xcd_slabs));
+    // Workgroup b: slab b % xcd_slabs of row block b / xcd_slabs, or, with
+    // XCD ranges, row block b / 8 of range b % 8's pieces.
+    int32_t slab, p_begin = a.p_begin, p_end = a.p_end;
+    int64_t blk;
+    if (a.ranges) {
+        const uint32_t k = blockIdx.x & 7u;
+        slab = 0;
+        blk = int64_t(blockIdx.x >> 3);
+        p_begin = a.range_first[k];
+        p_end = a.range_first[k + 1];
+    } else {
+        slab = int32_t(blockIdx.x % uint32_t(a.xcd_slabs));
+        blk = int64_t(blockIdx.x / uint32_t(a.xcd_slabs));
+    }
     const int64_t group = (blk * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kRowsPerWave +
                           lane / kLanesPerRow;
-    const int64_t p64 = int64_t(a.p_begin) + group;
-    if (p64 >= int64_t(a.p_end)) return;  // a row's lanes leave together
+    const int64_t p64 = int64_t(p_begin) + group;
+    if (p64 >= int64_t(p_end)) return;  // a row's lanes leave together
     const int32_t p = int32_t(p64);
     const int32_t kb = a.ptr[p], ke = a.ptr_end ? a.ptr_end[p] : a.ptr[p + 1];
     if (ke - kb > a.skip_above) return;  // a long row, done by the chunk kernel
@@ -458,12 +470,29 @@ __global__ __launch_bounds__(256) void k_row_reduce(const RowPass a, const Reduc
         (int64_t(blockIdx.x) * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kGroupsPerWave + lane / LPR;
     if (i64 >= int64_t(r.count)) return;
     const int32_t i = r.first + int32_t(i64);
-    const int32_t p = r.pos[i], s0 = r.slot[i], ns = r.cnt[i];
+    const int32_t p = r.pos ? r.pos[i] : i, s0 = r.slot[i], ns = r.cnt[i];
     const int32_t row = a.node_of ? a.node_of[p] : p;
     for (int32_t c0 = l * VE; c0 < a.ncols; c0 += LPR * VE) {
         float acc[VE];
-        Load<float, VE>::run(r.scratch + int64_t(s0) * a.ncols + c0, acc);
-        for (int32_t s = 1; s < ns; ++s) {
+        if (ns > 0) {
+            Load<float, VE>::run(r.scratch + int64_t(s0) * a.ncols + c0, acc);
+        } else {
+#pragma unroll
+            for (int q = 0; q < VE; ++q) acc[q] = 0.0f;
+        }
+        int32_t s = 1;
+        // Four slot loads in flight, added in slot order (the same bits as one
+        // at a time).
+        for (; s + 4 <= ns; s += 4) {
+            float v[4][VE];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) Load<float, VE>::run(r.scratch + int64_t(s0 + s + u) * a.ncols + c0, v[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int q = 0; q < VE; ++q) acc[q] += v[u][q];
+        }
+        for (; s < ns; ++s) {
             float v[VE];
             Load<float, VE>::run(r.scratch + int64_t(s0 + s) * a.ncols + c0, v);
 #pragma unroll
@@ -610,7 +639,14 @@ bool wave_per_row(const RowPass &rp) {
 template <class P, int VE, int LPR>
 hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
     constexpr int64_t kRowsPerBlock = 4 * (64 / LPR);
-    if (lk.kind == Kind::kPass) {
+    if (lk.kind == Kind::kPass && rp.ranges) {
+        // One grid for the 8 ranges: row block j of range k is workgroup 8 j + k.
+        int64_t most = 0;
+        for (int k = 0; k < 8; ++k) most = std::max<int64_t>(most, int64_t(rp.range_first[k + 1]) - rp.range_first[k]);
+        if (most <= 0) return hipSuccess;
+        k_row_pass<P, VE, LPR, 1>
+            <<<dim3(unsigned((most + kRowsPerBlock - 1) / kRowsPerBlock * 8)), dim3(256), 0, stream>>>(rp);
+    } else if (lk.kind == Kind::kPass) {
         const int64_t n = int64_t(rp.p_end) - rp.p_begin;
         if (n <= 0) return hipSuccess;
         const int64_t ns = rp.xcd_slabs;

@@ -76,6 +76,12 @@ struct RowPass {
     // one place, in the same order: the same bits as without slabs.
     int32_t xcd_slabs = 1;
     int32_t slab_w = 0;
+    // XCD column ranges: when set, the positions are pieces grouped by the
+    // column range they gather from, range k holding [range_first[k],
+    // range_first[k + 1]); workgroup b works on range b % 8 (its XCD), p_begin
+    // and p_end are ignored, and xcd_slabs must be 1.
+    bool ranges = false;
+    int32_t range_first[9] = {};
 };
 
 // Wave slots of the device (256 CUs x 4 SIMDs x 8 waves): a pass with fewer
@@ -97,8 +103,9 @@ struct ChunkArgs {
 };
 hipError_t launch_row_chunks(const RowPass &rp, const ChunkArgs &c, hipStream_t stream);
 
-// Rows [first, first + count) of a reduction list: row pos[i] is the sum of
-// scratch rows slot[i] .. slot[i] + cnt[i] - 1, added in order, then finished.
+// Rows [first, first + count) of a reduction list: row pos[i] (i itself when
+// pos is null) is the sum of scratch rows slot[i] .. slot[i] + cnt[i] - 1,
+// added in order (0 when cnt[i] is 0), then finished.
 struct ReduceArgs {
     const int32_t *pos = nullptr;
     const int32_t *slot = nullptr;

@@ -10,6 +10,9 @@
 //                the same way, for the adjoint's Delta^T product.
 //   children     per level position, the node ids of the children, for the
 //                adjoint's reverse (leaves-to-roots) gather.
+//   pieces       (matrices up to CBM_RANGE_MAX_ROWS rows) each forward and
+//                backward row cut by column range of the gathered matrix, so
+//                that the workgroups of one XCD gather one range only.
 #include "common.hpp"
 #include "../../../include/cbm_host.h"
 
@@ -60,6 +63,116 @@ int32_t split_long_rows(const std::vector<int32_t> &ptr, int32_t b, int32_t n_lo
         }
     }
     return slots;
+}
+
+// Column-range pieces of an ordered row list (see CBM_P_FPC_B in cbm_host.h).
+// Row p consists of one or two parts, [ptr[p], cut[p]) and [cut[p], ptr[p+1])
+// (cut null: one part), each sorted by gathered index (bit 31 is a sign).
+// With two parts the row's entries are rewritten into `order` range by
+// range, part 0's entries of a range before part 1's, so that a range is one
+// contiguous run [b, e) whose part-0 entries end at mid: a dual pass reads
+// [b, mid) from its second source and [mid, e) from its first.
+struct Pieces {
+    std::vector<int32_t> b, e, mid, slot, range_first, rs_slot, rs_cnt, order;
+};
+
+// Returns false (and builds nothing) if the pieces would not fit int32 numbering.
+bool range_pieces(int32_t n, const std::vector<int32_t> &ptr, const std::vector<int32_t> &idx,
+                  const std::vector<int32_t> *cut, Pieces &pc) {
+    constexpr int32_t R = CBM_XCD_RANGES;
+    const int64_t width = (int64_t(n) + R - 1) / R;
+    const int parts = cut ? 2 : 1;
+    // bound[p][q][k]: first entry of part q of row p in range k (k = 0..R).
+    const size_t stride = size_t(parts) * (R + 1);
+    std::vector<int32_t> bound(size_t(n) * stride);
+    std::vector<int32_t> count(size_t(n) * R, 0);  // pieces of row p in range k
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int32_t p = 0; p < n; ++p) {
+        for (int q = 0; q < parts; ++q) {
+            const int32_t lo = q == 0 ? ptr[size_t(p)] : (*cut)[size_t(p)];
+            const int32_t hi = (q == 0 && cut) ? (*cut)[size_t(p)] : ptr[size_t(p) + 1];
+            int32_t *bd = bound.data() + size_t(p) * stride + size_t(q) * (R + 1);
+            int32_t i = lo;
+            for (int32_t k = 0; k <= R; ++k) {
+                const int64_t start = std::min<int64_t>(int64_t(k) * width, int64_t(n));
+                while (i < hi && int64_t(uint32_t(idx[size_t(i)]) & ~kSignBit) < start) ++i;
+                bd[k] = k == R ? hi : i;
+            }
+        }
+        for (int32_t k = 0; k < R; ++k) {
+            int32_t len = 0;
+            for (int q = 0; q < parts; ++q) {
+                const int32_t *bd = bound.data() + size_t(p) * stride + size_t(q) * (R + 1);
+                len += bd[k + 1] - bd[k];
+            }
+            count[size_t(p) * R + size_t(k)] = (len + CBM_PIECE - 1) / CBM_PIECE;
+        }
+    }
+    int64_t all = 0;
+    for (const int32_t c : count) all += c;
+    if (all > INT32_MAX) return false;
+    // Slots: a row's pieces are consecutive, in (range, part, piece) order.
+    pc.rs_slot.assign(size_t(n), 0);
+    pc.rs_cnt.assign(size_t(n), 0);
+    int64_t total = 0;
+    for (int32_t p = 0; p < n; ++p) {
+        int32_t c = 0;
+        for (int32_t k = 0; k < R; ++k) c += count[size_t(p) * R + size_t(k)];
+        pc.rs_slot[size_t(p)] = int32_t(total);
+        pc.rs_cnt[size_t(p)] = c;
+        total += c;
+    }
+    // Piece order: grouped by range, rows ascending inside a range.
+    pc.range_first.assign(size_t(R) + 1, 0);
+    std::vector<int64_t> at(size_t(n) * R);
+    int64_t next = 0;
+    for (int32_t k = 0; k < R; ++k) {
+        pc.range_first[size_t(k)] = int32_t(next);
+        for (int32_t p = 0; p < n; ++p) {
+            at[size_t(p) * R + size_t(k)] = next;
+            next += count[size_t(p) * R + size_t(k)];
+        }
+    }
+    pc.range_first[size_t(R)] = int32_t(next);
+    pc.b.resize(size_t(total));
+    pc.e.resize(size_t(total));
+    pc.slot.resize(size_t(total));
+    if (cut) {
+        pc.mid.resize(size_t(total));
+        pc.order.resize(idx.size());
+    }
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int32_t p = 0; p < n; ++p) {
+        int32_t s = pc.rs_slot[size_t(p)];
+        int32_t w = ptr[size_t(p)];  // next entry of the rewritten row (two parts)
+        for (int32_t k = 0; k < R; ++k) {
+            int64_t o = at[size_t(p) * R + size_t(k)];
+            // The range's run [rb, re) and the end of its part-0 entries.
+            int32_t rb, re, mid;
+            if (cut) {
+                const int32_t *b0 = bound.data() + size_t(p) * stride;
+                const int32_t *b1 = b0 + (R + 1);
+                rb = w;
+                for (int32_t i = b0[k]; i < b0[k + 1]; ++i) pc.order[size_t(w++)] = idx[size_t(i)];
+                mid = w;
+                for (int32_t i = b1[k]; i < b1[k + 1]; ++i) pc.order[size_t(w++)] = idx[size_t(i)];
+                re = w;
+            } else {
+                const int32_t *bd = bound.data() + size_t(p) * stride;
+                rb = bd[k];
+                re = bd[k + 1];
+                mid = re;
+            }
+            for (int32_t c0 = rb; c0 < re; c0 += CBM_PIECE, ++o, ++s) {
+                const int32_t c1 = std::min(re, c0 + CBM_PIECE);
+                pc.b[size_t(o)] = c0;
+                pc.e[size_t(o)] = c1;
+                pc.slot[size_t(o)] = s;
+                if (cut) pc.mid[size_t(o)] = std::min(std::max(mid, c0), c1);
+            }
+        }
+    }
+    return true;
 }
 
 }  // namespace
@@ -243,6 +356,18 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
             plan->arr[CBM_P_FRED_CNT].swap(f.red_cnt);
             plan->stat[CBM_PS_FWD_SLOTS] = max_slots;
         }
+        const bool ranges = n <= CBM_RANGE_MAX_ROWS;
+        if (ranges) {
+            Pieces pc;
+            if (!range_pieces(n, fptr, fcol, nullptr, pc)) pc = Pieces();
+            plan->stat[CBM_PS_FWD_PIECES] = int64_t(pc.b.size());
+            plan->arr[CBM_P_FPC_B].swap(pc.b);
+            plan->arr[CBM_P_FPC_E].swap(pc.e);
+            plan->arr[CBM_P_FPC_SLOT].swap(pc.slot);
+            plan->arr[CBM_P_FPC_RANGE].swap(pc.range_first);
+            plan->arr[CBM_P_FRS_SLOT].swap(pc.rs_slot);
+            plan->arr[CBM_P_FRS_CNT].swap(pc.rs_cnt);
+        }
 
         // Backward: Delta^T (the CSC of Delta), columns ordered long first,
         // then short, ascending within each part; rows ascending per column.
@@ -291,6 +416,19 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
             plan->arr[CBM_P_BRED_POS].swap(b.red_pos);
             plan->arr[CBM_P_BRED_SLOT].swap(b.red_slot);
             plan->arr[CBM_P_BRED_CNT].swap(b.red_cnt);
+            if (ranges && plan->stat[CBM_PS_FWD_PIECES] > 0) {
+                Pieces pc;
+                if (!range_pieces(n, bptr, brow, &bmid, pc)) pc = Pieces();
+                plan->stat[CBM_PS_BWD_PIECES] = int64_t(pc.b.size());
+                plan->arr[CBM_P_BPC_B].swap(pc.b);
+                plan->arr[CBM_P_BPC_E].swap(pc.e);
+                plan->arr[CBM_P_BPC_MID].swap(pc.mid);
+                plan->arr[CBM_P_BPC_SLOT].swap(pc.slot);
+                plan->arr[CBM_P_BPC_RANGE].swap(pc.range_first);
+                plan->arr[CBM_P_BRS_SLOT].swap(pc.rs_slot);
+                plan->arr[CBM_P_BRS_CNT].swap(pc.rs_cnt);
+                plan->arr[CBM_P_BPC_ROW].swap(pc.order);
+            }
         }
 
         // Internal nodes (those with children) by level, and their children;

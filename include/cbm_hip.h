@@ -123,6 +123,26 @@ typedef struct cbm_desc {
     const int32_t *rred_slot;
     const int32_t *rred_cnt;
     int32_t rev_slots;
+    /* column-range pieces (plan arrays CBM_P_FPC_* .. CBM_P_BRS_*, stats
+       FWD_PIECES / BWD_PIECES; counts 0 and NULL pointers when the plan has
+       none): piece offsets per range on the host [CBM_XCD_RANGES + 1], the
+       piece tables and per-row slot tables on the device */
+    const int32_t *fpc_range_host;
+    const int32_t *fpc_b;
+    const int32_t *fpc_e;
+    const int32_t *fpc_slot;
+    const int32_t *frs_slot;
+    const int32_t *frs_cnt;
+    int32_t fwd_pieces;
+    const int32_t *bpc_range_host;
+    const int32_t *bpc_b;
+    const int32_t *bpc_e;
+    const int32_t *bpc_mid;
+    const int32_t *bpc_slot;
+    const int32_t *brs_slot;
+    const int32_t *brs_cnt;
+    int32_t bwd_pieces;
+    const int32_t *bpc_row;
 } cbm_desc;
 
 typedef struct cbm_handle cbm_handle;
@@ -250,6 +270,22 @@ int cbm_set_wave_rows(cbm_handle *h, int32_t mode);
  * order: results are identical to slabs = 1 under the packed row mapping.
  */
 int cbm_set_xcd_slabs(cbm_handle *h, int32_t slabs);
+
+/*
+ * XCD column ranges (-1 automatic, the default; 0 off; 1 on when the plan has
+ * range pieces).  The gathered matrix's rows are cut into 8 ranges and the
+ * workgroups of XCD k (workgroup b runs on XCD b mod 8) gather range k only,
+ * so each XCD's 4 MB L2 serves an eighth of the matrix.  The forward Delta
+ * pass and the adjoint Delta^T pass then write one partial row per piece
+ * (a row's entries in one range, at most CBM_PIECE of them) and a reduction
+ * adds each row's pieces in order; the forward's level propagation follows
+ * as usual.  Results are deterministic but rounded in another order than
+ * with ranges off.  Automatic: on when the plan has pieces, Delta holds 64
+ * entries per row or more on average, and N x F x 4 bytes is between 16 and
+ * 256 MiB.  Needs F x 4 bytes of scratch per piece; cbm_reserve sizes it for
+ * the current choice.
+ */
+int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode);
 
 /* ------------------------------------------------------------------------- */
 /* Candidate edges of the structure build on the GPU.                         */

@@ -220,7 +220,26 @@ enum {
     CBM_P_RRED_POS = 38,   /* int32 [Rr]   their internal-node positions                                 */
     CBM_P_RRED_SLOT = 39,  /* int32 [Rr]   first slot                                                    */
     CBM_P_RRED_CNT = 40,   /* int32 [Rr]   number of slots                                               */
-    CBM_P_N_ARRAYS = 41
+    /* Column-range pieces (built only for n_rows <= CBM_RANGE_MAX_ROWS, empty otherwise).  The
+       gathered index space is cut into CBM_XCD_RANGES equal ranges; the entries of a row that fall
+       in one range are cut into pieces of at most CBM_PIECE entries.  Pieces are grouped by range; each writes one partial row (slot),
+       and the slots of a row are consecutive, in (range, part, piece) order. */
+    CBM_P_FPC_B = 41,      /* int32 [Pf]   forward pieces: first entry (into FWD_COL)                    */
+    CBM_P_FPC_E = 42,      /* int32 [Pf]   end entry                                                     */
+    CBM_P_FPC_SLOT = 43,   /* int32 [Pf]   partial slot                                                  */
+    CBM_P_FPC_RANGE = 44,  /* int32 [R+1]  piece offsets per range                                       */
+    CBM_P_FRS_SLOT = 45,   /* int32 [N]    first slot of each level position                             */
+    CBM_P_FRS_CNT = 46,    /* int32 [N]    number of slots (pieces) of each level position               */
+    CBM_P_BPC_B = 47,      /* int32 [Pb]   backward pieces: first entry (into BPC_ROW)                   */
+    CBM_P_BPC_E = 48,      /* int32 [Pb]   end entry                                                     */
+    CBM_P_BPC_MID = 49,    /* int32 [Pb]   first entry of an internal row (leaf rows come first)         */
+    CBM_P_BPC_SLOT = 50,   /* int32 [Pb]   partial slot                                                  */
+    CBM_P_BPC_RANGE = 51,  /* int32 [R+1]  piece offsets per range                                       */
+    CBM_P_BRS_SLOT = 52,   /* int32 [N]    first slot of each backward position                          */
+    CBM_P_BRS_CNT = 53,    /* int32 [N]    number of slots of each backward position                     */
+    CBM_P_BPC_ROW = 54,    /* int32 [nnz]  BWD_ROW's entries rewritten range by range (a row's extent is
+                                           unchanged): in each range the leaf rows, then internal rows  */
+    CBM_P_N_ARRAYS = 55
 };
 
 /* Scalars of a cbm_plan, addressed by cbm_plan_stat(). */
@@ -231,11 +250,16 @@ enum {
     CBM_PS_BWD_NLONG = 3,  /* long rows at the start of the backward order           */
     CBM_PS_FWD_GSLOTS = 4, /* partial slots of all forward levels together (FCK_GSLOT) */
     CBM_PS_REV_SLOTS = 5,  /* partial slots needed by the largest reverse-pass level    */
-    CBM_PS_N_STATS = 6
+    CBM_PS_FWD_PIECES = 6, /* forward column-range pieces (0: none built)               */
+    CBM_PS_BWD_PIECES = 7, /* backward column-range pieces                              */
+    CBM_PS_N_STATS = 8
 };
 
 #define CBM_LONG_ROW 256
 #define CBM_CHUNK 1024
+#define CBM_XCD_RANGES 8
+#define CBM_PIECE 256
+#define CBM_RANGE_MAX_ROWS 524288
 
 typedef struct cbm_plan cbm_plan;
 

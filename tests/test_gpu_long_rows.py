@@ -86,7 +86,9 @@ def test_split_and_fused_forward_are_bit_identical(hub, cuda, mode, dtype, f):
     a, mats = hub
     h = mats[mode].cusparse_extension
     xt = torch.from_numpy(_x((HUB_N, f), 11 + f)).to(cuda).to(dtype)
-    for wave, depth in ((False, None), (False, 10), (True, None), (True, 10), (None, 1), (None, 0)):
+    for ranges, wave, depth in ((False, False, None), (False, False, 10), (False, True, None), (False, True, 10),
+                                (False, None, 1), (False, None, 0), (True, None, None), (True, None, 1)):
+        h.set_xcd_ranges(ranges)
         h.set_wave_rows(wave)
         h.set_kahan_depth(depth)
         ys = []
@@ -104,6 +106,7 @@ def test_split_and_fused_forward_are_bit_identical(hub, cuda, mode, dtype, f):
         assert O.max_rel_error(ys[1].cpu().numpy(), O.forward(a, xt.float().cpu().numpy(), mode)) <= 1e-5
     h.set_kahan_depth(32)
     h.set_wave_rows(None)
+    h.set_xcd_ranges(None)
 
 
 def test_compensation_from_level_zero_equals_level_one(hub, cuda):
@@ -228,3 +231,67 @@ def test_xcd_slabs_give_the_same_bits(hub, cuda, mode, dtype, f):
     xe = xt.float().cpu().numpy()
     assert O.max_rel_error(res[(8, True)].cpu().numpy(), O.forward(a, xe, mode)) <= 1e-5
     assert O.max_rel_error(res[(8, "t")].cpu().numpy(), O.adjoint(a, xe, mode)) <= 1e-5
+
+
+@pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
+@pytest.mark.parametrize("dtype,f", [(torch.float32, 1), (torch.float32, 8), (torch.float32, 64),
+                                     (torch.float32, 300), (torch.bfloat16, 64), (torch.bfloat16, 256)])
+def test_xcd_ranges_against_the_oracle(hub, star, cuda, mode, dtype, f):
+    """Gathers by XCD column range (one partial row per range piece, reduced
+    in order, fused with the level propagation): forward and adjoint against
+    FP64 on the hub graph (hub rows of several pieces per range, deep tree)
+    and the star graph (a 2500-child list); repeat calls give the same bits;
+    with ranges off the results are within rounding of them."""
+    for a, n, m in ((hub[0], HUB_N, hub[1][mode]), (star[0], star[1], star[2][mode])):
+        h = m.cusparse_extension
+        assert h.plan_stats["fwd_pieces"] > 0 and h.plan_stats["bwd_pieces"] > 0
+        xt = torch.from_numpy(_x((n, f), 90 + f)).to(cuda).to(dtype)
+        xe = xt.float().cpu().numpy()
+        out = {}
+        for ranges in (True, False, True):
+            h.set_xcd_ranges(ranges)
+            y = torch.full((n, f), float("nan"), device=cuda)
+            dx = torch.full((n, f), float("nan"), device=cuda)
+            h.forward(xt, y)
+            h.matmul_t(xt, dx)
+            if ranges in out:
+                assert torch.equal(y, out[ranges][0]) and torch.equal(dx, out[ranges][1])
+            out[ranges] = (y, dx)
+        h.set_xcd_ranges(None)
+        y, dx = out[True]
+        assert O.max_rel_error(y.cpu().numpy(), O.forward(a, xe, mode)) <= 1e-5, (n, mode, f)
+        assert O.max_rel_error(dx.cpu().numpy(), O.adjoint(a, xe, mode)) <= 1e-5, (n, mode, f)
+        assert O.max_rel_error(y.cpu().numpy(), out[False][0].cpu().numpy()) <= 1e-5
+
+
+def test_xcd_ranges_integer_exact_and_halves(hub, cuda):
+    """Integer-valued inputs are exact by range too; delta_spmm + propagate
+    give the product's bits, plain and compensated."""
+    a, mats = hub
+    h = mats["ax"].cusparse_extension
+    h.set_xcd_ranges(True)
+    x = np.random.default_rng(12).integers(-3, 4, size=(HUB_N, 24)).astype(np.float32)
+    xt = torch.from_numpy(x).to(cuda)
+    y = torch.empty(HUB_N, 24, device=cuda)
+    dx = torch.empty(HUB_N, 24, device=cuda)
+    h.forward(xt, y)
+    h.matmul_t(xt, dx)
+    assert np.array_equal(y.cpu().numpy().astype(np.float64), O.forward(a, x, "ax"))
+    assert np.array_equal(dx.cpu().numpy().astype(np.float64), O.adjoint(a, x, "ax"))
+    for depth in (None, 2):
+        h.set_kahan_depth(depth)
+        for name, m in mats.items():
+            hm = m.cusparse_extension
+            hm.set_xcd_ranges(True)
+            hm.set_kahan_depth(depth)
+            xf = torch.from_numpy(_x((HUB_N, 40), 7)).to(cuda)
+            y1 = torch.empty(HUB_N, 40, device=cuda)
+            y2 = torch.empty(HUB_N, 40, device=cuda)
+            hm.forward(xf, y1)
+            hm.delta_spmm(xf, y2)
+            hm.propagate(y2)
+            assert torch.equal(y1, y2), (name, depth)
+            hm.set_kahan_depth(32)
+            hm.set_xcd_ranges(None)
+    h.set_kahan_depth(32)
+    h.set_xcd_ranges(None)

@@ -277,11 +277,15 @@ def test_reference_style_construction(graph1, cuda):
         h.matmul_AX_ADX(torch.from_numpy(x).to(cuda), y)
 
 
-def test_graph_capture_replays(mats, cuda):
+@pytest.mark.parametrize("ranges", [None, True])
+def test_graph_capture_replays(mats, cuda, ranges):
     """Forward + adjoint captured in a HIP graph replay correctly several times,
-    with eager work and synchronisations between the replays."""
+    with eager work and synchronisations between the replays (with the XCD
+    column ranges as chosen by size, and forced on)."""
     m, _ = mats["dadx"]
     h = m.cusparse_extension
+    h.set_xcd_ranges(ranges)
+    h.reserve(64)
     x = torch.from_numpy(_rand((N1, 64), 31)).to(cuda)
     y = torch.empty(N1, 64, device=cuda)
     dx = torch.empty(N1, 64, device=cuda)
@@ -308,3 +312,4 @@ def test_graph_capture_replays(mats, cuda):
         graph.replay()
         torch.cuda.synchronize()
         assert torch.equal(y, y_ref) and torch.equal(dx, dx_ref)
+    h.set_xcd_ranges(None)

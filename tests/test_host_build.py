@@ -9,7 +9,7 @@ import scipy.sparse as sp
 import torch
 
 from cbm import cbm_mkl_cpp, structure
-from hubgraph import HUB_N, hub_graph
+from hubgraph import HUB_N, hub_graph, star_graph
 from oracle import oracle as O
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "cbm_small.json")
@@ -327,3 +327,68 @@ def test_reverse_chunk_tables_on_a_star():
                      pl.rred_cnt.numpy()[r0:r1], st["rev_slots"])
     assert rn.sum() >= 1 and rred[-1] >= 1
 
+
+
+def _check_pieces(n, ptr, idx, cut, b, e, mid, slot, rng, rs_slot, rs_cnt, order=None, internal=None):
+    """Pieces tile every row exactly, in slot order, each inside one column
+    range and within the piece size; grouped by range.  Backward (cut given):
+    the pieces index `order`, each row's entries rewritten range by range,
+    with a piece's leaf rows before mid and its internal rows from mid."""
+    width = (n + structure.XCD_RANGES - 1) // structure.XCD_RANGES
+    n_pc = len(b)
+    assert len(rng) == structure.XCD_RANGES + 1 and rng[0] == 0 and rng[-1] == n_pc
+    assert np.all(np.diff(rng) >= 0)
+    assert sorted(slot.tolist()) == list(range(n_pc))
+    piece_of_slot = np.empty(n_pc, np.int64)
+    piece_of_slot[slot] = np.arange(n_pc)
+    range_of_piece = np.searchsorted(rng, np.arange(n_pc), side="right") - 1
+    if cut is not None:
+        for p in range(n):  # the rewritten row holds the same entries
+            assert sorted(order[ptr[p]:ptr[p + 1]].tolist()) == sorted(idx[ptr[p]:ptr[p + 1]].tolist())
+        idx = order
+    col = (idx.view(np.uint32) & 0x7FFFFFFF).astype(np.int64)
+    assert np.all(e - b >= 1) and np.all(e - b <= structure.PIECE)
+    for p in range(n):
+        pcs = piece_of_slot[rs_slot[p]:rs_slot[p] + rs_cnt[p]]
+        if rs_cnt[p] == 0:
+            assert ptr[p] == ptr[p + 1]
+            continue
+        covered = np.concatenate([np.arange(b[i], e[i]) for i in pcs])
+        assert sorted(covered.tolist()) == list(range(ptr[p], ptr[p + 1]))
+        ranges = [range_of_piece[i] for i in pcs]
+        assert ranges == sorted(ranges)  # (range, part, piece) order
+        for i in pcs:
+            k = col[b[i]:e[i]] // width
+            assert np.all(k == range_of_piece[i])
+            if cut is not None:
+                assert b[i] <= mid[i] <= e[i]
+                assert not internal[col[b[i]:mid[i]]].any() and internal[col[mid[i]:e[i]]].all()
+
+
+def test_range_pieces_tile_the_rows():
+    def check(ei, n):
+        b, pl = _plan_for(ei.numpy(), n, 0)
+        assert pl.stats["fwd_pieces"] == len(pl.fpc_b) > 0 and pl.stats["bwd_pieces"] == len(pl.bpc_b) > 0
+        par = b.parent.numpy()
+        internal = np.bincount(par[par >= 0], minlength=n) > 0
+        _check_pieces(n, pl.fwd_ptr.numpy(), pl.fwd_col.numpy(), None, pl.fpc_b.numpy(), pl.fpc_e.numpy(), None,
+                      pl.fpc_slot.numpy(), pl.fpc_range.numpy(), pl.frs_slot.numpy(), pl.frs_cnt.numpy())
+        _check_pieces(n, pl.bwd_ptr.numpy(), pl.bwd_row.numpy(), pl.bwd_mid.numpy(), pl.bpc_b.numpy(),
+                      pl.bpc_e.numpy(), pl.bpc_mid.numpy(), pl.bpc_slot.numpy(), pl.bpc_range.numpy(),
+                      pl.brs_slot.numpy(), pl.brs_cnt.numpy(), pl.bpc_row.numpy(), internal)
+        return pl
+
+    check(hub_graph(1500), 1500)
+    # the star graph's core rows hold more than one piece per range
+    ei, n = star_graph()
+    pl = check(ei, n)
+    assert int((pl.fpc_e - pl.fpc_b).max()) == structure.PIECE or int((pl.bpc_e - pl.bpc_b).max()) == structure.PIECE
+
+
+def test_no_range_pieces_above_the_size_limit():
+    n = structure.RANGE_MAX_ROWS + 1
+    ei = torch.tensor([[0, 1, n - 1, 0], [1, 0, 0, n - 1]], dtype=torch.int32)
+    b = _build(ei.numpy(), n, 0)
+    pl = structure.plan(n, b.csr_ptr, b.csr_col, b.csr_val, b.tree_src, b.tree_dst)
+    assert pl.stats["fwd_pieces"] == 0 and pl.stats["bwd_pieces"] == 0
+    assert pl.fpc_b.numel() == 0 and pl.fpc_range.numel() == 0
