@@ -605,6 +605,9 @@ int adjoint_reverse(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, fl
     for (int32_t l = h->d.n_levels - 1; l >= 0; --l) {
         const int32_t b = h->rev_ptr[size_t(l)], e = h->rev_ptr[size_t(l) + 1];
         if (e <= b) continue;
+        // A level's launch holds only its own nodes: a thin level gives each
+        // node a whole wavefront (its child list in sub-group slices).
+        rv.rows_hint = e - b;
         ck.first = h->rck_ptr[size_t(l)];
         ck.count = h->rck_ptr[size_t(l) + 1] - ck.first;
         rd.first = h->rred_ptr[size_t(l)];
