@@ -313,7 +313,8 @@ def measure(name, w, ei, m, args, dev, stream):
         "max_rel_error_bf16_vs_fp32_cbm": vs_fp32,
         "ms_xcd_slabs_bf16": xcd,
         "ms_xcd_ranges": xr,
-        "xcd_ranges_default": bool(xr) and (n * f * 4 <= (256 << 20)),
+        "xcd_ranges_default": {"bf16": h.xcd_ranges_active(torch.bfloat16, f),
+                               "fp32": h.xcd_ranges_active(torch.float32, f)},
     }
     return rec, out, (x32, g32, x16, g16), m_csr
 

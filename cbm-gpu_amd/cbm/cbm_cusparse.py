@@ -315,6 +315,15 @@ class cusparseCBM:
         code = -1 if mode is None else int(bool(mode))
         _lib.check_hip(self._lib.cbm_set_xcd_ranges(self._h, code), "set_xcd_ranges")
 
+    def xcd_ranges_active(self, dtype, n_cols):
+        """Whether a call with dtype (torch.float32 or torch.bfloat16) storage
+        and n_cols columns runs its gathers by XCD column range now."""
+        if dtype not in (torch.float32, torch.bfloat16):
+            raise TypeError(f"dtype must be float32 or bfloat16, got {dtype}")
+        code = self._lib.cbm_xcd_ranges_active(self._h, BF16 if dtype == torch.bfloat16 else F32, int(n_cols))
+        _lib.check_hip(min(code, 0), "xcd_ranges_active")
+        return bool(code)
+
     def set_xcd_slabs(self, slabs):
         """Feature slabs spread over the XCDs (1 = off); same results."""
         _lib.check_hip(self._lib.cbm_set_xcd_slabs(self._h, int(slabs)), "set_xcd_slabs")

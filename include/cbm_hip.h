@@ -281,11 +281,20 @@ int cbm_set_xcd_slabs(cbm_handle *h, int32_t slabs);
  * adds each row's pieces in order; the forward's level propagation follows
  * as usual.  Results are deterministic but rounded in another order than
  * with ranges off.  Automatic: on when the plan has pieces, Delta holds 64
- * entries per row or more on average, and N x F x 4 bytes is between 16 and
- * 256 MiB.  Needs F x 4 bytes of scratch per piece; cbm_reserve sizes it for
- * the current choice.
+ * entries per row or more on average, and the gathered matrix (N x F entries
+ * of the call's storage type: 2 bytes for BF16, 4 for FP32) is between 24 and
+ * 256 MiB, so a BF16 and an FP32 call of one width may choose differently.
+ * Needs F x 4 bytes of scratch per piece; cbm_reserve sizes it for the current
+ * choice of either storage type.
  */
 int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode);
+
+/*
+ * 1 when a dtype call of n_cols columns runs its gather passes by XCD column
+ * range under the handle's current setting, 0 when not, CBM_ERR_ARG for
+ * invalid arguments.  Reports the automatic choice above; computes nothing.
+ */
+int cbm_xcd_ranges_active(const cbm_handle *h, int32_t dtype, int32_t n_cols);
 
 /* ------------------------------------------------------------------------- */
 /* Candidate edges of the structure build on the GPU.                         */

@@ -295,3 +295,26 @@ def test_xcd_ranges_integer_exact_and_halves(hub, cuda):
             hm.set_xcd_ranges(None)
     h.set_kahan_depth(32)
     h.set_xcd_ranges(None)
+
+
+def test_xcd_ranges_active_reports_the_choice(hub, cuda):
+    """The query follows the setting: forced on and off for both storage
+    types; automatic off at this size (a gathered matrix under 24 MiB), and
+    the automatic choice is the one the forward takes (its launch count)."""
+    _, mats = hub
+    h = mats["ax"].cusparse_extension
+    for mode, want in ((True, True), (False, False), (None, False)):
+        h.set_xcd_ranges(mode)
+        for dtype in (torch.float32, torch.bfloat16):
+            assert h.xcd_ranges_active(dtype, 64) is want, (mode, dtype)
+    x = torch.randn(HUB_N, 64, device=cuda)
+    y = torch.empty(HUB_N, 64, device=cuda)
+    launches = {}
+    for mode in (True, False, None):
+        h.set_xcd_ranges(mode)
+        h.forward(x, y)
+        launches[mode] = h.last_launches()
+    assert launches[None] == launches[False] != launches[True]
+    with pytest.raises(Exception):
+        h.xcd_ranges_active(torch.float16, 64)
+    h.set_xcd_ranges(None)
