@@ -227,12 +227,12 @@ def measure(name, w, ei, m, args, dev, stream):
                       "adjoint_delta_t": round(time_loop(lambda: h.adjoint_delta_t(g16, z, dx), K, W, stream), 4)}
     h.set_xcd_slabs(1)
     log(f"{name}: XCD slabs {xcd}")
-    # gathers by XCD column range (different rounding order), off and on
+    # gathers by XCD column range (different rounding order): off, and each piece set
     xr = {}
     if h.plan_stats.get("fwd_pieces", 0) > 0:
-        for on in (False, True):
+        for on in (False, 8, 4):
             h.set_xcd_ranges(on)
-            xr["on" if on else "off"] = {
+            xr[f"{on}_ranges" if on else "off"] = {
                 "step_bf16": round(time_loop(step(x16, g16), K, W, stream), 4),
                 "step_fp32": round(time_loop(step(x32, g32), K, W, stream), 4),
                 "forward": round(time_loop(lambda: h.forward(x16, y), K, W, stream), 4),

@@ -82,6 +82,22 @@ class CbmDesc(ctypes.Structure):
         ("brs_cnt", c_vp),
         ("bwd_pieces", c_i32),
         ("bpc_row", c_vp),
+        ("fpc4_range_host", c_vp),
+        ("fpc4_b", c_vp),
+        ("fpc4_e", c_vp),
+        ("fpc4_slot", c_vp),
+        ("frs4_slot", c_vp),
+        ("frs4_cnt", c_vp),
+        ("fwd_pieces4", c_i32),
+        ("bpc4_range_host", c_vp),
+        ("bpc4_b", c_vp),
+        ("bpc4_e", c_vp),
+        ("bpc4_mid", c_vp),
+        ("bpc4_slot", c_vp),
+        ("brs4_slot", c_vp),
+        ("brs4_cnt", c_vp),
+        ("bwd_pieces4", c_i32),
+        ("bpc4_row", c_vp),
     ]
 
 # Device arrays of the plan that the descriptor points at.
@@ -89,9 +105,10 @@ DEVICE_ARRAYS = ("lvl_node", "lvl_parent", "lvl_ppos", "fwd_ptr", "fwd_col", "re
                  "fck_slot", "fred_pos", "fred_slot", "fred_cnt", "bwd_node", "bwd_ptr", "bwd_row", "bwd_mid", "bck_pos",
                  "bck_meta", "bck_slot", "bred_pos", "bred_slot", "bred_cnt", "fck_gslot", "fred_gslot", "rck_pos", "rck_meta", "rck_slot", "rred_pos",
                  "rred_slot", "rred_cnt", "fpc_b", "fpc_e", "fpc_slot", "frs_slot", "frs_cnt", "bpc_b", "bpc_e", "bpc_mid",
-                 "bpc_slot", "brs_slot", "brs_cnt", "bpc_row")
+                 "bpc_slot", "brs_slot", "brs_cnt", "bpc_row", "fpc4_b", "fpc4_e", "fpc4_slot", "frs4_slot", "frs4_cnt",
+                 "bpc4_b", "bpc4_e", "bpc4_mid", "bpc4_slot", "brs4_slot", "brs4_cnt", "bpc4_row")
 HOST_ARRAYS = ("lvl_ptr", "lvl_nlong", "fck_ptr", "fred_ptr", "rev_ptr", "rev_nlong", "rck_ptr", "rred_ptr", "fpc_range",
-               "bpc_range")
+               "bpc_range", "fpc4_range", "bpc4_range")
 
 
 def declare(lib, sig):

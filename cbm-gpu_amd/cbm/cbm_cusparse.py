@@ -130,6 +130,7 @@ class cusparseCBM:
                        n_bck=int(self._dev["bck_pos"].numel()), n_bred=int(self._dev["bred_pos"].numel()),
                        bwd_slots=st["bwd_slots"], fwd_gslots=st["fwd_gslots"], rev_slots=st["rev_slots"],
                        fwd_pieces=st.get("fwd_pieces", 0), bwd_pieces=st.get("bwd_pieces", 0),
+                       fwd_pieces4=st.get("fwd_pieces4", 0), bwd_pieces4=st.get("bwd_pieces4", 0),
                        d=self._d.data_ptr() if self._d is not None else None, **ptrs)
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
@@ -311,18 +312,27 @@ class cusparseCBM:
 
     def set_xcd_ranges(self, mode):
         """XCD column ranges for the gather passes: None automatic (by size),
-        False off, True on (when the layout has range pieces)."""
-        code = -1 if mode is None else int(bool(mode))
+        False off, True on (when the layout has range pieces; the piece set
+        by row width), 8 or 4 on with that many ranges."""
+        if mode is None:
+            code = -1
+        elif isinstance(mode, bool):
+            code = int(mode)
+        elif mode in (4, 8):
+            code = int(mode)
+        else:
+            raise ValueError(f"set_xcd_ranges: None, a bool, 4 or 8, got {mode!r}")
         _lib.check_hip(self._lib.cbm_set_xcd_ranges(self._h, code), "set_xcd_ranges")
 
     def xcd_ranges_active(self, dtype, n_cols):
-        """Whether a call with dtype (torch.float32 or torch.bfloat16) storage
-        and n_cols columns runs its gathers by XCD column range now."""
+        """The number of XCD column ranges (8 or 4) a call with dtype
+        (torch.float32 or torch.bfloat16) storage and n_cols columns runs its
+        gathers with now, 0 for none."""
         if dtype not in (torch.float32, torch.bfloat16):
             raise TypeError(f"dtype must be float32 or bfloat16, got {dtype}")
         code = self._lib.cbm_xcd_ranges_active(self._h, BF16 if dtype == torch.bfloat16 else F32, int(n_cols))
         _lib.check_hip(min(code, 0), "xcd_ranges_active")
-        return bool(code)
+        return int(code)
 
     def set_xcd_slabs(self, slabs):
         """Feature slabs spread over the XCDs (1 = off); same results."""

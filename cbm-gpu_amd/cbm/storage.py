@@ -5,7 +5,8 @@ a file here holds everything a ``cbm4*`` object needs, so loading skips the
 candidate scan, the arborescence and the device layout.
 
 Format: one uncompressed ``.npz`` (numpy's zip of ``.npy`` arrays, read with
-``allow_pickle=False``), version 1:
+``allow_pickle=False``), version 2 (version 1 lacked the layout's 4-range
+piece set; such files still load, without it):
 
   meta               uint8   JSON text: format, version, n, alpha, the class
                              that saved it, build and layout statistics, the
@@ -32,7 +33,11 @@ import torch
 from . import structure
 
 FORMAT = "cbm-structure"
-VERSION = 1
+VERSION = 2
+# Layout arrays (and statistics) that version 1 files do not have.
+_NEW_IN_2 = ("fpc4_b", "fpc4_e", "fpc4_slot", "fpc4_range", "frs4_slot", "frs4_cnt", "bpc4_b", "bpc4_e", "bpc4_mid",
+             "bpc4_slot", "bpc4_range", "brs4_slot", "brs4_cnt", "bpc4_row")
+_NEW_STATS_IN_2 = ("fwd_pieces4", "bwd_pieces4")
 _CORE = ("delta_crow", "delta_col", "delta_sign", "mca_offset", "mca_src_idx", "mca_dst_idx")
 _DTYPES = {"delta_crow": np.int32, "delta_col": np.int32, "delta_sign": np.int8, "mca_offset": np.int32,
            "mca_src_idx": np.int32, "mca_dst_idx": np.int32}
@@ -108,10 +113,12 @@ def load(path):
             raise StructureFileError(f"{path}: unreadable meta record") from exc
         if meta.get("format") != FORMAT:
             raise StructureFileError(f"{path}: format {meta.get('format')!r}, expected {FORMAT!r}")
-        if meta.get("version") != VERSION:
-            raise StructureFileError(f"{path}: version {meta.get('version')}, this library reads {VERSION}")
-        names = list(_CORE) + ["plan/" + k for k in structure.PLAN_NAMES]
-        arrays = {}
+        version = meta.get("version")
+        if version not in (1, VERSION):
+            raise StructureFileError(f"{path}: version {version}, this library reads 1 and {VERSION}")
+        absent = set(_NEW_IN_2) if version == 1 else set()
+        names = list(_CORE) + ["plan/" + k for k in structure.PLAN_NAMES if k not in absent]
+        arrays = {"plan/" + k: np.zeros(0, np.int32) for k in absent}
         for name in names:
             if name not in z.files:
                 raise StructureFileError(f"{path}: array {name} is missing")
@@ -131,8 +138,11 @@ def load(path):
             or arrays["plan/bwd_row"].shape[0] != nnz or arrays["plan/ch_node"].shape[0] != e):
         raise StructureFileError(f"{path}: array sizes disagree with n = {n}")
     t = {k: torch.from_numpy(v.copy()) for k, v in arrays.items()}
+    stats = dict(meta["plan_stats"])
+    for k in _NEW_STATS_IN_2:
+        stats.setdefault(k, 0)
     plan = structure.Plan(n=n, depth=int(meta["plan_depth"]),
-                          arrays={k: t["plan/" + k] for k in structure.PLAN_NAMES}, stats=dict(meta["plan_stats"]))
+                          arrays={k: t["plan/" + k] for k in structure.PLAN_NAMES}, stats=stats)
     return Stored(n=n, alpha=int(meta["alpha"]), saved_by=meta.get("saved_by", ""), delta_crow=t["delta_crow"],
                   delta_col=t["delta_col"], delta_sign=t["delta_sign"], mca_offset=t["mca_offset"],
                   mca_src_idx=t["mca_src_idx"], mca_dst_idx=t["mca_dst_idx"], plan=plan,

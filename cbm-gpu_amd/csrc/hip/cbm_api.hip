@@ -26,7 +26,17 @@ struct cbm_handle {
     int32_t wave_rows = -1;   // short-row passes: whole wavefront per row (1), never (0), by size (-1)
     int32_t xcd_slabs = 1;    // feature slabs spread over the XCDs (1 = off)
     int32_t xcd_ranges = -1;  // column ranges per XCD: 1 on, 0 off, -1 by size
-    int32_t fpc_range[9] = {}, bpc_range[9] = {};  // host copies of the piece offsets per range
+    // Column-range piece sets: [0] 8 ranges (one per XCD), [1] 4 ranges (each
+    // on two XCDs, one column half each).
+    struct RangeSet {
+        int32_t n_ranges = 0;   // 0: the plan has no such pieces
+        int32_t fwd = 0, bwd = 0;
+        int32_t f_first[9] = {}, b_first[9] = {};  // host copies of the piece offsets per range
+        const int32_t *fpc_b = nullptr, *fpc_e = nullptr, *fpc_slot = nullptr, *frs_slot = nullptr,
+                      *frs_cnt = nullptr;
+        const int32_t *bpc_b = nullptr, *bpc_e = nullptr, *bpc_mid = nullptr, *bpc_slot = nullptr,
+                      *brs_slot = nullptr, *brs_cnt = nullptr, *bpc_row = nullptr;
+    } rs[2];
     float *partial = nullptr; // one partial row per range piece
     int64_t partial_floats = 0;
     float *comp = nullptr;    // compensation rows of those levels
@@ -113,28 +123,49 @@ int ensure_comp(cbm_handle *h, int32_t n_cols) {
     return CBM_OK;
 }
 
-// Whether the gather passes of an n_cols-wide call run by XCD column range;
-// elem is the byte size of the gathered matrix's entries (2 for BF16 X or G,
-// 4 for FP32).  Automatic: rows of 64 entries or more on average (so that a
-// row's pieces are not mostly partial-row overhead) and a gathered matrix of
-// 24 to 256 MiB (smaller ones hit the L2s well enough without the partial
-// rows: N = 132,534 at F = 64 measured 1.70 ms per BF16 step without ranges,
-// 1.79 ms with, at 16 MiB of BF16 X; its FP32 step, 32 MiB, 2.58 -> 2.06 ms;
-// larger ones leave each XCD's range far beyond its L2).
-bool use_ranges(const cbm_handle *h, int32_t n_cols, int elem) {
-    if (h->d.fwd_pieces <= 0 || h->d.bwd_pieces <= 0 || h->xcd_ranges == 0) return false;
-    if (h->xcd_ranges > 0) return true;
-    const int64_t bytes = int64_t(h->d.n_rows) * n_cols * elem;
-    return h->d.nnz >= 64 * int64_t(h->d.n_rows) && bytes >= (int64_t(24) << 20) && bytes <= (int64_t(256) << 20);
+using RangeSet = cbm_handle::RangeSet;
+
+// The piece set the gather passes of an n_cols-wide call run by XCD column
+// range with, or null for none; elem is the byte size of the gathered
+// matrix's entries (2 for BF16 X or G, 4 for FP32).  Automatic: rows of 64
+// entries or more on average (so that a row's pieces are not mostly
+// partial-row overhead) and a gathered matrix of 24 to 256 MiB (smaller ones
+// hit the L2s well enough without the partial rows: N = 132,534 at F = 64
+// measured 1.70 ms per BF16 step without ranges, 1.79 ms with, at 16 MiB of
+// BF16 X; its FP32 step, 32 MiB, 2.58 -> 2.06 ms; larger ones leave each
+// XCD's range far beyond its L2).  The 4-range set halves the partial rows
+// and each gather: it is taken when half a row is 256 bytes or more (F = 256:
+// BF16 step 4.40 -> 4.17 ms, FP32 7.51 -> 7.33 ms; at F = 64 FP32, 128-byte
+// halves, 2.06 -> 2.18 ms, so the 8-range set stays there).
+const RangeSet *pick_ranges(const cbm_handle *h, int32_t n_cols, int elem) {
+    auto usable = [](const RangeSet &r) { return r.fwd > 0 && r.bwd > 0; };
+    const RangeSet &r8 = h->rs[0], &r4 = h->rs[1];
+    switch (h->xcd_ranges) {
+        case 0: return nullptr;
+        case 8: return usable(r8) ? &r8 : nullptr;
+        case 4: return usable(r4) ? &r4 : nullptr;
+        case -1: {
+            const int64_t bytes = int64_t(h->d.n_rows) * n_cols * elem;
+            if (h->d.nnz < 64 * int64_t(h->d.n_rows) || bytes < (int64_t(24) << 20) || bytes > (int64_t(256) << 20))
+                return nullptr;
+            break;
+        }
+        default: break;
+    }
+    if (int64_t(n_cols) * elem >= 512 && usable(r4)) return &r4;
+    if (usable(r8)) return &r8;
+    return usable(r4) ? &r4 : nullptr;
 }
 
 int elem_of(int32_t dtype) { return dtype == CBM_BF16 ? 2 : 4; }
 
-// The partial rows are sized for either storage type, so that a reserved
-// handle serves BF16 and FP32 calls of this width without allocating.
+// The partial rows are sized for either storage type and either piece set,
+// so that a reserved handle serves BF16 and FP32 calls of this width without
+// allocating.
 int ensure_partial(cbm_handle *h, int32_t n_cols) {
-    if (!use_ranges(h, n_cols, 2) && !use_ranges(h, n_cols, 4)) return CBM_OK;
-    const int64_t need = int64_t(std::max(h->d.fwd_pieces, h->d.bwd_pieces)) * n_cols;
+    if (!pick_ranges(h, n_cols, 2) && !pick_ranges(h, n_cols, 4)) return CBM_OK;
+    const int64_t need =
+        int64_t(std::max({h->rs[0].fwd, h->rs[0].bwd, h->rs[1].fwd, h->rs[1].bwd})) * n_cols;
     if (need <= h->partial_floats) return CBM_OK;
     if (h->partial) {
         const hipError_t e = hipFree(h->partial);
@@ -296,35 +327,36 @@ int propagate_levels(cbm_handle *h, float *y, int64_t ldy, int32_t n_cols, hipSt
     return CBM_OK;
 }
 
-// Copies the 9 piece offsets of a range table into a pass.
-void set_ranges(cbmk::RowPass &rp, const int32_t *first) {
+// Copies the n_ranges + 1 piece offsets of a range table into a pass.
+void set_ranges(cbmk::RowPass &rp, const int32_t *first, int32_t n_ranges) {
     rp.ranges = true;
-    for (int k = 0; k < 9; ++k) rp.range_first[k] = first[k];
-    rp.xcd_slabs = 1;
+    rp.range_count = n_ranges;
+    for (int k = 0; k <= n_ranges; ++k) rp.range_first[k] = first[k];
+    rp.xcd_slabs = 8 / n_ranges;
     rp.wave_rows = 0;
     rp.skip_above = INT32_MAX;
 }
 
-// The forward by XCD column range: one pass over all forward pieces (each
-// gathers one range of X into its partial row; range k on XCD k), then a
+// The forward by XCD column range: one pass over all forward pieces of set
+// rs (each gathers one range of X into its partial row), then a
 // reduction that adds each row's pieces in order and applies the row scale --
 // per tree level with the parent term (compensated from the Kahan level on)
 // for the product, or over all rows at once without it for the Delta half
 // alone (the level propagation then adds the same parent terms, so the two
 // halves give the product's bits).
-int forward_ranges(cbm_handle *h, const cbmk::RowPass &base, bool with_parent, hipStream_t stream,
-                   const char *what) {
+int forward_ranges(cbm_handle *h, const RangeSet &rs, const cbmk::RowPass &base, bool with_parent,
+                   hipStream_t stream, const char *what) {
     cbmk::RowPass pc = base;
-    pc.node_of = h->d.fpc_slot;
+    pc.node_of = rs.fpc_slot;
     pc.parent_of = nullptr;
-    pc.ptr = h->d.fpc_b;
-    pc.ptr_end = h->d.fpc_e;
+    pc.ptr = rs.fpc_b;
+    pc.ptr_end = rs.fpc_e;
     pc.rowscale = nullptr;
     pc.escale = nullptr;
     pc.extra = cbmk::kExtraNone;
     pc.y = h->partial;
     pc.ldy = base.ncols;
-    set_ranges(pc, h->fpc_range);
+    set_ranges(pc, rs.f_first, rs.n_ranges);
     hipError_t e = cbmk::launch_row_pass(pc, stream);
     if (e != hipSuccess) return device_error(e, what);
     ++h->last_launches;
@@ -338,8 +370,8 @@ int forward_ranges(cbm_handle *h, const cbmk::RowPass &base, bool with_parent, h
     rr.ldc = base.ncols;
     cbmk::ReduceArgs rd;
     rd.pos = nullptr;
-    rd.slot = h->d.frs_slot;
-    rd.cnt = h->d.frs_cnt;
+    rd.slot = rs.frs_slot;
+    rd.cnt = rs.frs_cnt;
     rd.scratch = h->partial;
     const int32_t levels = with_parent ? h->d.n_levels : 1;
     for (int32_t l = 0; l < levels; ++l) {
@@ -447,23 +479,46 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
         set_error("cbm_create: missing forward chunk tables");
         return CBM_ERR_ARG;
     }
-    const bool fwd_pieces = d.fwd_pieces > 0, bwd_pieces = d.bwd_pieces > 0;
-    if (d.fwd_pieces < 0 || d.bwd_pieces < 0 ||
-        (fwd_pieces && (!d.fpc_range_host || !d.fpc_b || !d.fpc_e || !d.fpc_slot || !d.frs_slot || !d.frs_cnt)) ||
-        (bwd_pieces && (!d.bpc_range_host || !d.bpc_b || !d.bpc_e || !d.bpc_mid || !d.bpc_slot || !d.brs_slot ||
-                        !d.brs_cnt || !d.bpc_row))) {
-        set_error("cbm_create: incomplete range piece tables");
-        return CBM_ERR_ARG;
-    }
-    for (int pass = 0; pass < 2; ++pass) {
-        const int32_t *first = pass == 0 ? d.fpc_range_host : d.bpc_range_host;
-        const int32_t total = pass == 0 ? d.fwd_pieces : d.bwd_pieces;
-        if (total <= 0) continue;
-        bool ok = first[0] == 0 && first[8] == total;
-        for (int k = 0; k < 8; ++k) ok = ok && first[k + 1] >= first[k];
-        if (!ok) {
-            set_error("cbm_create: range piece offsets do not run from 0 to the piece count");
-            return CBM_ERR_ARG;
+    RangeSet sets[2];
+    {
+        RangeSet &a = sets[0], &b = sets[1];
+        a.n_ranges = 8;
+        a.fwd = d.fwd_pieces;
+        a.bwd = d.bwd_pieces;
+        a.fpc_b = d.fpc_b, a.fpc_e = d.fpc_e, a.fpc_slot = d.fpc_slot, a.frs_slot = d.frs_slot;
+        a.frs_cnt = d.frs_cnt;
+        a.bpc_b = d.bpc_b, a.bpc_e = d.bpc_e, a.bpc_mid = d.bpc_mid, a.bpc_slot = d.bpc_slot;
+        a.brs_slot = d.brs_slot, a.brs_cnt = d.brs_cnt, a.bpc_row = d.bpc_row;
+        b.n_ranges = 4;
+        b.fwd = d.fwd_pieces4;
+        b.bwd = d.bwd_pieces4;
+        b.fpc_b = d.fpc4_b, b.fpc_e = d.fpc4_e, b.fpc_slot = d.fpc4_slot, b.frs_slot = d.frs4_slot;
+        b.frs_cnt = d.frs4_cnt;
+        b.bpc_b = d.bpc4_b, b.bpc_e = d.bpc4_e, b.bpc_mid = d.bpc4_mid, b.bpc_slot = d.bpc4_slot;
+        b.brs_slot = d.brs4_slot, b.brs_cnt = d.brs4_cnt, b.bpc_row = d.bpc4_row;
+        const int32_t *firsts[2][2] = {{d.fpc_range_host, d.bpc_range_host}, {d.fpc4_range_host, d.bpc4_range_host}};
+        for (int k = 0; k < 2; ++k) {
+            RangeSet &r = sets[k];
+            if (r.fwd < 0 || r.bwd < 0 ||
+                (r.fwd > 0 && (!firsts[k][0] || !r.fpc_b || !r.fpc_e || !r.fpc_slot || !r.frs_slot || !r.frs_cnt)) ||
+                (r.bwd > 0 && (!firsts[k][1] || !r.bpc_b || !r.bpc_e || !r.bpc_mid || !r.bpc_slot || !r.brs_slot ||
+                               !r.brs_cnt || !r.bpc_row))) {
+                set_error("cbm_create: incomplete range piece tables (%d ranges)", r.n_ranges);
+                return CBM_ERR_ARG;
+            }
+            for (int pass = 0; pass < 2; ++pass) {
+                const int32_t *first = firsts[k][pass];
+                const int32_t total = pass == 0 ? r.fwd : r.bwd;
+                if (total <= 0) continue;
+                bool ok = first[0] == 0 && first[r.n_ranges] == total;
+                for (int q = 0; q < r.n_ranges; ++q) ok = ok && first[q + 1] >= first[q];
+                if (!ok) {
+                    set_error("cbm_create: range piece offsets do not run from 0 to the piece count (%d ranges)",
+                              r.n_ranges);
+                    return CBM_ERR_ARG;
+                }
+                std::copy(first, first + r.n_ranges + 1, pass == 0 ? r.f_first : r.b_first);
+            }
         }
     }
     cbm_handle *h = new (std::nothrow) cbm_handle;
@@ -472,10 +527,10 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
         return CBM_ERR_NOMEM;
     }
     h->d = d;
-    if (fwd_pieces) std::copy(d.fpc_range_host, d.fpc_range_host + 9, h->fpc_range);
-    if (bwd_pieces) std::copy(d.bpc_range_host, d.bpc_range_host + 9, h->bpc_range);
-    h->d.fpc_range_host = fwd_pieces ? h->fpc_range : nullptr;
-    h->d.bpc_range_host = bwd_pieces ? h->bpc_range : nullptr;
+    h->rs[0] = sets[0];
+    h->rs[1] = sets[1];
+    // the host range tables are the caller's: the handle keeps its own copies
+    h->d.fpc_range_host = h->d.bpc_range_host = h->d.fpc4_range_host = h->d.bpc4_range_host = nullptr;
     h->lvl_ptr.assign(d.lvl_ptr_host, d.lvl_ptr_host + L + 1);
     h->lvl_nlong.assign(d.lvl_nlong_host, d.lvl_nlong_host + L);
     h->fck_ptr.assign(d.fck_ptr_host, d.fck_ptr_host + L + 1);
@@ -536,11 +591,11 @@ extern "C" int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t l
     const bool split = use_split(h);
     return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
         const cbmk::RowPass rp = forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w);
-        if (use_ranges(h, n_cols, elem)) {
+        if (const RangeSet *rs = pick_ranges(h, n_cols, elem)) {
             // Pieces and one reduction over all rows, then the light level
             // propagation: the bits of the per-level reduction with the
             // parent term (forward_ranges with_parent), in fewer, fuller launches.
-            const int r = forward_ranges(h, rp, false, stream, "cbm_matmul");
+            const int r = forward_ranges(h, *rs, rp, false, stream, "cbm_matmul");
             return r == CBM_OK ? propagate_levels(h, y + c0, ldy, w, stream, "cbm_matmul") : r;
         }
         if (!split) return forward_levels(h, rp, true, stream, "cbm_matmul");
@@ -662,20 +717,20 @@ int adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, co
     rd.first = 0;
     rd.count = h->d.n_bred;
     rd.scratch = h->scratch;
-    if (use_ranges(h, n_cols, elem_of(dtype))) {
+    if (const RangeSet *rs = pick_ranges(h, n_cols, elem_of(dtype))) {
         // By XCD column range: every backward piece (one range of G and Z,
         // its leaf rows before its internal rows) into its partial row, then
         // one reduction over all positions.
         cbmk::RowPass pc = tp;
-        pc.node_of = h->d.bpc_slot;
-        pc.idx = h->d.bpc_row;
-        pc.ptr = h->d.bpc_b;
-        pc.ptr_end = h->d.bpc_e;
-        pc.mid = h->d.bpc_mid;
+        pc.node_of = rs->bpc_slot;
+        pc.idx = rs->bpc_row;
+        pc.ptr = rs->bpc_b;
+        pc.ptr_end = rs->bpc_e;
+        pc.mid = rs->bpc_mid;
         pc.rowscale = nullptr;
         pc.y = h->partial;
         pc.ldy = n_cols;
-        set_ranges(pc, h->bpc_range);
+        set_ranges(pc, rs->b_first, rs->n_ranges);
         hipError_t e = cbmk::launch_row_pass(pc, stream);
         if (e != hipSuccess) return device_error(e, "adjoint Delta^T pass");
         ++h->last_launches;
@@ -687,8 +742,8 @@ int adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, co
         rr.g2 = nullptr;
         cbmk::ReduceArgs pr;
         pr.pos = nullptr;
-        pr.slot = h->d.brs_slot;
-        pr.cnt = h->d.brs_cnt;
+        pr.slot = rs->brs_slot;
+        pr.cnt = rs->brs_cnt;
         pr.first = 0;
         pr.count = h->d.n_rows;
         pr.scratch = h->partial;
@@ -743,7 +798,8 @@ extern "C" int cbm_delta_spmm(cbm_handle *h, int32_t dtype, const void *x, int64
     const int elem = dtype == CBM_BF16 ? 2 : 4;
     return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
         const cbmk::RowPass rp = forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w);
-        if (use_ranges(h, n_cols, elem)) return forward_ranges(h, rp, false, stream, "cbm_delta_spmm");
+        if (const RangeSet *rs = pick_ranges(h, n_cols, elem))
+            return forward_ranges(h, *rs, rp, false, stream, "cbm_delta_spmm");
         return delta_flat(h, rp, stream, "cbm_delta_spmm");
     });
 }
@@ -798,7 +854,7 @@ extern "C" int cbm_set_xcd_slabs(cbm_handle *h, int32_t slabs) {
 }
 
 extern "C" int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode) {
-    if (!h || mode < -1 || mode > 1) {
+    if (!h || !(mode == -1 || mode == 0 || mode == 1 || mode == 4 || mode == 8)) {
         set_error("cbm_set_xcd_ranges: invalid arguments");
         return CBM_ERR_ARG;
     }
@@ -811,7 +867,8 @@ extern "C" int cbm_xcd_ranges_active(const cbm_handle *h, int32_t dtype, int32_t
         set_error("cbm_xcd_ranges_active: invalid arguments");
         return CBM_ERR_ARG;
     }
-    return use_ranges(h, n_cols, elem_of(dtype)) ? 1 : 0;
+    const RangeSet *rs = pick_ranges(h, n_cols, elem_of(dtype));
+    return rs ? rs->n_ranges : 0;
 }
 
 extern "C" int cbm_set_slab_bytes(cbm_handle *h, int64_t bytes) {

@@ -369,12 +369,14 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
     const int l = lane & (LPR - 1);
     const int sub = (lane / LPR) & (S - 1);
     // Workgroup b: slab b % xcd_slabs of row block b / xcd_slabs, or, with
-    // XCD ranges, row block b / 8 of range b % 8's pieces.
+    // XCD ranges, row block b / 8 of range (b % 8) / S's pieces, slab
+    // (b % 8) % S (S = xcd_slabs = 8 / range_count).
     int32_t slab, p_begin = a.p_begin, p_end = a.p_end;
     int64_t blk;
     if (a.ranges) {
-        const uint32_t k = blockIdx.x & 7u;
-        slab = 0;
+        const uint32_t x = blockIdx.x & 7u, per = uint32_t(a.xcd_slabs);
+        const uint32_t k = x / per;
+        slab = int32_t(x % per);
         blk = int64_t(blockIdx.x >> 3);
         p_begin = a.range_first[k];
         p_end = a.range_first[k + 1];
@@ -640,9 +642,11 @@ template <class P, int VE, int LPR>
 hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
     constexpr int64_t kRowsPerBlock = 4 * (64 / LPR);
     if (lk.kind == Kind::kPass && rp.ranges) {
-        // One grid for the 8 ranges: row block j of range k is workgroup 8 j + k.
+        // One grid for all ranges: row block j of range k, slab s is
+        // workgroup 8 j + k S + s.
         int64_t most = 0;
-        for (int k = 0; k < 8; ++k) most = std::max<int64_t>(most, int64_t(rp.range_first[k + 1]) - rp.range_first[k]);
+        for (int k = 0; k < rp.range_count; ++k)
+            most = std::max<int64_t>(most, int64_t(rp.range_first[k + 1]) - rp.range_first[k]);
         if (most <= 0) return hipSuccess;
         k_row_pass<P, VE, LPR, 1>
             <<<dim3(unsigned((most + kRowsPerBlock - 1) / kRowsPerBlock * 8)), dim3(256), 0, stream>>>(rp);
@@ -707,11 +711,19 @@ hipError_t dispatch_ve(const RowPass &rp, const Launch &lk, hipStream_t stream) 
     // either way, so its entry slices, and with them the bits, do not
     // change; narrower gathers than 16 vectors also measured slower.
     RowPass r = rp;
-    const int32_t want = std::min<int32_t>(std::max<int32_t>(1, rp.xcd_slabs),
-                                           std::max<int32_t>(1, rp.ncols / (16 * ve)));
-    int32_t w = (rp.ncols + want - 1) / want;
-    w = (w + ve - 1) / ve * ve;
-    r.xcd_slabs = (rp.ncols + w - 1) / w;
+    int32_t w;
+    if (rp.ranges) {
+        // The range mapping needs exactly xcd_slabs slabs per range (a slab
+        // left empty on narrow rows is fine: the others cover the columns).
+        w = (rp.ncols + rp.xcd_slabs - 1) / rp.xcd_slabs;
+        w = (w + ve - 1) / ve * ve;
+    } else {
+        const int32_t want = std::min<int32_t>(std::max<int32_t>(1, rp.xcd_slabs),
+                                               std::max<int32_t>(1, rp.ncols / (16 * ve)));
+        w = (rp.ncols + want - 1) / want;
+        w = (w + ve - 1) / ve * ve;
+        r.xcd_slabs = (rp.ncols + w - 1) / w;
+    }
     r.slab_w = w;
     if (vec) return dispatch_lpr<P, kVe>(r, lk, lanes_per_row(w, kVe), stream);
     return dispatch_lpr<P, 1>(r, lk, lanes_per_row(w, 1), stream);

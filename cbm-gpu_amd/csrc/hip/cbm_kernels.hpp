@@ -77,10 +77,13 @@ struct RowPass {
     int32_t xcd_slabs = 1;
     int32_t slab_w = 0;
     // XCD column ranges: when set, the positions are pieces grouped by the
-    // column range they gather from, range k holding [range_first[k],
-    // range_first[k + 1]); workgroup b works on range b % 8 (its XCD), p_begin
-    // and p_end are ignored, and xcd_slabs must be 1.
+    // column range they gather from, range k of range_count (2, 4 or 8)
+    // holding [range_first[k], range_first[k + 1]).  With S = 8 / range_count,
+    // workgroup b works on range (b % 8) / S over feature slab (b % 8) % S, so
+    // each XCD gathers one slab of one range; p_begin and p_end are ignored
+    // and xcd_slabs must be S.
     bool ranges = false;
+    int32_t range_count = 8;
     int32_t range_first[9] = {};
 };
 

@@ -12,7 +12,8 @@
 //                adjoint's reverse (leaves-to-roots) gather.
 //   pieces       (matrices up to CBM_RANGE_MAX_ROWS rows) each forward and
 //                backward row cut by column range of the gathered matrix, so
-//                that the workgroups of one XCD gather one range only.
+//                that the workgroups of one XCD gather one range only; two
+//                sets, for CBM_XCD_RANGES and CBM_XCD_RANGES_WIDE ranges.
 #include "common.hpp"
 #include "../../../include/cbm_host.h"
 
@@ -76,10 +77,10 @@ struct Pieces {
     std::vector<int32_t> b, e, mid, slot, range_first, rs_slot, rs_cnt, order;
 };
 
-// Returns false (and builds nothing) if the pieces would not fit int32 numbering.
+// R ranges of equal width over the gathered index space.  Returns false (and
+// builds nothing) if the pieces would not fit int32 numbering.
 bool range_pieces(int32_t n, const std::vector<int32_t> &ptr, const std::vector<int32_t> &idx,
-                  const std::vector<int32_t> *cut, Pieces &pc) {
-    constexpr int32_t R = CBM_XCD_RANGES;
+                  const std::vector<int32_t> *cut, int32_t R, Pieces &pc) {
     const int64_t width = (int64_t(n) + R - 1) / R;
     const int parts = cut ? 2 : 1;
     // bound[p][q][k]: first entry of part q of row p in range k (k = 0..R).
@@ -174,6 +175,22 @@ bool range_pieces(int32_t n, const std::vector<int32_t> &ptr, const std::vector<
     }
     return true;
 }
+
+// Where each piece set's tables go.
+struct PieceSlots {
+    int32_t n_ranges;
+    int fwd_stat, bwd_stat;
+    int fb, fe, fslot, frange, frs_slot, frs_cnt;
+    int bb, be, bmid, bslot, brange, brs_slot, brs_cnt, brow;
+};
+constexpr PieceSlots kPieceSets[2] = {
+    {CBM_XCD_RANGES, CBM_PS_FWD_PIECES, CBM_PS_BWD_PIECES, CBM_P_FPC_B, CBM_P_FPC_E, CBM_P_FPC_SLOT,
+     CBM_P_FPC_RANGE, CBM_P_FRS_SLOT, CBM_P_FRS_CNT, CBM_P_BPC_B, CBM_P_BPC_E, CBM_P_BPC_MID, CBM_P_BPC_SLOT,
+     CBM_P_BPC_RANGE, CBM_P_BRS_SLOT, CBM_P_BRS_CNT, CBM_P_BPC_ROW},
+    {CBM_XCD_RANGES_WIDE, CBM_PS_FWD_PIECES4, CBM_PS_BWD_PIECES4, CBM_P_FPC4_B, CBM_P_FPC4_E, CBM_P_FPC4_SLOT,
+     CBM_P_FPC4_RANGE, CBM_P_FRS4_SLOT, CBM_P_FRS4_CNT, CBM_P_BPC4_B, CBM_P_BPC4_E, CBM_P_BPC4_MID,
+     CBM_P_BPC4_SLOT, CBM_P_BPC4_RANGE, CBM_P_BRS4_SLOT, CBM_P_BRS4_CNT, CBM_P_BPC4_ROW},
+};
 
 }  // namespace
 
@@ -358,15 +375,17 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
         }
         const bool ranges = n <= CBM_RANGE_MAX_ROWS;
         if (ranges) {
-            Pieces pc;
-            if (!range_pieces(n, fptr, fcol, nullptr, pc)) pc = Pieces();
-            plan->stat[CBM_PS_FWD_PIECES] = int64_t(pc.b.size());
-            plan->arr[CBM_P_FPC_B].swap(pc.b);
-            plan->arr[CBM_P_FPC_E].swap(pc.e);
-            plan->arr[CBM_P_FPC_SLOT].swap(pc.slot);
-            plan->arr[CBM_P_FPC_RANGE].swap(pc.range_first);
-            plan->arr[CBM_P_FRS_SLOT].swap(pc.rs_slot);
-            plan->arr[CBM_P_FRS_CNT].swap(pc.rs_cnt);
+            for (const PieceSlots &ps : kPieceSets) {
+                Pieces pc;
+                if (!range_pieces(n, fptr, fcol, nullptr, ps.n_ranges, pc)) pc = Pieces();
+                plan->stat[ps.fwd_stat] = int64_t(pc.b.size());
+                plan->arr[ps.fb].swap(pc.b);
+                plan->arr[ps.fe].swap(pc.e);
+                plan->arr[ps.fslot].swap(pc.slot);
+                plan->arr[ps.frange].swap(pc.range_first);
+                plan->arr[ps.frs_slot].swap(pc.rs_slot);
+                plan->arr[ps.frs_cnt].swap(pc.rs_cnt);
+            }
         }
 
         // Backward: Delta^T (the CSC of Delta), columns ordered long first,
@@ -416,18 +435,19 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
             plan->arr[CBM_P_BRED_POS].swap(b.red_pos);
             plan->arr[CBM_P_BRED_SLOT].swap(b.red_slot);
             plan->arr[CBM_P_BRED_CNT].swap(b.red_cnt);
-            if (ranges && plan->stat[CBM_PS_FWD_PIECES] > 0) {
+            for (const PieceSlots &ps : kPieceSets) {
+                if (!ranges || plan->stat[ps.fwd_stat] <= 0) continue;
                 Pieces pc;
-                if (!range_pieces(n, bptr, brow, &bmid, pc)) pc = Pieces();
-                plan->stat[CBM_PS_BWD_PIECES] = int64_t(pc.b.size());
-                plan->arr[CBM_P_BPC_B].swap(pc.b);
-                plan->arr[CBM_P_BPC_E].swap(pc.e);
-                plan->arr[CBM_P_BPC_MID].swap(pc.mid);
-                plan->arr[CBM_P_BPC_SLOT].swap(pc.slot);
-                plan->arr[CBM_P_BPC_RANGE].swap(pc.range_first);
-                plan->arr[CBM_P_BRS_SLOT].swap(pc.rs_slot);
-                plan->arr[CBM_P_BRS_CNT].swap(pc.rs_cnt);
-                plan->arr[CBM_P_BPC_ROW].swap(pc.order);
+                if (!range_pieces(n, bptr, brow, &bmid, ps.n_ranges, pc)) pc = Pieces();
+                plan->stat[ps.bwd_stat] = int64_t(pc.b.size());
+                plan->arr[ps.bb].swap(pc.b);
+                plan->arr[ps.be].swap(pc.e);
+                plan->arr[ps.bmid].swap(pc.mid);
+                plan->arr[ps.bslot].swap(pc.slot);
+                plan->arr[ps.brange].swap(pc.range_first);
+                plan->arr[ps.brs_slot].swap(pc.rs_slot);
+                plan->arr[ps.brs_cnt].swap(pc.rs_cnt);
+                plan->arr[ps.brow].swap(pc.order);
             }
         }
 

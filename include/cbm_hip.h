@@ -143,6 +143,25 @@ typedef struct cbm_desc {
     const int32_t *brs_cnt;
     int32_t bwd_pieces;
     const int32_t *bpc_row;
+    /* the same for the 4-range set (plan arrays CBM_P_FPC4_* .. CBM_P_BPC4_ROW,
+       stats FWD_PIECES4 / BWD_PIECES4; offsets [5]): each of its ranges is
+       gathered on two XCDs, one column half each */
+    const int32_t *fpc4_range_host;
+    const int32_t *fpc4_b;
+    const int32_t *fpc4_e;
+    const int32_t *fpc4_slot;
+    const int32_t *frs4_slot;
+    const int32_t *frs4_cnt;
+    int32_t fwd_pieces4;
+    const int32_t *bpc4_range_host;
+    const int32_t *bpc4_b;
+    const int32_t *bpc4_e;
+    const int32_t *bpc4_mid;
+    const int32_t *bpc4_slot;
+    const int32_t *brs4_slot;
+    const int32_t *brs4_cnt;
+    int32_t bwd_pieces4;
+    const int32_t *bpc4_row;
 } cbm_desc;
 
 typedef struct cbm_handle cbm_handle;
@@ -273,26 +292,32 @@ int cbm_set_xcd_slabs(cbm_handle *h, int32_t slabs);
 
 /*
  * XCD column ranges (-1 automatic, the default; 0 off; 1 on when the plan has
- * range pieces).  The gathered matrix's rows are cut into 8 ranges and the
- * workgroups of XCD k (workgroup b runs on XCD b mod 8) gather range k only,
- * so each XCD's 4 MB L2 serves an eighth of the matrix.  The forward Delta
- * pass and the adjoint Delta^T pass then write one partial row per piece
- * (a row's entries in one range, at most CBM_PIECE of them) and a reduction
- * adds each row's pieces in order; the forward's level propagation follows
- * as usual.  Results are deterministic but rounded in another order than
- * with ranges off.  Automatic: on when the plan has pieces, Delta holds 64
- * entries per row or more on average, and the gathered matrix (N x F entries
- * of the call's storage type: 2 bytes for BF16, 4 for FP32) is between 24 and
- * 256 MiB, so a BF16 and an FP32 call of one width may choose differently.
- * Needs F x 4 bytes of scratch per piece; cbm_reserve sizes it for the current
- * choice of either storage type.
+ * range pieces, with the piece set chosen by row width; 8 or 4 on with that
+ * set).  The gathered matrix's rows are cut into ranges and each XCD
+ * (workgroup b runs on XCD b mod 8) gathers from one range only, so its 4 MB
+ * L2 serves an eighth of the matrix: with 8 ranges XCD k gathers range k
+ * whole; with 4 ranges XCDs 2k and 2k + 1 gather range k, one column half
+ * each.  The forward Delta pass and the adjoint Delta^T pass then write one
+ * partial row per piece (a row's entries in one range, at most CBM_PIECE of
+ * them) and a reduction adds each row's pieces in order; the forward's level
+ * propagation follows as usual.  Four ranges halve those partial rows but
+ * halve each gather too, so they are chosen when half a gathered row is
+ * 256 bytes or more (F x element size >= 512 bytes) and the plan has them.
+ * Results are deterministic but rounded in another order than with ranges
+ * off (and differently for the two sets).  Automatic: on when the plan has
+ * pieces, Delta holds 64 entries per row or more on average, and the
+ * gathered matrix (N x F entries of the call's storage type: 2 bytes for
+ * BF16, 4 for FP32) is between 24 and 256 MiB, so a BF16 and an FP32 call of
+ * one width may choose differently.  Needs F x 4 bytes of scratch per piece;
+ * cbm_reserve sizes it for the current choice of either storage type.
  */
 int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode);
 
 /*
- * 1 when a dtype call of n_cols columns runs its gather passes by XCD column
- * range under the handle's current setting, 0 when not, CBM_ERR_ARG for
- * invalid arguments.  Reports the automatic choice above; computes nothing.
+ * The number of column ranges (8 or 4) a dtype call of n_cols columns runs
+ * its gather passes with under the handle's current setting, 0 when it runs
+ * without, CBM_ERR_ARG for invalid arguments.  Reports the choice described
+ * above; computes nothing.
  */
 int cbm_xcd_ranges_active(const cbm_handle *h, int32_t dtype, int32_t n_cols);
 

@@ -239,7 +239,24 @@ enum {
     CBM_P_BRS_CNT = 53,    /* int32 [N]    number of slots of each backward position                     */
     CBM_P_BPC_ROW = 54,    /* int32 [nnz]  BWD_ROW's entries rewritten range by range (a row's extent is
                                            unchanged): in each range the leaf rows, then internal rows  */
-    CBM_P_N_ARRAYS = 55
+    /* The same pieces for CBM_XCD_RANGES_WIDE ranges (same layout, same size limit).  Each of these
+       ranges is gathered on two XCDs, one column half each, which halves the partial rows per row
+       for rows wide enough to split (see cbm_set_xcd_ranges). */
+    CBM_P_FPC4_B = 55,
+    CBM_P_FPC4_E = 56,
+    CBM_P_FPC4_SLOT = 57,
+    CBM_P_FPC4_RANGE = 58,
+    CBM_P_FRS4_SLOT = 59,
+    CBM_P_FRS4_CNT = 60,
+    CBM_P_BPC4_B = 61,
+    CBM_P_BPC4_E = 62,
+    CBM_P_BPC4_MID = 63,
+    CBM_P_BPC4_SLOT = 64,
+    CBM_P_BPC4_RANGE = 65,
+    CBM_P_BRS4_SLOT = 66,
+    CBM_P_BRS4_CNT = 67,
+    CBM_P_BPC4_ROW = 68,
+    CBM_P_N_ARRAYS = 69
 };
 
 /* Scalars of a cbm_plan, addressed by cbm_plan_stat(). */
@@ -252,12 +269,15 @@ enum {
     CBM_PS_REV_SLOTS = 5,  /* partial slots needed by the largest reverse-pass level    */
     CBM_PS_FWD_PIECES = 6, /* forward column-range pieces (0: none built)               */
     CBM_PS_BWD_PIECES = 7, /* backward column-range pieces                              */
-    CBM_PS_N_STATS = 8
+    CBM_PS_FWD_PIECES4 = 8, /* forward pieces of the CBM_XCD_RANGES_WIDE set            */
+    CBM_PS_BWD_PIECES4 = 9, /* backward pieces of that set                               */
+    CBM_PS_N_STATS = 10
 };
 
 #define CBM_LONG_ROW 256
 #define CBM_CHUNK 1024
 #define CBM_XCD_RANGES 8
+#define CBM_XCD_RANGES_WIDE 4
 #define CBM_PIECE 256
 #define CBM_RANGE_MAX_ROWS 524288
 

@@ -235,21 +235,26 @@ def test_xcd_slabs_give_the_same_bits(hub, cuda, mode, dtype, f):
 
 @pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
 @pytest.mark.parametrize("dtype,f", [(torch.float32, 1), (torch.float32, 8), (torch.float32, 64),
-                                     (torch.float32, 300), (torch.bfloat16, 64), (torch.bfloat16, 256)])
+                                     (torch.float32, 300), (torch.bfloat16, 20), (torch.bfloat16, 64),
+                                     (torch.bfloat16, 256)])
 def test_xcd_ranges_against_the_oracle(hub, star, cuda, mode, dtype, f):
     """Gathers by XCD column range (one partial row per range piece, reduced
-    in order, fused with the level propagation): forward and adjoint against
+    in order, fused with the level propagation), with the 8-range set (one
+    range per XCD) and the 4-range set (each range on two XCDs, one column
+    half each, the second half empty for F = 1): forward and adjoint against
     FP64 on the hub graph (hub rows of several pieces per range, deep tree)
     and the star graph (a 2500-child list); repeat calls give the same bits;
     with ranges off the results are within rounding of them."""
     for a, n, m in ((hub[0], HUB_N, hub[1][mode]), (star[0], star[1], star[2][mode])):
         h = m.cusparse_extension
-        assert h.plan_stats["fwd_pieces"] > 0 and h.plan_stats["bwd_pieces"] > 0
+        for k in ("fwd_pieces", "bwd_pieces", "fwd_pieces4", "bwd_pieces4"):
+            assert h.plan_stats[k] > 0, k
         xt = torch.from_numpy(_x((n, f), 90 + f)).to(cuda).to(dtype)
         xe = xt.float().cpu().numpy()
         out = {}
-        for ranges in (True, False, True):
+        for ranges in (8, 4, False, 8, 4):
             h.set_xcd_ranges(ranges)
+            assert h.xcd_ranges_active(dtype, f) == (ranges or 0)
             y = torch.full((n, f), float("nan"), device=cuda)
             dx = torch.full((n, f), float("nan"), device=cuda)
             h.forward(xt, y)
@@ -258,55 +263,62 @@ def test_xcd_ranges_against_the_oracle(hub, star, cuda, mode, dtype, f):
                 assert torch.equal(y, out[ranges][0]) and torch.equal(dx, out[ranges][1])
             out[ranges] = (y, dx)
         h.set_xcd_ranges(None)
-        y, dx = out[True]
-        assert O.max_rel_error(y.cpu().numpy(), O.forward(a, xe, mode)) <= 1e-5, (n, mode, f)
-        assert O.max_rel_error(dx.cpu().numpy(), O.adjoint(a, xe, mode)) <= 1e-5, (n, mode, f)
-        assert O.max_rel_error(y.cpu().numpy(), out[False][0].cpu().numpy()) <= 1e-5
+        for ranges in (8, 4):
+            y, dx = out[ranges]
+            assert O.max_rel_error(y.cpu().numpy(), O.forward(a, xe, mode)) <= 1e-5, (n, mode, f, ranges)
+            assert O.max_rel_error(dx.cpu().numpy(), O.adjoint(a, xe, mode)) <= 1e-5, (n, mode, f, ranges)
+            assert O.max_rel_error(y.cpu().numpy(), out[False][0].cpu().numpy()) <= 1e-5
 
 
 def test_xcd_ranges_integer_exact_and_halves(hub, cuda):
-    """Integer-valued inputs are exact by range too; delta_spmm + propagate
-    give the product's bits, plain and compensated."""
+    """Integer-valued inputs are exact by range too (both piece sets);
+    delta_spmm + propagate give the product's bits, plain and compensated."""
     a, mats = hub
     h = mats["ax"].cusparse_extension
-    h.set_xcd_ranges(True)
     x = np.random.default_rng(12).integers(-3, 4, size=(HUB_N, 24)).astype(np.float32)
     xt = torch.from_numpy(x).to(cuda)
-    y = torch.empty(HUB_N, 24, device=cuda)
-    dx = torch.empty(HUB_N, 24, device=cuda)
-    h.forward(xt, y)
-    h.matmul_t(xt, dx)
-    assert np.array_equal(y.cpu().numpy().astype(np.float64), O.forward(a, x, "ax"))
-    assert np.array_equal(dx.cpu().numpy().astype(np.float64), O.adjoint(a, x, "ax"))
-    for depth in (None, 2):
-        h.set_kahan_depth(depth)
-        for name, m in mats.items():
-            hm = m.cusparse_extension
-            hm.set_xcd_ranges(True)
-            hm.set_kahan_depth(depth)
-            xf = torch.from_numpy(_x((HUB_N, 40), 7)).to(cuda)
-            y1 = torch.empty(HUB_N, 40, device=cuda)
-            y2 = torch.empty(HUB_N, 40, device=cuda)
-            hm.forward(xf, y1)
-            hm.delta_spmm(xf, y2)
-            hm.propagate(y2)
-            assert torch.equal(y1, y2), (name, depth)
-            hm.set_kahan_depth(32)
-            hm.set_xcd_ranges(None)
+    for ranges in (8, 4):
+        h.set_xcd_ranges(ranges)
+        y = torch.empty(HUB_N, 24, device=cuda)
+        dx = torch.empty(HUB_N, 24, device=cuda)
+        h.forward(xt, y)
+        h.matmul_t(xt, dx)
+        assert np.array_equal(y.cpu().numpy().astype(np.float64), O.forward(a, x, "ax")), ranges
+        assert np.array_equal(dx.cpu().numpy().astype(np.float64), O.adjoint(a, x, "ax")), ranges
+    for ranges in (8, 4):
+        for depth in (None, 2):
+            for name, m in mats.items():
+                hm = m.cusparse_extension
+                hm.set_xcd_ranges(ranges)
+                hm.set_kahan_depth(depth)
+                xf = torch.from_numpy(_x((HUB_N, 40), 7)).to(cuda)
+                y1 = torch.empty(HUB_N, 40, device=cuda)
+                y2 = torch.empty(HUB_N, 40, device=cuda)
+                hm.forward(xf, y1)
+                hm.delta_spmm(xf, y2)
+                hm.propagate(y2)
+                assert torch.equal(y1, y2), (name, depth, ranges)
+                hm.set_kahan_depth(32)
+                hm.set_xcd_ranges(None)
     h.set_kahan_depth(32)
     h.set_xcd_ranges(None)
 
 
 def test_xcd_ranges_active_reports_the_choice(hub, cuda):
-    """The query follows the setting: forced on and off for both storage
-    types; automatic off at this size (a gathered matrix under 24 MiB), and
-    the automatic choice is the one the forward takes (its launch count)."""
+    """The query follows the setting: forced sets, forced on (the 4-range set
+    once half a row is 256 bytes: F x element size >= 512) and off for both
+    storage types; automatic off at this size (a gathered matrix under
+    24 MiB), and the automatic choice is the one the forward takes (its
+    launch count)."""
     _, mats = hub
     h = mats["ax"].cusparse_extension
-    for mode, want in ((True, True), (False, False), (None, False)):
+    for mode, f, want32, want16 in ((True, 64, 8, 8), (True, 128, 4, 8), (True, 256, 4, 4), (8, 256, 8, 8),
+                                    (4, 64, 4, 4), (False, 256, 0, 0), (None, 256, 0, 0)):
         h.set_xcd_ranges(mode)
-        for dtype in (torch.float32, torch.bfloat16):
-            assert h.xcd_ranges_active(dtype, 64) is want, (mode, dtype)
+        assert h.xcd_ranges_active(torch.float32, f) == want32, (mode, f)
+        assert h.xcd_ranges_active(torch.bfloat16, f) == want16, (mode, f)
+    with pytest.raises(ValueError):
+        h.set_xcd_ranges(2)
     x = torch.randn(HUB_N, 64, device=cuda)
     y = torch.empty(HUB_N, 64, device=cuda)
     launches = {}
@@ -318,3 +330,31 @@ def test_xcd_ranges_active_reports_the_choice(hub, cuda):
     with pytest.raises(Exception):
         h.xcd_ranges_active(torch.float16, 64)
     h.set_xcd_ranges(None)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_xcd_ranges_with_column_slabs(hub, cuda, dtype):
+    """Column slabs over the range path: each slab's pieces write partial rows
+    of the slab's width and the reduction adds them per slab; forward and
+    adjoint stay within FP32 rounding of FP64, and the halves of the forward
+    still give the product's bits."""
+    a, mats = hub
+    for mode, m in mats.items():
+        h = m.cusparse_extension
+        h.set_xcd_ranges(True)
+        elem = 2 if dtype == torch.bfloat16 else 4
+        h.set_slab_bytes(HUB_N * 32 * elem)  # 96 columns in slabs of 32
+        xt = torch.from_numpy(_x((HUB_N, 96), 41)).to(cuda).to(dtype)
+        xe = xt.float().cpu().numpy()
+        y = torch.full((HUB_N, 96), float("nan"), device=cuda)
+        dx = torch.full((HUB_N, 96), float("nan"), device=cuda)
+        h.forward(xt, y)
+        h.matmul_t(xt, dx)
+        y2 = torch.full((HUB_N, 96), float("nan"), device=cuda)
+        h.delta_spmm(xt, y2)
+        h.propagate(y2)
+        h.set_slab_bytes(0)
+        h.set_xcd_ranges(None)
+        assert O.max_rel_error(y.cpu().numpy(), O.forward(a, xe, mode)) <= 1e-5, mode
+        assert O.max_rel_error(dx.cpu().numpy(), O.adjoint(a, xe, mode)) <= 1e-5, mode
+        assert torch.equal(y, y2), mode

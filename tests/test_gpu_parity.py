@@ -277,11 +277,11 @@ def test_reference_style_construction(graph1, cuda):
         h.matmul_AX_ADX(torch.from_numpy(x).to(cuda), y)
 
 
-@pytest.mark.parametrize("ranges", [None, True])
+@pytest.mark.parametrize("ranges", [None, 8, 4])
 def test_graph_capture_replays(mats, cuda, ranges):
     """Forward + adjoint captured in a HIP graph replay correctly several times,
     with eager work and synchronisations between the replays (with the XCD
-    column ranges as chosen by size, and forced on)."""
+    column ranges as chosen by size, and forced on with either piece set)."""
     m, _ = mats["dadx"]
     h = m.cusparse_extension
     h.set_xcd_ranges(ranges)

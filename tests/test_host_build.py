@@ -329,14 +329,15 @@ def test_reverse_chunk_tables_on_a_star():
 
 
 
-def _check_pieces(n, ptr, idx, cut, b, e, mid, slot, rng, rs_slot, rs_cnt, order=None, internal=None):
+def _check_pieces(n, ptr, idx, cut, b, e, mid, slot, rng, rs_slot, rs_cnt, order=None, internal=None,
+                  n_ranges=structure.XCD_RANGES):
     """Pieces tile every row exactly, in slot order, each inside one column
     range and within the piece size; grouped by range.  Backward (cut given):
     the pieces index `order`, each row's entries rewritten range by range,
     with a piece's leaf rows before mid and its internal rows from mid."""
-    width = (n + structure.XCD_RANGES - 1) // structure.XCD_RANGES
+    width = (n + n_ranges - 1) // n_ranges
     n_pc = len(b)
-    assert len(rng) == structure.XCD_RANGES + 1 and rng[0] == 0 and rng[-1] == n_pc
+    assert len(rng) == n_ranges + 1 and rng[0] == 0 and rng[-1] == n_pc
     assert np.all(np.diff(rng) >= 0)
     assert sorted(slot.tolist()) == list(range(n_pc))
     piece_of_slot = np.empty(n_pc, np.int64)
@@ -371,11 +372,19 @@ def test_range_pieces_tile_the_rows():
         assert pl.stats["fwd_pieces"] == len(pl.fpc_b) > 0 and pl.stats["bwd_pieces"] == len(pl.bpc_b) > 0
         par = b.parent.numpy()
         internal = np.bincount(par[par >= 0], minlength=n) > 0
-        _check_pieces(n, pl.fwd_ptr.numpy(), pl.fwd_col.numpy(), None, pl.fpc_b.numpy(), pl.fpc_e.numpy(), None,
-                      pl.fpc_slot.numpy(), pl.fpc_range.numpy(), pl.frs_slot.numpy(), pl.frs_cnt.numpy())
-        _check_pieces(n, pl.bwd_ptr.numpy(), pl.bwd_row.numpy(), pl.bwd_mid.numpy(), pl.bpc_b.numpy(),
-                      pl.bpc_e.numpy(), pl.bpc_mid.numpy(), pl.bpc_slot.numpy(), pl.bpc_range.numpy(),
-                      pl.brs_slot.numpy(), pl.brs_cnt.numpy(), pl.bpc_row.numpy(), internal)
+        for r, sfx in ((structure.XCD_RANGES, ""), (structure.XCD_RANGES_WIDE, "4")):
+            a = {k: pl.arrays[k.replace("_", sfx + "_", 1)].numpy() for k in (
+                "fpc_b", "fpc_e", "fpc_slot", "fpc_range", "frs_slot", "frs_cnt", "bpc_b", "bpc_e", "bpc_mid",
+                "bpc_slot", "bpc_range", "brs_slot", "brs_cnt", "bpc_row")}
+            assert pl.stats["fwd_pieces" + sfx] == len(a["fpc_b"]) > 0
+            assert pl.stats["bwd_pieces" + sfx] == len(a["bpc_b"]) > 0
+            _check_pieces(n, pl.fwd_ptr.numpy(), pl.fwd_col.numpy(), None, a["fpc_b"], a["fpc_e"], None,
+                          a["fpc_slot"], a["fpc_range"], a["frs_slot"], a["frs_cnt"], n_ranges=r)
+            _check_pieces(n, pl.bwd_ptr.numpy(), pl.bwd_row.numpy(), pl.bwd_mid.numpy(), a["bpc_b"], a["bpc_e"],
+                          a["bpc_mid"], a["bpc_slot"], a["bpc_range"], a["brs_slot"], a["brs_cnt"], a["bpc_row"],
+                          internal, n_ranges=r)
+        # fewer, wider ranges: no more pieces than the 8-range set
+        assert pl.stats["fwd_pieces4"] <= pl.stats["fwd_pieces"]
         return pl
 
     check(hub_graph(1500), 1500)
@@ -391,4 +400,6 @@ def test_no_range_pieces_above_the_size_limit():
     b = _build(ei.numpy(), n, 0)
     pl = structure.plan(n, b.csr_ptr, b.csr_col, b.csr_val, b.tree_src, b.tree_dst)
     assert pl.stats["fwd_pieces"] == 0 and pl.stats["bwd_pieces"] == 0
+    assert pl.stats["fwd_pieces4"] == 0 and pl.stats["bwd_pieces4"] == 0
     assert pl.fpc_b.numel() == 0 and pl.fpc_range.numel() == 0
+    assert pl.arrays["fpc4_b"].numel() == 0 and pl.arrays["bpc4_range"].numel() == 0

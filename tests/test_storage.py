@@ -125,3 +125,33 @@ def test_save_checks_its_input(built, tmp_path):
         _save(tmp_path / "x.npz", b, pl, n=2999)
     with pytest.raises(ValueError, match="delta_sign"):
         _save(tmp_path / "y.npz", b, pl, delta_sign=torch.zeros_like(b.csr_val))
+
+
+def test_version_1_files_load_without_the_four_range_pieces(built, tmp_path):
+    """A version 1 file (no 4-range piece set) loads with that set empty and
+    the rest of the layout unchanged."""
+    b, pl = built
+    path = tmp_path / "v2.npz"
+    _save(path, b, pl)
+    new = {"plan/" + k for k in storage._NEW_IN_2}
+
+    def as_v1(name, a):
+        if name in new:
+            return None
+        if name == "meta":
+            m = json.loads(bytes(a).decode())
+            m["version"] = 1
+            for k in storage._NEW_STATS_IN_2:
+                m["plan_stats"].pop(k)
+            for k in new:
+                m["crc32"].pop(k)
+            return np.frombuffer(json.dumps(m).encode(), dtype=np.uint8)
+        return a
+    _rewrite(path, tmp_path / "v1.npz", as_v1)
+    st = storage.load(tmp_path / "v1.npz")
+    assert st.plan.stats["fwd_pieces4"] == 0 and st.plan.stats["bwd_pieces4"] == 0
+    for k in structure.PLAN_NAMES:
+        if k in storage._NEW_IN_2:
+            assert st.plan.arrays[k].numel() == 0, k
+        else:
+            assert torch.equal(st.plan.arrays[k], pl.arrays[k]), k
