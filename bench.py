@@ -230,14 +230,16 @@ def measure(name, w, ei, m, args, dev, stream):
     # gathers by XCD column range (different rounding order): off, and each piece set
     xr = {}
     if h.plan_stats.get("fwd_pieces", 0) > 0:
-        for on in (False, 8, 4):
+        for on, phases in ((False, 1), (8, 1), (4, 1), (8, 2), (4, 2)):
             h.set_xcd_ranges(on)
-            xr[f"{on}_ranges" if on else "off"] = {
+            h.set_xcd_phases(phases)
+            xr[f"{on}_ranges_{phases}_phases" if on else "off"] = {
                 "step_bf16": round(time_loop(step(x16, g16), K, W, stream), 4),
                 "step_fp32": round(time_loop(step(x32, g32), K, W, stream), 4),
                 "forward": round(time_loop(lambda: h.forward(x16, y), K, W, stream), 4),
                 "adjoint_delta_t": round(time_loop(lambda: h.adjoint_delta_t(g16, z, dx), K, W, stream), 4)}
         h.set_xcd_ranges(None)
+        h.set_xcd_phases(None)
         log(f"{name}: XCD ranges {xr}")
     # results the CPU leg compares against FP64 (fresh, from the BF16 and FP32 steps)
     out = {}

@@ -324,6 +324,13 @@ class cusparseCBM:
             raise ValueError(f"set_xcd_ranges: None, a bool, 4 or 8, got {mode!r}")
         _lib.check_hip(self._lib.cbm_set_xcd_ranges(self._h, code), "set_xcd_ranges")
 
+    def set_xcd_phases(self, phases):
+        """Column phases of the XCD range passes: None automatic, 1, 2 or 4
+        (each XCD gathers its range one column slab after the other); the
+        results do not change."""
+        code = -1 if phases is None else int(phases)
+        _lib.check_hip(self._lib.cbm_set_xcd_phases(self._h, code), "set_xcd_phases")
+
     def xcd_ranges_active(self, dtype, n_cols):
         """The number of XCD column ranges (8 or 4) a call with dtype
         (torch.float32 or torch.bfloat16) storage and n_cols columns runs its

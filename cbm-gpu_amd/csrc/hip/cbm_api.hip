@@ -25,7 +25,8 @@ struct cbm_handle {
     int32_t fwd_split = -1;   // forward form: 0 fused per level, 1 split, -1 automatic
     int32_t wave_rows = -1;   // short-row passes: whole wavefront per row (1), never (0), by size (-1)
     int32_t xcd_slabs = 1;    // feature slabs spread over the XCDs (1 = off)
-    int32_t xcd_ranges = -1;  // column ranges per XCD: 1 on, 0 off, -1 by size
+    int32_t xcd_ranges = -1;  // column ranges per XCD: 1 on, 0 off, -1 by size, 8 / 4 that piece set
+    int32_t xcd_phases = -1;  // column phases of a range pass: 1, 2, 4, or -1 automatic
     // Column-range piece sets: [0] 8 ranges (one per XCD), [1] 4 ranges (each
     // on two XCDs, one column half each).
     struct RangeSet {
@@ -158,6 +159,21 @@ const RangeSet *pick_ranges(const cbm_handle *h, int32_t n_cols, int elem) {
 }
 
 int elem_of(int32_t dtype) { return dtype == CBM_BF16 ? 2 : 4; }
+
+// Column phases of a range pass with piece set rs: each XCD gathers its
+// range one column slab after the other, so its L2 holds a smaller working
+// set, at the price of narrower gathers and one more read of every index
+// per phase.  Automatic: 2 when an XCD's share of the gathered matrix (N x F
+// x elem / 8) is 12 MiB or more and a phase's gather stays 256 bytes or more
+// (N = 132,534, F = 256 FP32, 17 MiB per XCD: step 7.34 -> 6.21 ms with the
+// 4-range set; 4 phases, or 2 at 8.5 MiB per XCD (the BF16 step), measured
+// slower than 1).
+int32_t range_phases(const cbm_handle *h, const RangeSet &rs, int32_t n_cols, int elem) {
+    if (h->xcd_phases > 0) return h->xcd_phases;
+    const int64_t per_xcd = int64_t(h->d.n_rows) * n_cols * elem / 8;
+    const int64_t gather = int64_t(n_cols) * elem / (8 / rs.n_ranges * 2);
+    return per_xcd >= (int64_t(12) << 20) && gather >= 256 ? 2 : 1;
+}
 
 // The partial rows are sized for either storage type and either piece set,
 // so that a reserved handle serves BF16 and FP32 calls of this width without
@@ -328,11 +344,12 @@ int propagate_levels(cbm_handle *h, float *y, int64_t ldy, int32_t n_cols, hipSt
 }
 
 // Copies the n_ranges + 1 piece offsets of a range table into a pass.
-void set_ranges(cbmk::RowPass &rp, const int32_t *first, int32_t n_ranges) {
+void set_ranges(cbmk::RowPass &rp, const int32_t *first, int32_t n_ranges, int32_t phases) {
     rp.ranges = true;
     rp.range_count = n_ranges;
+    rp.range_phases = phases;
     for (int k = 0; k <= n_ranges; ++k) rp.range_first[k] = first[k];
-    rp.xcd_slabs = 8 / n_ranges;
+    rp.xcd_slabs = 8 / n_ranges * phases;
     rp.wave_rows = 0;
     rp.skip_above = INT32_MAX;
 }
@@ -356,7 +373,7 @@ int forward_ranges(cbm_handle *h, const RangeSet &rs, const cbmk::RowPass &base,
     pc.extra = cbmk::kExtraNone;
     pc.y = h->partial;
     pc.ldy = base.ncols;
-    set_ranges(pc, rs.f_first, rs.n_ranges);
+    set_ranges(pc, rs.f_first, rs.n_ranges, range_phases(h, rs, base.ncols, base.g_bf16 ? 2 : 4));
     hipError_t e = cbmk::launch_row_pass(pc, stream);
     if (e != hipSuccess) return device_error(e, what);
     ++h->last_launches;
@@ -730,7 +747,7 @@ int adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, co
         pc.rowscale = nullptr;
         pc.y = h->partial;
         pc.ldy = n_cols;
-        set_ranges(pc, rs->b_first, rs->n_ranges);
+        set_ranges(pc, rs->b_first, rs->n_ranges, range_phases(h, *rs, n_cols, elem_of(dtype)));
         hipError_t e = cbmk::launch_row_pass(pc, stream);
         if (e != hipSuccess) return device_error(e, "adjoint Delta^T pass");
         ++h->last_launches;
@@ -859,6 +876,15 @@ extern "C" int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode) {
         return CBM_ERR_ARG;
     }
     h->xcd_ranges = mode;
+    return CBM_OK;
+}
+
+extern "C" int cbm_set_xcd_phases(cbm_handle *h, int32_t phases) {
+    if (!h || !(phases == -1 || phases == 1 || phases == 2 || phases == 4)) {
+        set_error("cbm_set_xcd_phases: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    h->xcd_phases = phases;
     return CBM_OK;
 }
 

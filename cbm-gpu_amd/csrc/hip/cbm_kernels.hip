@@ -369,15 +369,16 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
     const int l = lane & (LPR - 1);
     const int sub = (lane / LPR) & (S - 1);
     // Workgroup b: slab b % xcd_slabs of row block b / xcd_slabs, or, with
-    // XCD ranges, row block b / 8 of range (b % 8) / S's pieces, slab
-    // (b % 8) % S (S = xcd_slabs = 8 / range_count).
+    // XCD ranges, the range, row block and slab of the mapping in RowPass.
     int32_t slab, p_begin = a.p_begin, p_end = a.p_end;
     int64_t blk;
     if (a.ranges) {
-        const uint32_t x = blockIdx.x & 7u, per = uint32_t(a.xcd_slabs);
+        const uint32_t x = blockIdx.x & 7u, per = 8u / uint32_t(a.range_count);
         const uint32_t k = x / per;
-        slab = int32_t(x % per);
-        blk = int64_t(blockIdx.x >> 3);
+        const int64_t b8 = int64_t(blockIdx.x >> 3);
+        const int64_t phase = b8 / a.range_blocks;
+        blk = b8 - phase * a.range_blocks;
+        slab = int32_t(int64_t(x % per) * a.range_phases + phase);
         p_begin = a.range_first[k];
         p_end = a.range_first[k + 1];
     } else {
@@ -642,14 +643,16 @@ template <class P, int VE, int LPR>
 hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
     constexpr int64_t kRowsPerBlock = 4 * (64 / LPR);
     if (lk.kind == Kind::kPass && rp.ranges) {
-        // One grid for all ranges: row block j of range k, slab s is
-        // workgroup 8 j + k S + s.
+        // One grid for all ranges and phases: 8 workgroups (one per XCD) per
+        // row block, all row blocks of a phase before the next phase's.
         int64_t most = 0;
         for (int k = 0; k < rp.range_count; ++k)
             most = std::max<int64_t>(most, int64_t(rp.range_first[k + 1]) - rp.range_first[k]);
         if (most <= 0) return hipSuccess;
-        k_row_pass<P, VE, LPR, 1>
-            <<<dim3(unsigned((most + kRowsPerBlock - 1) / kRowsPerBlock * 8)), dim3(256), 0, stream>>>(rp);
+        RowPass r = rp;
+        r.range_blocks = int32_t((most + kRowsPerBlock - 1) / kRowsPerBlock);
+        k_row_pass<P, VE, LPR, 1><<<dim3(unsigned(int64_t(r.range_blocks) * r.range_phases * 8)), dim3(256), 0,
+                                    stream>>>(r);
     } else if (lk.kind == Kind::kPass) {
         const int64_t n = int64_t(rp.p_end) - rp.p_begin;
         if (n <= 0) return hipSuccess;

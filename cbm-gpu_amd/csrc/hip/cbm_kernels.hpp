@@ -78,12 +78,18 @@ struct RowPass {
     int32_t slab_w = 0;
     // XCD column ranges: when set, the positions are pieces grouped by the
     // column range they gather from, range k of range_count (2, 4 or 8)
-    // holding [range_first[k], range_first[k + 1]).  With S = 8 / range_count,
-    // workgroup b works on range (b % 8) / S over feature slab (b % 8) % S, so
-    // each XCD gathers one slab of one range; p_begin and p_end are ignored
-    // and xcd_slabs must be S.
+    // holding [range_first[k], range_first[k + 1]).  With S = 8 / range_count
+    // XCDs per range and range_phases phases, the columns are cut into
+    // xcd_slabs = S * range_phases slabs; workgroup b works on range
+    // (b % 8) / S, row block (b / 8) % range_blocks, slab
+    // ((b % 8) % S) * range_phases + (b / 8) / range_blocks, so each XCD
+    // gathers one range and, phase after phase (workgroups are dispatched in
+    // order), one slab of it at a time.  p_begin and p_end are ignored;
+    // range_blocks is set by the launcher.
     bool ranges = false;
     int32_t range_count = 8;
+    int32_t range_phases = 1;
+    int32_t range_blocks = 0;
     int32_t range_first[9] = {};
 };
 

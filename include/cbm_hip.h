@@ -314,6 +314,18 @@ int cbm_set_xcd_slabs(cbm_handle *h, int32_t slabs);
 int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode);
 
 /*
+ * Column phases of the XCD range passes (-1 automatic, the default; 1, 2 or
+ * 4).  With P phases each XCD gathers its range one column slab (1/P of its
+ * columns) after the other -- the grid holds every row block of a phase
+ * before the next phase's -- so its L2 serves a P times smaller working set,
+ * at the price of P times narrower gathers and P reads of every index.
+ * Automatic: 2 when an XCD's share of the gathered matrix (N x F x element
+ * size / 8) is 12 MiB or more and a phase's gather stays 256 bytes or more,
+ * else 1.  Same sums in the same order as 1 phase: the bits do not change.
+ */
+int cbm_set_xcd_phases(cbm_handle *h, int32_t phases);
+
+/*
  * The number of column ranges (8 or 4) a dtype call of n_cols columns runs
  * its gather passes with under the handle's current setting, 0 when it runs
  * without, CBM_ERR_ARG for invalid arguments.  Reports the choice described

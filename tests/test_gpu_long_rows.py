@@ -262,6 +262,17 @@ def test_xcd_ranges_against_the_oracle(hub, star, cuda, mode, dtype, f):
             if ranges in out:
                 assert torch.equal(y, out[ranges][0]) and torch.equal(dx, out[ranges][1])
             out[ranges] = (y, dx)
+        # column phases change which workgroup sums which columns, not the sums
+        for ranges in (8, 4):
+            h.set_xcd_ranges(ranges)
+            for phases in (2, 4):
+                h.set_xcd_phases(phases)
+                y = torch.full((n, f), float("nan"), device=cuda)
+                dx = torch.full((n, f), float("nan"), device=cuda)
+                h.forward(xt, y)
+                h.matmul_t(xt, dx)
+                assert torch.equal(y, out[ranges][0]) and torch.equal(dx, out[ranges][1]), (ranges, phases)
+            h.set_xcd_phases(None)
         h.set_xcd_ranges(None)
         for ranges in (8, 4):
             y, dx = out[ranges]
@@ -319,6 +330,8 @@ def test_xcd_ranges_active_reports_the_choice(hub, cuda):
         assert h.xcd_ranges_active(torch.bfloat16, f) == want16, (mode, f)
     with pytest.raises(ValueError):
         h.set_xcd_ranges(2)
+    with pytest.raises(Exception):
+        h.set_xcd_phases(3)
     x = torch.randn(HUB_N, 64, device=cuda)
     y = torch.empty(HUB_N, 64, device=cuda)
     launches = {}
