@@ -128,9 +128,12 @@ class Adjacency:
             self.t_col = _copy_out(self._lib.cbm_adjacency_t_col(self._p), self.nnz, torch.int32)
 
     def close(self):
-        if self._p:
-            self._lib.cbm_adjacency_free(self._p)
-            self._p = ctypes.c_void_p()
+        # Also runs from __del__ at interpreter exit, when module globals
+        # (ctypes among them) may already be gone: touch none of them.
+        p = getattr(self, "_p", None)
+        if p:
+            self._p = None
+            self._lib.cbm_adjacency_free(p)
 
     def __enter__(self):
         return self

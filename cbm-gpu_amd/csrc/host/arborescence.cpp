@@ -106,7 +106,31 @@ ArborescenceResult min_arborescence(int32_t n, int32_t root, const int32_t *src,
     const size_t m = size_t(n_edges);
     LeftistForest heaps(w, m);
     std::vector<int32_t> heap(size_t(n), -1);
-    for (size_t e = 0; e < m; ++e) heap[size_t(dst[e])] = heaps.meld(heap[size_t(dst[e])], int32_t(e));
+    {
+        // Initial heaps, built in parallel: each vertex's entering edges as a
+        // chain in (weight, id) order, every edge the left child of the one
+        // before it -- a leftist heap whose right spines have length one.  A
+        // heap's shape never changes a result: pops return the unique least
+        // (reduced weight, id), whatever the melds made of the tree.
+        std::vector<int64_t> start(size_t(n) + 1, 0);
+        for (size_t e = 0; e < m; ++e) ++start[size_t(dst[e]) + 1];
+        for (int32_t v = 0; v < n; ++v) start[size_t(v) + 1] += start[size_t(v)];
+        std::vector<int32_t> order(m);
+        {
+            std::vector<int64_t> fill(start.begin(), start.end() - 1);
+            for (size_t e = 0; e < m; ++e) order[size_t(fill[size_t(dst[e])]++)] = int32_t(e);
+        }
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int32_t v = 0; v < n; ++v) {
+            const int64_t b = start[size_t(v)], e = start[size_t(v) + 1];
+            if (b == e) continue;
+            std::sort(order.begin() + b, order.begin() + e, [&](int32_t x, int32_t y) {
+                return w[size_t(x)] != w[size_t(y)] ? w[size_t(x)] < w[size_t(y)] : x < y;
+            });
+            for (int64_t k = b; k + 1 < e; ++k) heaps.left[size_t(order[size_t(k)])] = order[size_t(k) + 1];
+            heap[size_t(v)] = order[size_t(b)];
+        }
+    }
 
     UndoUnionFind uf(n);
     std::vector<int32_t> walk_mark(size_t(n), -1);  // start vertex of the walk holding this super-node

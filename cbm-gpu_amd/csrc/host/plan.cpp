@@ -391,8 +391,42 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
         // Backward: Delta^T (the CSC of Delta), columns ordered long first,
         // then short, ascending within each part; rows ascending per column.
         {
-            std::vector<int32_t> ccount(size_t(n), 0);
-            for (int64_t k = 0; k < nnz; ++k) ++ccount[size_t(col[k])];
+            // Rows of leaves first, then rows of internal nodes: the adjoint
+            // reads a leaf's term from G and an internal node's from Z, and
+            // this split keeps both loops free of per-entry branches.
+            auto internal = [&](int32_t r) { return kptr[size_t(r) + 1] > kptr[size_t(r)]; };
+            // A parallel transpose that writes what a serial one would: the
+            // rows are cut into T chunks of about nnz / T entries; chunk t
+            // counts its leaf and internal entries per column, and its
+            // entries of column j go after those of chunks 0 .. t-1 in each
+            // part, so every column still lists its rows in ascending order.
+            const int T = std::max(1, omp_get_max_threads());
+            std::vector<int32_t> row_cut(size_t(T) + 1, n);
+            row_cut[0] = 0;
+            for (int t = 1; t < T; ++t)
+                row_cut[size_t(t)] = int32_t(std::upper_bound(crow, crow + n + 1, int64_t(nnz) * t / T) - crow) - 1;
+            for (int t = 1; t <= T; ++t) row_cut[size_t(t)] = std::max(row_cut[size_t(t)], row_cut[size_t(t) - 1]);
+            // cnt[(2 t + part) n + j]: entries of column j in chunk t (part 0 leaf rows, 1 internal rows)
+            std::vector<int32_t> cnt(size_t(2 * T) * size_t(n), 0);
+#pragma omp parallel for schedule(static, 1)
+            for (int t = 0; t < T; ++t) {
+                int32_t *leaf = cnt.data() + size_t(2 * t) * size_t(n), *inner = leaf + n;
+                for (int32_t r = row_cut[size_t(t)]; r < row_cut[size_t(t) + 1]; ++r) {
+                    int32_t *c = internal(r) ? inner : leaf;
+                    for (int32_t k = crow[r]; k < crow[r + 1]; ++k) ++c[size_t(col[k])];
+                }
+            }
+            std::vector<int32_t> ccount(size_t(n), 0), cleaf(size_t(n), 0);
+#pragma omp parallel for schedule(static)
+            for (int32_t j = 0; j < n; ++j) {
+                int32_t all = 0, lf = 0;
+                for (int t = 0; t < T; ++t) {
+                    lf += cnt[size_t(2 * t) * size_t(n) + size_t(j)];
+                    all += cnt[size_t(2 * t) * size_t(n) + size_t(j)] + cnt[size_t(2 * t + 1) * size_t(n) + size_t(j)];
+                }
+                ccount[size_t(j)] = all;
+                cleaf[size_t(j)] = lf;
+            }
             auto &bnode = plan->arr[CBM_P_BWD_NODE];
             bnode.resize(size_t(n));
             int32_t n_long = 0;
@@ -408,24 +442,34 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
             bptr.assign(size_t(n) + 1, 0);
             for (int32_t p = 0; p < n; ++p) bptr[size_t(p) + 1] = bptr[size_t(p)] + ccount[size_t(bnode[size_t(p)])];
             brow.resize(size_t(nnz));
-            // Rows of leaves first, then rows of internal nodes: the adjoint
-            // reads a leaf's term from G and an internal node's from Z, and
-            // this split keeps both loops free of per-entry branches.
-            auto internal = [&](int32_t r) { return kptr[size_t(r) + 1] > kptr[size_t(r)]; };
-            std::vector<int32_t> n_leaf(static_cast<size_t>(n), 0);
-            for (int32_t r = 0; r < n; ++r)
-                if (!internal(r))
-                    for (int32_t k = crow[r]; k < crow[r + 1]; ++k) ++n_leaf[size_t(pos_of[size_t(col[k])])];
             auto &bmid = plan->arr[CBM_P_BWD_MID];
             bmid.resize(size_t(n));
-            std::vector<int32_t> fill_leaf(bptr.begin(), bptr.end() - 1), fill_int(static_cast<size_t>(n));
-            for (int32_t p = 0; p < n; ++p) bmid[size_t(p)] = fill_int[size_t(p)] = bptr[size_t(p)] + n_leaf[size_t(p)];
-            for (int32_t r = 0; r < n; ++r) {
-                auto &fill = internal(r) ? fill_int : fill_leaf;
-                for (int32_t k = crow[r]; k < crow[r + 1]; ++k)
-                    brow[size_t(fill[size_t(pos_of[size_t(col[k])])]++)] =
-                        int32_t(uint32_t(r) | (val[k] < 0.0f ? kSignBit : 0u));
+            for (int32_t p = 0; p < n; ++p) bmid[size_t(p)] = bptr[size_t(p)] + cleaf[size_t(bnode[size_t(p)])];
+            // The counts become each chunk's first slot per column and part.
+#pragma omp parallel for schedule(static)
+            for (int32_t j = 0; j < n; ++j) {
+                const int32_t p = pos_of[size_t(j)];
+                int32_t at_leaf = bptr[size_t(p)], at_int = bmid[size_t(p)];
+                for (int t = 0; t < T; ++t) {
+                    int32_t &lf = cnt[size_t(2 * t) * size_t(n) + size_t(j)];
+                    int32_t &in = cnt[size_t(2 * t + 1) * size_t(n) + size_t(j)];
+                    const int32_t nl = lf, ni = in;
+                    lf = at_leaf;
+                    in = at_int;
+                    at_leaf += nl;
+                    at_int += ni;
+                }
             }
+#pragma omp parallel for schedule(static, 1)
+            for (int t = 0; t < T; ++t) {
+                int32_t *leaf = cnt.data() + size_t(2 * t) * size_t(n), *inner = leaf + n;
+                for (int32_t r = row_cut[size_t(t)]; r < row_cut[size_t(t) + 1]; ++r) {
+                    int32_t *at = internal(r) ? inner : leaf;
+                    for (int32_t k = crow[r]; k < crow[r + 1]; ++k)
+                        brow[size_t(at[size_t(col[k])]++)] = int32_t(uint32_t(r) | (val[k] < 0.0f ? kSignBit : 0u));
+                }
+            }
+            std::vector<int32_t>().swap(cnt);
             Split b;
             plan->stat[CBM_PS_BWD_SLOTS] = split_long_rows(bptr, 0, n_long, b);
             plan->stat[CBM_PS_BWD_NLONG] = n_long;
