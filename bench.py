@@ -106,7 +106,6 @@ def time_loop(fn, steps, warmup, stream):
 
 
 def build_workload(name, alpha, cols, dev):
-    import torch
     from cbm import structure
     w = dict(WORKLOADS[name])
     if alpha is not None:
@@ -116,6 +115,34 @@ def build_workload(name, alpha, cols, dev):
     t0 = time.perf_counter()
     ei = structure.copying_model(w["n"], w["blocks"], w["p"], w["lam"], w["seed_deg"], w["seed"])
     log(f"graph {name}: N={w['n']} nnz={ei.shape[1]} ({time.perf_counter() - t0:.1f}s)")
+    return _build_cbm(w, ei, dev)
+
+
+def read_graph(path):
+    """A local graph file: MatrixMarket (.mtx), numpy edge_index (.npy /
+    .npz), or a whitespace-separated edge list (anything else)."""
+    from cbm import datasets
+    if path.endswith(".mtx"):
+        return datasets.read_matrix_market(path)
+    if path.endswith((".npy", ".npz")):
+        return datasets.read_edge_index(path)
+    return datasets.read_edge_list(path)
+
+
+def file_workload(path, product, alpha, cols, dev):
+    """A workload over a graph read from a local file, with the reference
+    harness's defaults (alpha = 32, F = 512: scripts/matmul.sh:9-17,
+    benchmark/benchmark_matmul_cuda.py:16)."""
+    t0 = time.perf_counter()
+    ei, n = read_graph(path)
+    log(f"graph {path}: N={n} nnz={ei.shape[1]} ({time.perf_counter() - t0:.1f}s)")
+    w = {"n": n, "alpha": 32 if alpha is None else alpha, "cols": 512 if cols is None else cols,
+         "product": product or "mm"}
+    return _build_cbm(w, ei, dev)
+
+
+def _build_cbm(w, ei, dev):
+    import torch
     t0 = time.perf_counter()
     m = _class(CBM_CLASSES, w["product"])(ei, torch.ones(ei.shape[1]), alpha=w["alpha"], device=dev)
     w["build_s"] = round(time.perf_counter() - t0, 2)
@@ -441,7 +468,13 @@ def main():
     ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
     ap.add_argument("--alpha", type=int, default=None)
     ap.add_argument("--cols", type=int, default=None)
-    ap.add_argument("--secondary", default="c4", help="second workload reported under 'secondary' ('' for none)")
+    ap.add_argument("--secondary", default=None,
+                    help="second workload reported under 'secondary' ('' for none; default c4, none with --graph)")
+    ap.add_argument("--graph", default=None, metavar="FILE",
+                    help="measure a local graph file (MatrixMarket .mtx, numpy edge_index .npy/.npz, or an edge "
+                         "list) instead of a synthetic workload; alpha defaults to 32 and F to 512 there")
+    ap.add_argument("--product", default=None, choices=sorted(CBM_CLASSES),
+                    help="the product measured on a --graph file (default mm)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--gcn", action="store_true",
                     help="time the 2-layer GCN training step (c3 graph, hidden 256) instead; CBM vs rocSPARSE CSR")
@@ -475,21 +508,31 @@ def main():
     if args.gcn:
         gcn_step(args, dev, stream)
         return
+    name = os.path.basename(args.graph) if args.graph else args.workload
+    if args.secondary is None:
+        args.secondary = "" if args.graph else "c4"
     if args.build_only:
         from cbm import structure
-        w = dict(WORKLOADS[args.workload])
-        if args.alpha is not None:
-            w["alpha"] = args.alpha
-        ei = structure.copying_model(w["n"], w["blocks"], w["p"], w["lam"], w["seed_deg"], w["seed"])
+        if args.graph:
+            ei, n = read_graph(args.graph)
+            w = {"n": n, "alpha": 32 if args.alpha is None else args.alpha}
+        else:
+            w = dict(WORKLOADS[args.workload])
+            if args.alpha is not None:
+                w["alpha"] = args.alpha
+            ei = structure.copying_model(w["n"], w["blocks"], w["p"], w["lam"], w["seed_deg"], w["seed"])
         stages = build_stages(ei, w["n"], w["alpha"], dev, stream)
-        print(json.dumps({"workload": args.workload, "n": w["n"], "nnz": int(ei.shape[1]), "alpha": w["alpha"],
+        print(json.dumps({"workload": name, "n": w["n"], "nnz": int(ei.shape[1]), "alpha": w["alpha"],
                           "build_stages": stages}), flush=True)
         return
-    w, ei, m = build_workload(args.workload, args.alpha, args.cols, dev)
+    if args.graph:
+        w, ei, m = file_workload(args.graph, args.product, args.alpha, args.cols, dev)
+    else:
+        w, ei, m = build_workload(args.workload, args.alpha, args.cols, dev)
     if args.profile:
         profile_steps(w, m, args, dev, stream)
         return
-    rec, gpu_out, inputs, m_csr = measure(args.workload, w, ei, m, args, dev, stream)
+    rec, gpu_out, inputs, m_csr = measure(name, w, ei, m, args, dev, stream)
     rec["build_s"] = w["build_s"]
     rec["build_stages"] = build_stages(ei, w["n"], w["alpha"], dev, stream)
     cpu = None
@@ -501,7 +544,7 @@ def main():
     del gpu_out, inputs, m, m_csr
 
     secondary = None
-    if args.secondary and args.secondary != args.workload:
+    if args.secondary and args.secondary != name:
         w2, ei2, m2 = build_workload(args.secondary, None, None, dev)
         secondary, out2, in2, m_csr2 = measure(args.secondary, w2, ei2, m2, args, dev, stream)
         secondary["build_s"] = w2["build_s"]
@@ -512,7 +555,7 @@ def main():
 
     ms16 = rec["ms_step_bf16"]
     dom = rec["dominant_phase"]
-    traffic = measured_traffic(args.workload, dom)
+    traffic = None if args.graph else measured_traffic(args.workload, dom)
     out = {
         "metric": "A@X fwd+bwd ms & HBM GB/s (% roofline), CBM-HIP vs CSR on-GPU, FP32 & BF16",
         "value": ms16,
@@ -525,7 +568,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16-storage/f32-accumulate",
-        "data": "synthetic (seeded community copying model; X, G ~ N(0,1))",
+        "data": (f"graph file {name}; X, G ~ N(0,1)" if args.graph
+                 else "synthetic (seeded community copying model; X, G ~ N(0,1))"),
         "config": {"workload": rec["workload"], "nnz_delta": rec["nnz_delta"], "depth": rec["depth"],
                    "compression": rec["compression"]},
         "roofline": {"bound": "hbm", "achieved": rec["dominant_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
