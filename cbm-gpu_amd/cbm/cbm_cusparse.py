@@ -131,6 +131,9 @@ class cusparseCBM:
                        bwd_slots=st["bwd_slots"], fwd_gslots=st["fwd_gslots"], rev_slots=st["rev_slots"],
                        fwd_pieces=st.get("fwd_pieces", 0), bwd_pieces=st.get("bwd_pieces", 0),
                        fwd_pieces4=st.get("fwd_pieces4", 0), bwd_pieces4=st.get("bwd_pieces4", 0),
+                       tf_level=st.get("tf_level", self.depth), tr_level=st.get("tr_level", self.depth),
+                       tf_frags=max(0, int(self._dev["tf_frag"].numel()) - 1),
+                       tr_frags=max(0, int(self._dev["tr_frag"].numel()) - 1),
                        d=self._d.data_ptr() if self._d is not None else None, **ptrs)
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
@@ -323,6 +326,11 @@ class cusparseCBM:
         else:
             raise ValueError(f"set_xcd_ranges: None, a bool, 4 or 8, got {mode!r}")
         _lib.check_hip(self._lib.cbm_set_xcd_ranges(self._h, code), "set_xcd_ranges")
+
+    def set_tree_tails(self, on):
+        """Tree tails in one launch each (True, the default) or per level
+        (False); the forward's bits do not change."""
+        _lib.check_hip(self._lib.cbm_set_tree_tails(self._h, int(bool(on))), "set_tree_tails")
 
     def set_xcd_phases(self, phases):
         """Column phases of the XCD range passes: None automatic, 1, 2 or 4

@@ -18,9 +18,10 @@ _PLAN_NAMES = ("lvl_ptr", "lvl_node", "lvl_parent", "fwd_ptr", "fwd_col", "bwd_p
                "rred_ptr", "rred_pos", "rred_slot", "rred_cnt", "fpc_b", "fpc_e", "fpc_slot", "fpc_range", "frs_slot",
                "frs_cnt", "bpc_b", "bpc_e", "bpc_mid", "bpc_slot", "bpc_range", "brs_slot", "brs_cnt",
                "bpc_row", "fpc4_b", "fpc4_e", "fpc4_slot", "fpc4_range", "frs4_slot", "frs4_cnt", "bpc4_b", "bpc4_e",
-               "bpc4_mid", "bpc4_slot", "bpc4_range", "brs4_slot", "brs4_cnt", "bpc4_row")
+               "bpc4_mid", "bpc4_slot", "bpc4_range", "brs4_slot", "brs4_cnt", "bpc4_row", "tf_frag", "tf_seg", "tf_pos",
+               "tr_frag", "tr_seg", "tr_pos")
 _PLAN_STATS = ("depth", "fwd_slots", "bwd_slots", "bwd_nlong", "fwd_gslots", "rev_slots", "fwd_pieces", "bwd_pieces",
-               "fwd_pieces4", "bwd_pieces4")
+               "fwd_pieces4", "bwd_pieces4", "tf_level", "tr_level")
 # CBM_XCD_RANGES, CBM_XCD_RANGES_WIDE, CBM_PIECE, CBM_RANGE_MAX_ROWS
 XCD_RANGES, XCD_RANGES_WIDE, PIECE, RANGE_MAX_ROWS = 8, 4, 256, 524288
 PLAN_NAMES = _PLAN_NAMES

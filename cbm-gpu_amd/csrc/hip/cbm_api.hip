@@ -27,6 +27,7 @@ struct cbm_handle {
     int32_t xcd_slabs = 1;    // feature slabs spread over the XCDs (1 = off)
     int32_t xcd_ranges = -1;  // column ranges per XCD: 1 on, 0 off, -1 by size, 8 / 4 that piece set
     int32_t xcd_phases = -1;  // column phases of a range pass: 1, 2, 4, or -1 automatic
+    int32_t tree_tails = 1;   // tree tails in one launch each (1) or per level (0)
     // Column-range piece sets: [0] 8 ranges (one per XCD), [1] 4 ranges (each
     // on two XCDs, one column half each).
     struct RangeSet {
@@ -322,8 +323,11 @@ int delta_flat(cbm_handle *h, cbmk::RowPass rp, hipStream_t stream, const char *
     return run_range(h, rp, 0, h->d.n_rows, 0, ck, rd, stream, what);
 }
 
+bool tails_on(const cbm_handle *h) { return h->tree_tails != 0; }
+
 // In-place propagation of levels 1 .. L-1, y_v += s_v y_parent(v), with the
-// compensated form from the handle's Kahan level on.
+// compensated form from the handle's Kahan level on: one launch per level
+// down to the tree tail, then one launch for the tail.
 int propagate_levels(cbm_handle *h, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream, const char *what) {
     const float *ps = h->d.mode == CBM_MODE_DADX ? h->ps_pos : nullptr;
     cbmk::CompArgs cm;
@@ -331,12 +335,26 @@ int propagate_levels(cbm_handle *h, float *y, int64_t ldy, int32_t n_cols, hipSt
     cm.comp = h->comp;
     cm.base = comp_base(h);
     cm.ldc = n_cols;
-    for (int32_t l = 1; l < h->d.n_levels; ++l) {
+    const int32_t L = h->d.n_levels;
+    const bool tail = tails_on(h) && h->d.tf_level < L;
+    for (int32_t l = 1; l < (tail ? h->d.tf_level : L); ++l) {
         const int32_t b = h->lvl_ptr[size_t(l)], e = h->lvl_ptr[size_t(l) + 1];
         const hipError_t err =
             kahan_at(h, l) ? cbmk::launch_propagate_comp(b, e, h->d.lvl_node, h->d.lvl_parent, ps, cm, y, ldy, n_cols,
                                                          stream)
                            : cbmk::launch_propagate(b, e, h->d.lvl_node, h->d.lvl_parent, ps, y, ldy, n_cols, stream);
+        if (err != hipSuccess) return device_error(err, what);
+        ++h->last_launches;
+    }
+    if (tail) {
+        cbmk::TailArgs t;
+        t.frag = h->d.tf_frag;
+        t.seg = h->d.tf_seg;
+        t.pos = h->d.tf_pos;
+        t.n_frag = h->d.tf_frags;
+        const bool comp = h->kahan_level >= 0 && std::max<int32_t>(h->kahan_level, 1) < L;
+        const hipError_t err = cbmk::launch_propagate_tail(t, h->d.lvl_node, h->d.lvl_parent, ps, cm,
+                                                           comp ? cm.base : INT32_MAX, y, ldy, n_cols, stream);
         if (err != hipSuccess) return device_error(err, what);
         ++h->last_launches;
     }
@@ -495,6 +513,21 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
         (d.rred_ptr_host[L] > 0 && (!d.rred_pos || !d.rred_slot || !d.rred_cnt))) {
         set_error("cbm_create: missing forward chunk tables");
         return CBM_ERR_ARG;
+    }
+    for (int k = 0; k < 2; ++k) {
+        const int32_t level = k == 0 ? d.tf_level : d.tr_level, frags = k == 0 ? d.tf_frags : d.tr_frags;
+        const int32_t *arrs[3] = {k == 0 ? d.tf_frag : d.tr_frag, k == 0 ? d.tf_seg : d.tr_seg,
+                                  k == 0 ? d.tf_pos : d.tr_pos};
+        const bool bad_level = level < (k == 0 ? 1 : 0) || level > L;
+        const bool bad_tables = level < L && (frags < 1 || !arrs[0] || !arrs[1] || !arrs[2]);
+        bool long_lists = false;  // the reverse tail holds no long child lists (they take the chunk kernels)
+        if (k == 1 && !bad_level)
+            for (int32_t l = level; l < L; ++l) long_lists = long_lists || d.rev_nlong_host[l] > 0;
+        if (bad_level || bad_tables || long_lists) {
+            set_error("cbm_create: invalid %s tree tail (level %d of %d, %d fragments)", k == 0 ? "forward" : "reverse",
+                      level, L, frags);
+            return CBM_ERR_ARG;
+        }
     }
     RangeSet sets[2];
     {
@@ -681,7 +714,19 @@ int adjoint_reverse(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, fl
     rd.slot = h->d.rred_slot;
     rd.cnt = h->d.rred_cnt;
     rd.scratch = h->scratch;
-    for (int32_t l = h->d.n_levels - 1; l >= 0; --l) {
+    int32_t first_level = h->d.n_levels;  // levels from here down run as the tail
+    if (tails_on(h) && h->d.tr_level < h->d.n_levels) {
+        cbmk::TailArgs t;
+        t.frag = h->d.tr_frag;
+        t.seg = h->d.tr_seg;
+        t.pos = h->d.tr_pos;
+        t.n_frag = h->d.tr_frags;
+        const hipError_t err = cbmk::launch_row_tail(rv, t, stream);
+        if (err != hipSuccess) return device_error(err, "adjoint reverse pass (tail)");
+        ++h->last_launches;
+        first_level = h->d.tr_level;
+    }
+    for (int32_t l = first_level - 1; l >= 0; --l) {
         const int32_t b = h->rev_ptr[size_t(l)], e = h->rev_ptr[size_t(l) + 1];
         if (e <= b) continue;
         // A level's launch holds only its own nodes: a thin level gives each
@@ -876,6 +921,15 @@ extern "C" int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode) {
         return CBM_ERR_ARG;
     }
     h->xcd_ranges = mode;
+    return CBM_OK;
+}
+
+extern "C" int cbm_set_tree_tails(cbm_handle *h, int32_t mode) {
+    if (!h || mode < 0 || mode > 1) {
+        set_error("cbm_set_tree_tails: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    h->tree_tails = mode;
     return CBM_OK;
 }
 

@@ -415,6 +415,38 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
     }
 }
 
+// A row pass over a tree tail: workgroup f finishes fragment f's segments,
+// deepest first, one group of LPR lanes per row and a workgroup barrier
+// between segments (the rows of one segment read those of the next deeper
+// one, written by this workgroup).
+template <class P, int VE, int LPR>
+__global__ __launch_bounds__(256) void k_row_tail(const RowPass a, const TailArgs t) {
+    constexpr int kGroups = 256 / LPR;
+    const int lane = int(threadIdx.x & 63u);
+    const int l = lane & (LPR - 1);
+    const int32_t group = int32_t(threadIdx.x / LPR);
+    const int32_t f = int32_t(blockIdx.x);
+    constexpr int32_t kSpan = LPR * VE;
+    const int32_t passes = (a.ncols + kSpan - 1) / kSpan;
+    for (int32_t s = t.frag[f + 1] - 1; s >= t.frag[f]; --s) {
+        for (int32_t i = t.seg[s] + group; i < t.seg[s + 1]; i += kGroups) {
+            const int32_t p = t.pos[i];
+            const int32_t kb = a.ptr[p], ke = a.ptr_end ? a.ptr_end[p] : a.ptr[p + 1];
+            const int32_t row = a.node_of ? a.node_of[p] : p;
+            for (int32_t pass = 0; pass < passes; ++pass) {
+                const int32_t c0 = pass * kSpan + l * VE;
+                const bool active = c0 < a.ncols;
+                float acc[VE];
+#pragma unroll
+                for (int q = 0; q < VE; ++q) acc[q] = 0.0f;
+                gather_sum<P, VE, LPR>(a, p, kb, ke, l, c0, active, acc);
+                if (active) finish_row<P, VE>(a, p, row, c0, acc);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Long rows: one workgroup of BS threads per chunk of a row.  The chunk's
 // entries are split into contiguous slices, one per group of LPR lanes; the
 // groups' partial sums are added in group order through LDS.  A one-chunk
@@ -505,19 +537,12 @@ __global__ __launch_bounds__(256) void k_row_reduce(const RowPass a, const Reduc
     }
 }
 
+// y[node_of[p]] += parscale[p] * y[parent_of[p]], the columns of lane l of
+// a group of LPR lanes.
 template <int VE, int LPR>
-__global__ __launch_bounds__(256) void k_propagate(int32_t p_begin, int32_t p_end, const int32_t *__restrict__ node_of,
-                                                   const int32_t *__restrict__ parent_of,
-                                                   const float *__restrict__ parscale, float *y, int64_t ldy,
-                                                   int32_t ncols) {
-    constexpr int kGroupsPerWave = 64 / LPR;
-    const int lane = int(threadIdx.x & 63u);
-    const int l = lane & (LPR - 1);
-    const int64_t group =
-        (int64_t(blockIdx.x) * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kGroupsPerWave + lane / LPR;
-    const int64_t p64 = int64_t(p_begin) + group;
-    if (p64 >= int64_t(p_end)) return;
-    const int32_t p = int32_t(p64);
+__device__ inline void propagate_row(int32_t p, int l, const int32_t *__restrict__ node_of,
+                                     const int32_t *__restrict__ parent_of, const float *__restrict__ parscale,
+                                     float *y, int64_t ldy, int32_t ncols) {
     const int32_t par = parent_of[p];
     if (par < 0) return;
     const int32_t row = node_of[p];
@@ -533,6 +558,26 @@ __global__ __launch_bounds__(256) void k_propagate(int32_t p_begin, int32_t p_en
 }
 
 template <int VE, int LPR>
+__device__ inline void propagate_row_comp(int32_t p, int l, const int32_t *__restrict__ node_of,
+                                          const int32_t *__restrict__ parent_of, const float *__restrict__ parscale,
+                                          const CompArgs &c, float *y, int64_t ldy, int32_t ncols);
+
+template <int VE, int LPR>
+__global__ __launch_bounds__(256) void k_propagate(int32_t p_begin, int32_t p_end, const int32_t *__restrict__ node_of,
+                                                   const int32_t *__restrict__ parent_of,
+                                                   const float *__restrict__ parscale, float *y, int64_t ldy,
+                                                   int32_t ncols) {
+    constexpr int kGroupsPerWave = 64 / LPR;
+    const int lane = int(threadIdx.x & 63u);
+    const int l = lane & (LPR - 1);
+    const int64_t group =
+        (int64_t(blockIdx.x) * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kGroupsPerWave + lane / LPR;
+    const int64_t p64 = int64_t(p_begin) + group;
+    if (p64 >= int64_t(p_end)) return;
+    propagate_row<VE, LPR>(int32_t(p64), l, node_of, parent_of, parscale, y, ldy, ncols);
+}
+
+template <int VE, int LPR>
 __global__ __launch_bounds__(256) void k_propagate_comp(int32_t p_begin, int32_t p_end,
                                                         const int32_t *__restrict__ node_of,
                                                         const int32_t *__restrict__ parent_of,
@@ -545,7 +590,15 @@ __global__ __launch_bounds__(256) void k_propagate_comp(int32_t p_begin, int32_t
         (int64_t(blockIdx.x) * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kGroupsPerWave + lane / LPR;
     const int64_t p64 = int64_t(p_begin) + group;
     if (p64 >= int64_t(p_end)) return;
-    const int32_t p = int32_t(p64);
+    propagate_row_comp<VE, LPR>(int32_t(p64), l, node_of, parent_of, parscale, c, y, ldy, ncols);
+}
+
+// The compensated form: (y, comp) of position p from the parent's y and
+// comp rows; positions from c.base on own a comp row.
+template <int VE, int LPR>
+__device__ inline void propagate_row_comp(int32_t p, int l, const int32_t *__restrict__ node_of,
+                                          const int32_t *__restrict__ parent_of, const float *__restrict__ parscale,
+                                          const CompArgs &c, float *y, int64_t ldy, int32_t ncols) {
     const int32_t par = parent_of[p];
     const int32_t row = node_of[p];
     const float es = parscale ? parscale[p] : 1.0f;
@@ -568,6 +621,33 @@ __global__ __launch_bounds__(256) void k_propagate_comp(int32_t p_begin, int32_t
             store_row<VE>(y + int64_t(row) * ldy + c0, acc);
         }
         store_row<VE>(c.comp + int64_t(p - c.base) * c.ldc + c0, comp);
+    }
+}
+
+// The forward propagation of a tree tail: workgroup f finishes fragment f's
+// segments, root first, one group of LPR lanes per row and a workgroup
+// barrier between segments; positions from comp_first on take the
+// compensated form.
+template <int VE, int LPR>
+__global__ __launch_bounds__(256) void k_propagate_tail(const TailArgs t, const int32_t *__restrict__ node_of,
+                                                        const int32_t *__restrict__ parent_of,
+                                                        const float *__restrict__ parscale, const CompArgs c,
+                                                        int32_t comp_first, float *y, int64_t ldy, int32_t ncols) {
+    constexpr int kGroups = 256 / LPR;
+    const int lane = int(threadIdx.x & 63u);
+    const int l = lane & (LPR - 1);
+    const int32_t group = int32_t(threadIdx.x / LPR);
+    const int32_t f = int32_t(blockIdx.x);
+    for (int32_t s = t.frag[f]; s < t.frag[f + 1]; ++s) {
+        for (int32_t i = t.seg[s] + group; i < t.seg[s + 1]; i += kGroups) {
+            const int32_t p = t.pos[i];
+            if (p >= comp_first) {
+                propagate_row_comp<VE, LPR>(p, l, node_of, parent_of, parscale, c, y, ldy, ncols);
+            } else {
+                propagate_row<VE, LPR>(p, l, node_of, parent_of, parscale, y, ldy, ncols);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -622,12 +702,13 @@ bool aligned16(const void *p, int64_t ld, int elem) {
     return (reinterpret_cast<uintptr_t>(p) & 15u) == 0 && ((ld * elem) & 15) == 0;
 }
 
-enum class Kind { kPass, kChunk, kReduce };
+enum class Kind { kPass, kChunk, kReduce, kTail };
 
 struct Launch {
     Kind kind;
     const ChunkArgs *chunk;
     const ReduceArgs *reduce;
+    const TailArgs *tail = nullptr;
 };
 
 // Whether a short-row pass gives each row a whole wavefront (see RowPass).
@@ -665,6 +746,13 @@ hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
         }
         k_row_pass<P, VE, LPR, 1>
             <<<dim3(unsigned((n + kRowsPerBlock - 1) / kRowsPerBlock * ns)), dim3(256), 0, stream>>>(rp);
+    } else if (lk.kind == Kind::kTail) {
+        if constexpr (P::kDual && P::kExtra == kExtraSelf) {  // the reverse pass only
+            if (lk.tail->n_frag <= 0) return hipSuccess;
+            k_row_tail<P, VE, LPR><<<dim3(unsigned(lk.tail->n_frag)), dim3(256), 0, stream>>>(rp, *lk.tail);
+        } else {
+            return hipErrorInvalidValue;
+        }
     } else if (lk.kind == Kind::kChunk) {
         if (lk.chunk->count <= 0) return hipSuccess;
         constexpr int kBs = (16 * LPR > 256) ? 16 * LPR : 256;
@@ -780,6 +868,34 @@ hipError_t launch_row_chunks(const RowPass &rp, const ChunkArgs &c, hipStream_t 
 
 hipError_t launch_row_reduce(const RowPass &rp, const ReduceArgs &r, hipStream_t stream) {
     return dispatch(rp, Launch{Kind::kReduce, nullptr, &r}, stream);
+}
+
+hipError_t launch_row_tail(const RowPass &rp, const TailArgs &t, hipStream_t stream) {
+    RowPass r = rp;
+    r.xcd_slabs = 1;  // one workgroup covers every column of its rows
+    return dispatch(r, Launch{Kind::kTail, nullptr, nullptr, &t}, stream);
+}
+
+hipError_t launch_propagate_tail(const TailArgs &t, const int32_t *node_of, const int32_t *parent_of,
+                                 const float *parscale, const CompArgs &c, int32_t comp_first, float *y, int64_t ldy,
+                                 int32_t ncols, hipStream_t stream) {
+    if (t.n_frag <= 0) return hipSuccess;
+    const bool comp = comp_first < INT32_MAX;
+    const bool vec = (ncols % 4) == 0 && aligned16(y, ldy, 4) && (!comp || aligned16(c.comp, c.ldc, 4));
+    const int ve = vec ? 4 : 1;
+    const int lpr = lanes_per_row(ncols, ve);
+#define CBM_TAIL_CASE(V, L)                                                                                     \
+    if (ve == V && lpr == L) {                                                                                  \
+        k_propagate_tail<V, L><<<dim3(unsigned(t.n_frag)), dim3(256), 0, stream>>>(t, node_of, parent_of, parscale, \
+                                                                                  c, comp_first, y, ldy, ncols);  \
+        return hipGetLastError();                                                                               \
+    }
+    CBM_TAIL_CASE(4, 1) CBM_TAIL_CASE(4, 2) CBM_TAIL_CASE(4, 4) CBM_TAIL_CASE(4, 8) CBM_TAIL_CASE(4, 16)
+    CBM_TAIL_CASE(4, 32) CBM_TAIL_CASE(4, 64)
+    CBM_TAIL_CASE(1, 1) CBM_TAIL_CASE(1, 2) CBM_TAIL_CASE(1, 4) CBM_TAIL_CASE(1, 8) CBM_TAIL_CASE(1, 16)
+    CBM_TAIL_CASE(1, 32) CBM_TAIL_CASE(1, 64)
+#undef CBM_TAIL_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_propagate(int32_t p_begin, int32_t p_end, const int32_t *node_of, const int32_t *parent_of,

@@ -144,6 +144,30 @@ hipError_t launch_propagate_comp(int32_t p_begin, int32_t p_end, const int32_t *
                                  const float *parscale, const CompArgs &c, float *y, int64_t ldy, int32_t ncols,
                                  hipStream_t stream);
 
+// A tree tail (plan arrays CBM_P_TF_* / CBM_P_TR_*): fragment f holds the
+// segments [frag[f], frag[f+1]), one per tree level from the fragment's root
+// down, segment s the positions pos[seg[s] .. seg[s+1]).  A tail launch gives
+// each fragment one 256-thread workgroup, which finishes the segments one
+// after another with a workgroup barrier between them.
+struct TailArgs {
+    const int32_t *frag = nullptr;
+    const int32_t *seg = nullptr;
+    const int32_t *pos = nullptr;
+    int32_t n_frag = 0;
+};
+
+// The forward propagation of a tail, root segments first: each position as in
+// launch_propagate, or as in launch_propagate_comp from position comp_first
+// on (the same arithmetic, so the same bits as the per-level launches).
+hipError_t launch_propagate_tail(const TailArgs &t, const int32_t *node_of, const int32_t *parent_of,
+                                 const float *parscale, const CompArgs &c, int32_t comp_first, float *y, int64_t ldy,
+                                 int32_t ncols, hipStream_t stream);
+
+// A row pass over a tail, deepest segments first (the adjoint's reverse pass:
+// a node's internal children sit one segment deeper in its own fragment).
+// Each row is summed by one group of lanes, in entry order.
+hipError_t launch_row_tail(const RowPass &rp, const TailArgs &t, hipStream_t stream);
+
 // The reference's branch-serial propagation (cbm/cbm.cu:25-91) for the
 // operator surface: one workgroup per branch and column block walks the
 // branch's edges in order, y[dst] += y[src] (mult null) or

@@ -176,6 +176,99 @@ bool range_pieces(int32_t n, const std::vector<int32_t> &ptr, const std::vector<
     return true;
 }
 
+// Tree tail of one pass (see CBM_P_TF_FRAG in cbm_host.h): the deepest levels
+// [level, L) in one launch, one workgroup per fragment.  member(v): whether
+// the pass computes node v (every node for the forward propagation, internal
+// nodes for the reverse pass); pos_of[v]: v's position in the pass's order.
+// The first tail level is chosen by a cost model in microseconds: a level of
+// its own costs t_level (a dependent launch); the tail costs t_launch plus
+// t_step per step, a fragment taking a step per level it spans and one per
+// four of its nodes (a 256-thread workgroup finishes four of the widest rows
+// at a time), and the tail as many steps as its slowest fragment or as all
+// fragments' steps over the 2048 workgroups the device runs at once, if
+// more (many small fragments: N = 2,449,029 with 396k tree edges measured
+// 0.13 ms per-level against 0.53 ms as a tail from level 1).  No tail starts
+// above min_level; none is built (level = L) unless it saves time.
+struct Tail {
+    int32_t level = 0;
+    std::vector<int32_t> frag, seg, pos;
+};
+
+template <class Member>
+Tail tree_tail(int32_t n, int32_t levels, const std::vector<int32_t> &depth, const std::vector<int32_t> &parent,
+               const std::vector<int32_t> &kptr, const std::vector<int32_t> &kid, const std::vector<int32_t> &by_level,
+               const std::vector<int32_t> &level_ptr, Member member, const std::vector<int32_t> &pos_of,
+               int32_t min_level, double t_level, double t_launch, double t_step) {
+    Tail t;
+    t.level = levels;
+    // Subtree aggregates, deepest nodes first: members below and at v, and
+    // the deepest level holding one (-1: none).
+    std::vector<int32_t> size(size_t(n), 0), deepest(size_t(n), -1);
+    for (int64_t i = int64_t(n) - 1; i >= 0; --i) {
+        const int32_t v = by_level[size_t(i)];
+        if (member(v)) {
+            size[size_t(v)] += 1;
+            deepest[size_t(v)] = std::max(deepest[size_t(v)], depth[size_t(v)]);
+        }
+        const int32_t u = parent[size_t(v)];
+        if (u >= 0) {
+            size[size_t(u)] += size[size_t(v)];
+            deepest[size_t(u)] = std::max(deepest[size_t(u)], deepest[size_t(v)]);
+        }
+    }
+    std::vector<int32_t> busy(size_t(levels) + 1, 0);  // levels from l down holding members
+    for (int32_t l = levels - 1; l >= 0; --l) {
+        bool any = false;
+        for (int32_t i = level_ptr[size_t(l)]; i < level_ptr[size_t(l) + 1] && !any; ++i) any = member(by_level[size_t(i)]);
+        busy[size_t(l)] = busy[size_t(l) + 1] + (any ? 1 : 0);
+    }
+    constexpr double kResidentGroups = 2048.0;
+    double best = 0.0;
+    for (int32_t a = std::max(min_level, 0); a < levels; ++a) {
+        double slowest = 0.0, all = 0.0;
+        for (int32_t i = level_ptr[size_t(a)]; i < level_ptr[size_t(a) + 1]; ++i) {
+            const int32_t r = by_level[size_t(i)];
+            if (size[size_t(r)] == 0) continue;
+            const double steps = double(deepest[size_t(r)] - a + 1) + double((size[size_t(r)] + 3) / 4);
+            slowest = std::max(slowest, steps);
+            all += steps;
+        }
+        const double gain =
+            t_level * busy[size_t(a)] - (t_launch + t_step * std::max(slowest, all / kResidentGroups));
+        if (gain > best) {
+            best = gain;
+            t.level = a;
+        }
+    }
+    if (t.level >= levels) return t;
+    // Fragments: the level's nodes with members below, in level order; each
+    // fragment's members level by level (a breadth-first walk), ascending
+    // position inside a level.
+    const int32_t a = t.level;
+    t.frag.push_back(0);
+    t.seg.push_back(0);
+    std::vector<int32_t> cur, next;
+    for (int32_t i = level_ptr[size_t(a)]; i < level_ptr[size_t(a) + 1]; ++i) {
+        const int32_t r = by_level[size_t(i)];
+        if (size[size_t(r)] == 0) continue;
+        cur.assign(1, r);
+        while (!cur.empty()) {
+            const size_t first = t.pos.size();
+            next.clear();
+            for (const int32_t v : cur) {
+                if (member(v)) t.pos.push_back(pos_of[size_t(v)]);
+                for (int32_t k = kptr[size_t(v)]; k < kptr[size_t(v) + 1]; ++k)
+                    if (size[size_t(kid[size_t(k)])] > 0) next.push_back(kid[size_t(k)]);
+            }
+            std::sort(t.pos.begin() + int64_t(first), t.pos.end());
+            t.seg.push_back(int32_t(t.pos.size()));
+            cur.swap(next);
+        }
+        t.frag.push_back(int32_t(t.seg.size()) - 1);
+    }
+    return t;
+}
+
 // Where each piece set's tables go.
 struct PieceSlots {
     int32_t n_ranges;
@@ -292,13 +385,22 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
                 lvl_node[size_t(fill[size_t(2 * depth[size_t(v)] + (is_long(v) ? 0 : 1))]++)] = v;
         }
         for (int32_t p = 0; p < n; ++p) lvl_parent[size_t(p)] = parent[size_t(lvl_node[size_t(p)])];
+        std::vector<int32_t> pos_of_node(static_cast<size_t>(n));
+        for (int32_t p = 0; p < n; ++p) pos_of_node[size_t(lvl_node[size_t(p)])] = p;
         {
             auto &ppos = plan->arr[CBM_P_LVL_PPOS];
-            std::vector<int32_t> pos_of_node(static_cast<size_t>(n));
-            for (int32_t p = 0; p < n; ++p) pos_of_node[size_t(lvl_node[size_t(p)])] = p;
             ppos.resize(size_t(n));
             for (int32_t p = 0; p < n; ++p)
                 ppos[size_t(p)] = lvl_parent[size_t(p)] < 0 ? -1 : pos_of_node[size_t(lvl_parent[size_t(p)])];
+        }
+        // Forward tail: level 0 has no parent term, so a tail starts at 1 or below.
+        {
+            Tail tf = tree_tail(n, levels, depth, parent, kptr, kid, lvl_node, lvl_ptr, [](int32_t) { return true; },
+                                pos_of_node, 1, 3.5, 4.0, 1.0);
+            plan->stat[CBM_PS_TF_LEVEL] = tf.level;
+            plan->arr[CBM_P_TF_FRAG].swap(tf.frag);
+            plan->arr[CBM_P_TF_SEG].swap(tf.seg);
+            plan->arr[CBM_P_TF_POS].swap(tf.pos);
         }
 
         // Forward Delta in level order with packed signs.
@@ -561,6 +663,25 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
             plan->arr[CBM_P_RRED_SLOT].swap(r.red_slot);
             plan->arr[CBM_P_RRED_CNT].swap(r.red_cnt);
             plan->stat[CBM_PS_REV_SLOTS] = max_slots;
+        }
+        // Reverse tail, below the deepest level with long child lists (those
+        // take the chunk kernels of the per-level pass).
+        {
+            const auto &rptr = plan->arr[CBM_P_REV_PTR];
+            const auto &rnode = plan->arr[CBM_P_REV_NODE];
+            const auto &rnlong = plan->arr[CBM_P_REV_NLONG];
+            std::vector<int32_t> rev_pos_of(size_t(n), -1);
+            for (int32_t i = 0; i < rptr[size_t(levels)]; ++i) rev_pos_of[size_t(rnode[size_t(i)])] = i;
+            int32_t min_level = 0;
+            for (int32_t l = 0; l < levels; ++l)
+                if (rnlong[size_t(l)] > 0) min_level = l + 1;
+            Tail tr = tree_tail(n, levels, depth, parent, kptr, kid, lvl_node, lvl_ptr,
+                                [&](int32_t v) { return kptr[size_t(v) + 1] > kptr[size_t(v)]; }, rev_pos_of,
+                                min_level, 7.0, 5.0, 2.0);
+            plan->stat[CBM_PS_TR_LEVEL] = tr.level;
+            plan->arr[CBM_P_TR_FRAG].swap(tr.frag);
+            plan->arr[CBM_P_TR_SEG].swap(tr.seg);
+            plan->arr[CBM_P_TR_POS].swap(tr.pos);
         }
         *out = plan.release();
         return CBM_OK;

@@ -162,6 +162,20 @@ typedef struct cbm_desc {
     const int32_t *brs4_cnt;
     int32_t bwd_pieces4;
     const int32_t *bpc4_row;
+    /* tree tails (plan arrays CBM_P_TF_* / CBM_P_TR_*, stats TF_LEVEL /
+       TR_LEVEL): the first level of each tail (n_levels: no tail), the number
+       of fragments (the fragment offsets hold one more entry), and the device
+       tables */
+    int32_t tf_level;
+    int32_t tf_frags;
+    const int32_t *tf_frag;
+    const int32_t *tf_seg;
+    const int32_t *tf_pos;
+    int32_t tr_level;
+    int32_t tr_frags;
+    const int32_t *tr_frag;
+    const int32_t *tr_seg;
+    const int32_t *tr_pos;
 } cbm_desc;
 
 typedef struct cbm_handle cbm_handle;
@@ -312,6 +326,19 @@ int cbm_set_xcd_slabs(cbm_handle *h, int32_t slabs);
  * cbm_reserve sizes it for the current choice of either storage type.
  */
 int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode);
+
+/*
+ * Tree tails (1 on, the default; 0 off).  The deepest tree levels of the
+ * forward propagation and of the adjoint's reverse pass, each of which would
+ * be a short dependent launch, are finished in one launch: one workgroup per
+ * fragment (a node of the tail's first level with its descendants) works
+ * through the fragment's levels with a workgroup barrier between them.  The
+ * plan chooses each tail's first level by a cost model (cbm_plan_create).
+ * The forward gives the same bits either way; the reverse pass sums each row
+ * in entry order by one group of lanes, which can round differently from the
+ * per-level mapping.
+ */
+int cbm_set_tree_tails(cbm_handle *h, int32_t mode);
 
 /*
  * Column phases of the XCD range passes (-1 automatic, the default; 1, 2 or

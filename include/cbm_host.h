@@ -256,7 +256,19 @@ enum {
     CBM_P_BRS4_SLOT = 66,
     CBM_P_BRS4_CNT = 67,
     CBM_P_BPC4_ROW = 68,
-    CBM_P_N_ARRAYS = 69
+    /* Tree tails: the deepest levels [TF_LEVEL, L) of the forward propagation (and [TR_LEVEL, L) of
+       the adjoint's reverse pass) are finished in one launch, one workgroup per fragment -- a node of
+       the tail's first level with its descendants -- whose levels run one after another between
+       workgroup barriers.  Fragment f holds segments [FRAG[f], FRAG[f+1]), one per level from the
+       fragment's root down; segment s holds positions POS[SEG[s] .. SEG[s+1]) (level positions for
+       the forward, internal-node positions of REV_NODE for the reverse), ascending. */
+    CBM_P_TF_FRAG = 69,    /* int32 [Ff+1] forward tail: segment offsets per fragment                  */
+    CBM_P_TF_SEG = 70,     /* int32 [Sf+1] position offsets per segment                                */
+    CBM_P_TF_POS = 71,     /* int32 [Mf]   level positions                                             */
+    CBM_P_TR_FRAG = 72,    /* int32 [Fr+1] reverse tail: segment offsets per fragment                  */
+    CBM_P_TR_SEG = 73,     /* int32 [Sr+1] position offsets per segment                                */
+    CBM_P_TR_POS = 74,     /* int32 [Mr]   internal-node positions                                     */
+    CBM_P_N_ARRAYS = 75
 };
 
 /* Scalars of a cbm_plan, addressed by cbm_plan_stat(). */
@@ -271,7 +283,9 @@ enum {
     CBM_PS_BWD_PIECES = 7, /* backward column-range pieces                              */
     CBM_PS_FWD_PIECES4 = 8, /* forward pieces of the CBM_XCD_RANGES_WIDE set            */
     CBM_PS_BWD_PIECES4 = 9, /* backward pieces of that set                               */
-    CBM_PS_N_STATS = 10
+    CBM_PS_TF_LEVEL = 10,  /* first level of the forward tail (L: none)                   */
+    CBM_PS_TR_LEVEL = 11,  /* first level of the reverse tail (L: none)                   */
+    CBM_PS_N_STATS = 12
 };
 
 #define CBM_LONG_ROW 256

@@ -37,7 +37,9 @@ def _desc(pl, host=None, **over):
               mode=MODE_AX, fwd_slots=st["fwd_slots"], bwd_nlong=st["bwd_nlong"], n_bck=int(pl.bck_pos.numel()),
               n_bred=int(pl.bred_pos.numel()), bwd_slots=st["bwd_slots"], fwd_gslots=st["fwd_gslots"],
               rev_slots=st["rev_slots"], fwd_pieces=st["fwd_pieces"], bwd_pieces=st["bwd_pieces"],
-              fwd_pieces4=st["fwd_pieces4"], bwd_pieces4=st["bwd_pieces4"], d=None, **fields)
+              fwd_pieces4=st["fwd_pieces4"], bwd_pieces4=st["bwd_pieces4"], tf_level=st["tf_level"],
+              tr_level=st["tr_level"], tf_frags=max(0, pl.tf_frag.numel() - 1),
+              tr_frags=max(0, pl.tr_frag.numel() - 1), d=None, **fields)
     kw.update(over)
     return CbmDesc(**kw), keep
 
@@ -105,3 +107,21 @@ def test_damaged_level_tables_and_modes_are_refused(plan):
     desc, keep = _desc(plan, n_edges=plan.n)  # a forest over n nodes has fewer than n edges
     rc, msg = _create(desc)
     assert rc == _lib.CBM_ERR_ARG and "incomplete descriptor" in msg
+
+
+def test_damaged_tree_tails_are_refused(plan):
+    L = plan.depth
+    assert plan.stats["tf_level"] < L or plan.stats["tr_level"] < L  # the hub graph has a tail
+    for over in (dict(tf_level=0), dict(tf_level=L + 1), dict(tr_level=-1), dict(tr_level=L + 1)):
+        desc, keep = _desc(plan, **over)
+        rc, msg = _create(desc)
+        assert rc == _lib.CBM_ERR_ARG and "tree tail" in msg, over
+    if plan.stats["tr_level"] < L:
+        desc, keep = _desc(plan, tr_frags=0)
+        assert _create(desc)[0] == _lib.CBM_ERR_ARG
+        desc, keep = _desc(plan, tr_pos=None)
+        assert _create(desc)[0] == _lib.CBM_ERR_ARG
+    # no tail at all (level = depth, no tables) is a valid layout
+    desc, keep = _desc(plan, tf_level=L, tr_level=L, tf_frags=0, tr_frags=0, tf_frag=None, tf_seg=None, tf_pos=None,
+                       tr_frag=None, tr_seg=None, tr_pos=None)
+    assert _create(desc)[0] == _lib.CBM_OK

@@ -371,3 +371,43 @@ def test_xcd_ranges_with_column_slabs(hub, cuda, dtype):
         assert O.max_rel_error(y.cpu().numpy(), O.forward(a, xe, mode)) <= 1e-5, mode
         assert O.max_rel_error(dx.cpu().numpy(), O.adjoint(a, xe, mode)) <= 1e-5, mode
         assert torch.equal(y, y2), mode
+
+
+@pytest.mark.parametrize("dtype,f", [(torch.float32, 1), (torch.float32, 64), (torch.float32, 300),
+                                     (torch.bfloat16, 64), (torch.bfloat16, 520)])
+def test_tree_tails(hub, cuda, dtype, f):
+    """The deepest levels in one launch (one workgroup per fragment): the
+    forward gives the per-level launches' bits, plain and compensated (Kahan
+    from level 2, inside the tail); the reverse pass (each row summed in
+    entry order) stays within FP32 rounding of them and of FP64, and
+    integer-valued inputs are exact."""
+    a, mats = hub
+    for mode, m in mats.items():
+        h = m.cusparse_extension
+        assert h.plan_stats["tf_level"] < h.depth and h.plan_stats["tr_level"] < h.depth
+        xt = torch.from_numpy(_x((HUB_N, f), 60 + f)).to(cuda).to(dtype)
+        xe = xt.float().cpu().numpy()
+        for depth in (None, 2):
+            h.set_kahan_depth(depth)
+            res = {}
+            for on in (True, False):
+                h.set_tree_tails(on)
+                y = torch.full((HUB_N, f), float("nan"), device=cuda)
+                dx = torch.full((HUB_N, f), float("nan"), device=cuda)
+                h.forward(xt, y)
+                launches = h.last_launches()
+                h.matmul_t(xt, dx)
+                res[on] = (y, dx, launches)
+            assert torch.equal(res[True][0], res[False][0]), (mode, depth)
+            assert res[True][2] < res[False][2]
+            assert O.max_rel_error(res[True][1].cpu().numpy(), res[False][1].cpu().numpy()) <= 1e-6, (mode, depth)
+            assert O.max_rel_error(res[True][0].cpu().numpy(), O.forward(a, xe, mode)) <= 1e-5, (mode, depth)
+            assert O.max_rel_error(res[True][1].cpu().numpy(), O.adjoint(a, xe, mode)) <= 1e-5, (mode, depth)
+        h.set_kahan_depth(32)
+        h.set_tree_tails(True)
+    h = mats["ax"].cusparse_extension
+    xi = np.random.default_rng(5).integers(-3, 4, size=(HUB_N, 24)).astype(np.float32)
+    xt = torch.from_numpy(xi).to(cuda)
+    dx = torch.empty(HUB_N, 24, device=cuda)
+    h.matmul_t(xt, dx)
+    assert np.array_equal(dx.cpu().numpy().astype(np.float64), O.adjoint(a, xi, "ax"))

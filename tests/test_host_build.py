@@ -403,3 +403,68 @@ def test_no_range_pieces_above_the_size_limit():
     assert pl.stats["fwd_pieces4"] == 0 and pl.stats["bwd_pieces4"] == 0
     assert pl.fpc_b.numel() == 0 and pl.fpc_range.numel() == 0
     assert pl.arrays["fpc4_b"].numel() == 0 and pl.arrays["bpc4_range"].numel() == 0
+
+
+def _check_tail(pl, prefix, member_positions, parent_position, children_positions):
+    """A tail's fragments partition the positions of levels >= its first
+    level, one segment per level from the fragment's root down, ascending
+    positions; the work of a segment depends only on its own fragment."""
+    L = pl.depth
+    a = pl.stats[prefix + "_level"]
+    frag, seg, pos = (pl.arrays[prefix + k].numpy() for k in ("_frag", "_seg", "_pos"))
+    if a >= L:
+        assert frag.size == 0 and seg.size == 0 and pos.size == 0
+        return None
+    want = sorted(p for l in range(a, L) for p in member_positions(l))
+    assert sorted(pos.tolist()) == want
+    assert frag[0] == 0 and frag[-1] == seg.size - 1 and seg[0] == 0 and seg[-1] == pos.size
+    where = {}  # position -> (fragment, depth in fragment)
+    for f in range(frag.size - 1):
+        for d, s in enumerate(range(frag[f], frag[f + 1])):
+            chunk = pos[seg[s]:seg[s + 1]]
+            assert chunk.size > 0 and np.all(np.diff(chunk) > 0)
+            for p in chunk.tolist():
+                where[p] = (f, d)
+    return a, where
+
+
+def test_tree_tails_are_closed_fragments():
+    ei = torch.as_tensor(hub_graph(1500))
+    b = _build(ei.numpy(), 1500, 0)
+    for n, e in ((1500, ei),):
+        pl = structure.plan(n, b.csr_ptr, b.csr_col, b.csr_val, b.tree_src, b.tree_dst)
+        assert pl.stats["tf_level"] < pl.depth and pl.stats["tr_level"] < pl.depth
+        lp = pl.lvl_ptr.numpy()
+        ppos = pl.lvl_ppos.numpy()
+        # forward: a position's parent is one segment up in its fragment, or above the tail
+        a, where = _check_tail(pl, "tf", lambda l: range(lp[l], lp[l + 1]), None, None)
+        for p, (f, d) in where.items():
+            q = ppos[p]
+            if d == 0:
+                assert q < lp[a]
+            else:
+                assert where[q] == (f, d - 1)
+        # reverse: a node's internal children are one segment down in its fragment
+        rp = pl.rev_ptr.numpy()
+        rnode = pl.rev_node.numpy()
+        cptr, cnode, cmid = pl.ch_ptr.numpy(), pl.ch_node.numpy(), pl.ch_mid.numpy()
+        rev_pos = {int(v): i for i, v in enumerate(rnode)}
+        a, where = _check_tail(pl, "tr", lambda l: range(rp[l], rp[l + 1]), None, None)
+        assert not pl.rev_nlong.numpy()[a:].any()
+        for i, (f, d) in where.items():
+            for k in range(cmid[i], cptr[i + 1]):
+                child = int(cnode[k]) & 0x7FFFFFFF
+                assert where[rev_pos[child]] == (f, d + 1)
+
+
+def test_chain_and_star_tails():
+    """A long chain goes to one fragment from level 1 (forward) or 0
+    (reverse); a star has nothing worth a tail below its root."""
+    n = 400
+    ei = torch.tensor([list(range(n - 1)) + list(range(1, n)), list(range(1, n)) + list(range(n - 1))],
+                      dtype=torch.int32)
+    b = _build(ei.numpy(), n, 0)
+    pl = structure.plan(n, b.csr_ptr, b.csr_col, b.csr_val, b.tree_src, b.tree_dst)
+    if pl.depth > 8:
+        assert pl.stats["tf_level"] == 1 and pl.tf_frag.numel() == 2
+        assert pl.stats["tr_level"] < pl.depth

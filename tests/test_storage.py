@@ -127,31 +127,35 @@ def test_save_checks_its_input(built, tmp_path):
         _save(tmp_path / "y.npz", b, pl, delta_sign=torch.zeros_like(b.csr_val))
 
 
-def test_version_1_files_load_without_the_four_range_pieces(built, tmp_path):
-    """A version 1 file (no 4-range piece set) loads with that set empty and
-    the rest of the layout unchanged."""
+@pytest.mark.parametrize("version", [1, 2])
+def test_older_files_load_without_the_newer_layout_parts(built, tmp_path, version):
+    """A version 1 file (no 4-range piece set, no tree tails) or version 2
+    file (no tree tails) loads with those parts empty -- no pieces, tails at
+    the depth, i.e. none -- and the rest of the layout unchanged."""
     b, pl = built
-    path = tmp_path / "v2.npz"
+    path = tmp_path / "v3.npz"
     _save(path, b, pl)
-    new = {"plan/" + k for k in storage._NEW_IN_2}
+    gone = set(storage._NEW_IN_3) | (set(storage._NEW_IN_2) if version == 1 else set())
+    gone_stats = set(storage._NEW_STATS_IN_3) | (set(storage._NEW_STATS_IN_2) if version == 1 else set())
 
-    def as_v1(name, a):
-        if name in new:
+    def older(name, a):
+        if name.startswith("plan/") and name[5:] in gone:
             return None
         if name == "meta":
             m = json.loads(bytes(a).decode())
-            m["version"] = 1
-            for k in storage._NEW_STATS_IN_2:
+            m["version"] = version
+            for k in gone_stats:
                 m["plan_stats"].pop(k)
-            for k in new:
-                m["crc32"].pop(k)
+            for k in gone:
+                m["crc32"].pop("plan/" + k)
             return np.frombuffer(json.dumps(m).encode(), dtype=np.uint8)
         return a
-    _rewrite(path, tmp_path / "v1.npz", as_v1)
-    st = storage.load(tmp_path / "v1.npz")
-    assert st.plan.stats["fwd_pieces4"] == 0 and st.plan.stats["bwd_pieces4"] == 0
+    _rewrite(path, tmp_path / "old.npz", older)
+    st = storage.load(tmp_path / "old.npz")
+    for k in gone_stats:
+        assert st.plan.stats[k] == (pl.depth if k in storage._NEW_STATS_IN_3 else 0), k
     for k in structure.PLAN_NAMES:
-        if k in storage._NEW_IN_2:
+        if k in gone:
             assert st.plan.arrays[k].numel() == 0, k
         else:
             assert torch.equal(st.plan.arrays[k], pl.arrays[k]), k
