@@ -221,19 +221,19 @@ int ensure_scratch(cbm_handle *h, int32_t n_cols) {
     return CBM_OK;
 }
 
-// One ordered row range whose first n_long rows are long: the chunk kernel,
-// the short-row pass and the reduction of multi-chunk rows, in that order.
+// One ordered row range whose first n_long rows are long: the chunk kernel
+// and the short-row pass (one grid), then the reduction of multi-chunk rows.
 int run_range(cbm_handle *h, cbmk::RowPass rp, int32_t p_begin, int32_t p_end, int32_t n_long,
               const cbmk::ChunkArgs &ck, const cbmk::ReduceArgs &rd, hipStream_t stream, const char *what) {
     hipError_t e = hipSuccess;
-    if (ck.count > 0) {
-        e = cbmk::launch_row_chunks(rp, ck, stream);
-        if (e != hipSuccess) return device_error(e, what);
-        ++h->last_launches;
-    }
     rp.p_begin = p_begin + n_long;
     rp.p_end = p_end;
-    if (rp.p_end > rp.p_begin) {
+    if (ck.count > 0) {
+        int launched = 0;
+        e = cbmk::launch_row_mixed(rp, ck, &launched, stream);
+        if (e != hipSuccess) return device_error(e, what);
+        h->last_launches += launched;
+    } else if (rp.p_end > rp.p_begin) {
         e = cbmk::launch_row_pass(rp, stream);
         if (e != hipSuccess) return device_error(e, what);
         ++h->last_launches;

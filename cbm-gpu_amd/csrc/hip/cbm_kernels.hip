@@ -362,7 +362,7 @@ __device__ inline void finish_row(const RowPass &a, int32_t p, int32_t row, int3
 // and the slices' sums added by a butterfly across the sub-groups (the same
 // order on every lane, so every sub-group holds the same bits).
 template <class P, int VE, int LPR, int S>
-__global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
+__device__ __forceinline__ void row_pass_block(const RowPass &a, uint32_t bid) {
     constexpr int kLanesPerRow = LPR * S;
     constexpr int kRowsPerWave = 64 / kLanesPerRow;
     const int lane = int(threadIdx.x & 63u);
@@ -373,17 +373,24 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
     int32_t slab, p_begin = a.p_begin, p_end = a.p_end;
     int64_t blk;
     if (a.ranges) {
-        const uint32_t x = blockIdx.x & 7u, per = 8u / uint32_t(a.range_count);
+        const uint32_t x = bid & 7u, per = 8u / uint32_t(a.range_count);
         const uint32_t k = x / per;
-        const int64_t b8 = int64_t(blockIdx.x >> 3);
+        const int64_t b8 = int64_t(bid >> 3);
         const int64_t phase = b8 / a.range_blocks;
         blk = b8 - phase * a.range_blocks;
         slab = int32_t(int64_t(x % per) * a.range_phases + phase);
-        p_begin = a.range_first[k];
-        p_end = a.range_first[k + 1];
+        // Constant indices only: a runtime index into the argument's array
+        // would copy the whole argument to scratch memory.
+#pragma unroll
+        for (uint32_t i = 0; i < 8; ++i) {
+            if (i == k) {
+                p_begin = a.range_first[i];
+                p_end = a.range_first[i + 1];
+            }
+        }
     } else {
-        slab = int32_t(blockIdx.x % uint32_t(a.xcd_slabs));
-        blk = int64_t(blockIdx.x / uint32_t(a.xcd_slabs));
+        slab = int32_t(bid % uint32_t(a.xcd_slabs));
+        blk = int64_t(bid / uint32_t(a.xcd_slabs));
     }
     const int64_t group = (blk * int64_t(blockDim.x >> 6) + int64_t(threadIdx.x >> 6)) * kRowsPerWave +
                           lane / kLanesPerRow;
@@ -413,6 +420,11 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
         }
         if (active && sub == 0) finish_row<P, VE>(a, p, row, c0, acc);
     }
+}
+
+template <class P, int VE, int LPR, int S>
+__global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
+    row_pass_block<P, VE, LPR, S>(a, blockIdx.x);
 }
 
 // A row pass over a tree tail: workgroup f finishes fragment f's segments,
@@ -452,11 +464,11 @@ __global__ __launch_bounds__(256) void k_row_tail(const RowPass a, const TailArg
 // groups' partial sums are added in group order through LDS.  A one-chunk
 // row is finished here; otherwise the chunk's sum goes to its scratch slot.
 template <class P, int VE, int LPR, int BS>
-__global__ __launch_bounds__(BS) void k_row_chunk(const RowPass a, const ChunkArgs c) {
+__device__ __forceinline__ void row_chunk_block(const RowPass &a, const ChunkArgs &c, uint32_t bid) {
     constexpr int kGroups = BS / LPR;
     __shared__ float part[BS * VE];
-    const int32_t slab = int32_t(blockIdx.x % uint32_t(a.xcd_slabs));
-    const int32_t chunk = c.first + int32_t(blockIdx.x / uint32_t(a.xcd_slabs));
+    const int32_t slab = int32_t(bid % uint32_t(a.xcd_slabs));
+    const int32_t chunk = c.first + int32_t(bid / uint32_t(a.xcd_slabs));
     const int32_t p = c.pos[chunk];
     const uint32_t meta = uint32_t(c.meta[chunk]);
     const int64_t idx = meta & 0xFFFFu, cnt = meta >> 16;
@@ -492,6 +504,26 @@ __global__ __launch_bounds__(BS) void k_row_chunk(const RowPass a, const ChunkAr
             }
         }
         __syncthreads();
+    }
+}
+
+template <class P, int VE, int LPR, int BS>
+__global__ __launch_bounds__(BS) void k_row_chunk(const RowPass a, const ChunkArgs c) {
+    row_chunk_block<P, VE, LPR, BS>(a, c, blockIdx.x);
+}
+
+// The long-row chunks and the short-row pass of one row range in one grid:
+// workgroups below chunk_blocks are chunks (dispatched first, so the long
+// rows, whose chains are the longest, start at once), the rest the short-row
+// pass.  Each workgroup does exactly what it does in the kernel of its kind,
+// so the bits are those of the two launches; only the wait between them is
+// gone, which is the time of a range too small to fill the chip.
+template <class P, int VE, int LPR, int S, int BS>
+__global__ __launch_bounds__(BS) void k_row_mixed(const RowPass a, const ChunkArgs c, uint32_t chunk_blocks) {
+    if (blockIdx.x < chunk_blocks) {
+        row_chunk_block<P, VE, LPR, BS>(a, c, blockIdx.x);
+    } else {
+        row_pass_block<P, VE, LPR, S>(a, blockIdx.x - chunk_blocks);
     }
 }
 
@@ -702,13 +734,14 @@ bool aligned16(const void *p, int64_t ld, int elem) {
     return (reinterpret_cast<uintptr_t>(p) & 15u) == 0 && ((ld * elem) & 15) == 0;
 }
 
-enum class Kind { kPass, kChunk, kReduce, kTail };
+enum class Kind { kPass, kChunk, kReduce, kTail, kMixed };
 
 struct Launch {
     Kind kind;
     const ChunkArgs *chunk;
     const ReduceArgs *reduce;
     const TailArgs *tail = nullptr;
+    int *launched = nullptr;  // kMixed: kernels launched
 };
 
 // Whether a short-row pass gives each row a whole wavefront (see RowPass).
@@ -734,6 +767,28 @@ hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
         r.range_blocks = int32_t((most + kRowsPerBlock - 1) / kRowsPerBlock);
         k_row_pass<P, VE, LPR, 1><<<dim3(unsigned(int64_t(r.range_blocks) * r.range_phases * 8)), dim3(256), 0,
                                     stream>>>(r);
+    } else if (lk.kind == Kind::kMixed) {
+        if (rp.ranges) return hipErrorInvalidValue;
+        const int64_t n = std::max<int64_t>(0, int64_t(rp.p_end) - rp.p_begin);
+        const int64_t ns = rp.xcd_slabs;
+        const int64_t chunk_blocks = int64_t(lk.chunk->count) * ns;
+        // The chunk kernel's workgroup size (16 groups of LPR lanes, at least
+        // 256 threads) for the whole grid; the short-row part packs its rows
+        // into workgroups of that size.
+        constexpr int kBs = (16 * LPR > 256) ? 16 * LPR : 256;
+        constexpr int64_t kWaves = kBs / 64;
+        *lk.launched = 1;
+        if constexpr (LPR < 64) {
+            if (wave_per_row<LPR>(rp)) {
+                k_row_mixed<P, VE, LPR, 64 / LPR, kBs>
+                    <<<dim3(unsigned(chunk_blocks + (n + kWaves - 1) / kWaves * ns)), dim3(kBs), 0, stream>>>(
+                        rp, *lk.chunk, uint32_t(chunk_blocks));
+                return hipGetLastError();
+            }
+        }
+        constexpr int64_t kRows = kWaves * (64 / LPR);
+        k_row_mixed<P, VE, LPR, 1, kBs><<<dim3(unsigned(chunk_blocks + (n + kRows - 1) / kRows * ns)), dim3(kBs), 0,
+                                         stream>>>(rp, *lk.chunk, uint32_t(chunk_blocks));
     } else if (lk.kind == Kind::kPass) {
         const int64_t n = int64_t(rp.p_end) - rp.p_begin;
         if (n <= 0) return hipSuccess;
@@ -864,6 +919,12 @@ hipError_t launch_row_pass(const RowPass &rp, hipStream_t stream) {
 
 hipError_t launch_row_chunks(const RowPass &rp, const ChunkArgs &c, hipStream_t stream) {
     return dispatch(rp, Launch{Kind::kChunk, &c, nullptr}, stream);
+}
+
+hipError_t launch_row_mixed(const RowPass &rp, const ChunkArgs &c, int *launched, hipStream_t stream) {
+    Launch lk{Kind::kMixed, &c, nullptr};
+    lk.launched = launched;
+    return dispatch(rp, lk, stream);
 }
 
 hipError_t launch_row_reduce(const RowPass &rp, const ReduceArgs &r, hipStream_t stream) {

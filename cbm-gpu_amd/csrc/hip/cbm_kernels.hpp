@@ -112,6 +112,12 @@ struct ChunkArgs {
 };
 hipError_t launch_row_chunks(const RowPass &rp, const ChunkArgs &c, hipStream_t stream);
 
+// The chunks of c and the short-row pass over [p_begin, p_end) of rp as one
+// grid (the same work and bits as launch_row_chunks then launch_row_pass).
+// *launched gets the number of kernels launched.  The range must not be an
+// XCD range pass.
+hipError_t launch_row_mixed(const RowPass &rp, const ChunkArgs &c, int *launched, hipStream_t stream);
+
 // Rows [first, first + count) of a reduction list: row pos[i] (i itself when
 // pos is null) is the sum of scratch rows slot[i] .. slot[i] + cnt[i] - 1,
 // added in order (0 when cnt[i] is 0), then finished.

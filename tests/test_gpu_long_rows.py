@@ -320,7 +320,8 @@ def test_xcd_ranges_active_reports_the_choice(hub, cuda):
     once half a row is 256 bytes: F x element size >= 512) and off for both
     storage types; automatic off at this size (a gathered matrix under
     24 MiB), and the automatic choice is the one the forward takes (its
-    launch count)."""
+    bits: a range pass sums a row range by range, so on these random inputs
+    it rounds differently from the row pass)."""
     _, mats = hub
     h = mats["ax"].cusparse_extension
     for mode, f, want32, want16 in ((True, 64, 8, 8), (True, 128, 4, 8), (True, 256, 4, 4), (8, 256, 8, 8),
@@ -332,14 +333,14 @@ def test_xcd_ranges_active_reports_the_choice(hub, cuda):
         h.set_xcd_ranges(2)
     with pytest.raises(Exception):
         h.set_xcd_phases(3)
-    x = torch.randn(HUB_N, 64, device=cuda)
-    y = torch.empty(HUB_N, 64, device=cuda)
-    launches = {}
+    x = torch.randn(HUB_N, 64, device=cuda, generator=torch.Generator(device=cuda).manual_seed(11))
+    ys = {}
     for mode in (True, False, None):
         h.set_xcd_ranges(mode)
-        h.forward(x, y)
-        launches[mode] = h.last_launches()
-    assert launches[None] == launches[False] != launches[True]
+        ys[mode] = torch.empty(HUB_N, 64, device=cuda)
+        h.forward(x, ys[mode])
+    assert torch.equal(ys[None], ys[False])
+    assert not torch.equal(ys[None], ys[True])
     with pytest.raises(Exception):
         h.xcd_ranges_active(torch.float16, 64)
     h.set_xcd_ranges(None)
