@@ -578,7 +578,7 @@ def main():
                      "traffic_gbs": round(traffic["bytes"] / (rec["ms_phase_bf16"][dom] * 1e-3) / 1e9, 1)
                      if traffic else None,
                      "traffic_source": traffic.get("source") if traffic else None,
-                     "kernel": f"{dom} phase (BF16): k_row_chunk + k_row_pass + k_row_reduce, dual pass",
+                     "kernel": f"{dom} phase (BF16): k_row_mixed (long-row chunks + short rows) + k_row_reduce, dual pass",
                      "algorithmic_bytes": rec["bytes_phase_bf16"][dom],
                      "ms": rec["ms_phase_bf16"][dom]},
         "detail": rec,
