@@ -753,6 +753,23 @@ bool wave_per_row(const RowPass &rp) {
     return rp.rows_hint > 0 && int64_t(rp.rows_hint) * LPR < 64 * kWaveSlots;
 }
 
+// Whether one grid of k_row_mixed keeps as many waves per CU as the chunk
+// kernel and the short-row pass it stands for (it needs the registers of the
+// larger part), asked of the runtime once per instantiation.
+template <class P, int VE, int LPR, int S, int BS>
+bool mixed_keeps_occupancy() {
+    static const bool keeps = [] {
+        int m = 0, c = 0, p = 0;
+        const bool ok =
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k_row_mixed<P, VE, LPR, S, BS>, BS, 0) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_row_chunk<P, VE, LPR, BS>, BS, 0) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, k_row_pass<P, VE, LPR, S>, 256, 0) == hipSuccess;
+        (void)hipGetLastError();
+        return ok && m * BS >= c * BS && m * BS >= p * 256;
+    }();
+    return keeps;
+}
+
 template <class P, int VE, int LPR>
 hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
     constexpr int64_t kRowsPerBlock = 4 * (64 / LPR);
@@ -777,18 +794,34 @@ hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
         // into workgroups of that size.
         constexpr int kBs = (16 * LPR > 256) ? 16 * LPR : 256;
         constexpr int64_t kWaves = kBs / 64;
+        constexpr int kWaveS = LPR < 64 ? 64 / LPR : 1;
+        constexpr int64_t kRows = kWaves * (64 / LPR);
+        bool wave = false;
+        if constexpr (LPR < 64) wave = wave_per_row<LPR>(rp);
+        const int64_t pass_blocks = (wave ? (n + kWaves - 1) / kWaves : (n + kRows - 1) / kRows) * ns;
+        // One grid gains the wait between the two launches and lets the
+        // short rows fill the chip while the chunks' longer chains finish.
+        // It runs with the registers of the larger part, so a grid made
+        // mostly of chunks (of a product that fills the chip) takes it only
+        // if that costs the chunks no waves.
+        const bool small = rp.rows_hint > 0 && int64_t(rp.rows_hint) * LPR < 64 * kWaveSlots;
+        if (!small && chunk_blocks > pass_blocks &&
+            !(wave ? mixed_keeps_occupancy<P, VE, LPR, kWaveS, kBs>() : mixed_keeps_occupancy<P, VE, LPR, 1, kBs>())) {
+            const hipError_t e = launch_one<P, VE, LPR>(rp, Launch{Kind::kChunk, lk.chunk, nullptr}, stream);
+            if (e != hipSuccess) return e;
+            *lk.launched = n > 0 ? 2 : 1;
+            return launch_one<P, VE, LPR>(rp, Launch{Kind::kPass, nullptr, nullptr}, stream);
+        }
         *lk.launched = 1;
         if constexpr (LPR < 64) {
-            if (wave_per_row<LPR>(rp)) {
-                k_row_mixed<P, VE, LPR, 64 / LPR, kBs>
-                    <<<dim3(unsigned(chunk_blocks + (n + kWaves - 1) / kWaves * ns)), dim3(kBs), 0, stream>>>(
-                        rp, *lk.chunk, uint32_t(chunk_blocks));
+            if (wave) {
+                k_row_mixed<P, VE, LPR, kWaveS, kBs><<<dim3(unsigned(chunk_blocks + pass_blocks)), dim3(kBs), 0,
+                                                       stream>>>(rp, *lk.chunk, uint32_t(chunk_blocks));
                 return hipGetLastError();
             }
         }
-        constexpr int64_t kRows = kWaves * (64 / LPR);
-        k_row_mixed<P, VE, LPR, 1, kBs><<<dim3(unsigned(chunk_blocks + (n + kRows - 1) / kRows * ns)), dim3(kBs), 0,
-                                         stream>>>(rp, *lk.chunk, uint32_t(chunk_blocks));
+        k_row_mixed<P, VE, LPR, 1, kBs><<<dim3(unsigned(chunk_blocks + pass_blocks)), dim3(kBs), 0, stream>>>(
+            rp, *lk.chunk, uint32_t(chunk_blocks));
     } else if (lk.kind == Kind::kPass) {
         const int64_t n = int64_t(rp.p_end) - rp.p_begin;
         if (n <= 0) return hipSuccess;

@@ -112,10 +112,11 @@ struct ChunkArgs {
 };
 hipError_t launch_row_chunks(const RowPass &rp, const ChunkArgs &c, hipStream_t stream);
 
-// The chunks of c and the short-row pass over [p_begin, p_end) of rp as one
-// grid (the same work and bits as launch_row_chunks then launch_row_pass).
-// *launched gets the number of kernels launched.  The range must not be an
-// XCD range pass.
+// The chunks of c and the short-row pass over [p_begin, p_end) of rp: one
+// grid, except for a grid made mostly of chunks that would run fewer waves
+// per CU than the chunk kernel (then launch_row_chunks, launch_row_pass; the
+// same work and bits either way).  *launched gets the
+// number of kernels launched.  The range must not be an XCD range pass.
 hipError_t launch_row_mixed(const RowPass &rp, const ChunkArgs &c, int *launched, hipStream_t stream);
 
 // Rows [first, first + count) of a reduction list: row pos[i] (i itself when
