@@ -313,6 +313,12 @@ class cusparseCBM:
         m = -1 if mode is None else int(bool(mode))
         _lib.check_hip(self._lib.cbm_set_wave_rows(self._h, m), "set_wave_rows")
 
+    def set_mixed_grid(self, mode):
+        """Long-row chunks and short rows of a gather pass: True one grid,
+        False two launches, None by rule (the default).  Same results."""
+        m = -1 if mode is None else int(bool(mode))
+        _lib.check_hip(self._lib.cbm_set_mixed_grid(self._h, m), "set_mixed_grid")
+
     def set_xcd_ranges(self, mode):
         """XCD column ranges for the gather passes: None automatic (by size),
         False off, True on (when the layout has range pieces; the piece set

@@ -24,6 +24,7 @@ struct cbm_handle {
     int32_t kahan_level = 32; // levels from here on propagate with compensation
     int32_t fwd_split = -1;   // forward form: 0 fused per level, 1 split, -1 automatic
     int32_t wave_rows = -1;   // short-row passes: whole wavefront per row (1), never (0), by size (-1)
+    int32_t mixed_grid = -1;  // chunks and short rows: one grid (1), two launches (0), by rule (-1)
     int32_t xcd_slabs = 1;    // feature slabs spread over the XCDs (1 = off)
     int32_t xcd_ranges = -1;  // column ranges per XCD: 1 on, 0 off, -1 by size, 8 / 4 that piece set
     int32_t xcd_phases = -1;  // column phases of a range pass: 1, 2, 4, or -1 automatic
@@ -451,6 +452,7 @@ cbmk::RowPass forward_pass(const cbm_handle *h, int32_t dtype, const void *x, in
     rp.ncols = n_cols;
     rp.rows_hint = h->d.n_rows;
     rp.wave_rows = h->wave_rows;
+    rp.mixed_grid = h->mixed_grid;
     rp.xcd_slabs = h->xcd_slabs;
     return rp;
 }
@@ -703,6 +705,7 @@ int adjoint_reverse(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, fl
     rv.ncols = n_cols;
     rv.rows_hint = h->d.n_rows;
     rv.wave_rows = h->wave_rows;
+    rv.mixed_grid = h->mixed_grid;
     rv.xcd_slabs = h->xcd_slabs;
     cbmk::ChunkArgs ck;
     ck.pos = h->d.rck_pos;
@@ -763,6 +766,7 @@ int adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, co
     tp.ncols = n_cols;
     tp.rows_hint = h->d.n_rows;
     tp.wave_rows = h->wave_rows;
+    tp.mixed_grid = h->mixed_grid;
     tp.xcd_slabs = h->xcd_slabs;
     tp.rowscale = h->d.mode == CBM_MODE_AX ? nullptr : h->d.d;
     cbmk::ChunkArgs ck;
@@ -903,6 +907,15 @@ extern "C" int cbm_set_wave_rows(cbm_handle *h, int32_t mode) {
         return CBM_ERR_ARG;
     }
     h->wave_rows = mode;
+    return CBM_OK;
+}
+
+extern "C" int cbm_set_mixed_grid(cbm_handle *h, int32_t mode) {
+    if (!h || mode < -1 || mode > 1) {
+        set_error("cbm_set_mixed_grid: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    h->mixed_grid = mode;
     return CBM_OK;
 }
 

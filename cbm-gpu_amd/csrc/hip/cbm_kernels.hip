@@ -805,8 +805,12 @@ hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
         // mostly of chunks (of a product that fills the chip) takes it only
         // if that costs the chunks no waves.
         const bool small = rp.rows_hint > 0 && int64_t(rp.rows_hint) * LPR < 64 * kWaveSlots;
-        if (!small && chunk_blocks > pass_blocks &&
-            !(wave ? mixed_keeps_occupancy<P, VE, LPR, kWaveS, kBs>() : mixed_keeps_occupancy<P, VE, LPR, 1, kBs>())) {
+        const bool one =
+            rp.mixed_grid > 0 ||
+            (rp.mixed_grid < 0 && (small || chunk_blocks <= pass_blocks ||
+                                   (wave ? mixed_keeps_occupancy<P, VE, LPR, kWaveS, kBs>()
+                                         : mixed_keeps_occupancy<P, VE, LPR, 1, kBs>())));
+        if (!one) {
             const hipError_t e = launch_one<P, VE, LPR>(rp, Launch{Kind::kChunk, lk.chunk, nullptr}, stream);
             if (e != hipSuccess) return e;
             *lk.launched = n > 0 ? 2 : 1;

@@ -68,6 +68,9 @@ struct RowPass {
     // that rule, 0 never, 1 always.
     int32_t rows_hint = 0;
     int32_t wave_rows = -1;
+    // Long-row chunks and short rows of one range (launch_row_mixed): one
+    // grid (1), two launches (0), or by the rule there (-1).
+    int32_t mixed_grid = -1;
     // XCD feature slabs: the columns are cut into xcd_slabs slabs of slab_w
     // (set by the launcher, at least 16 vectors wide) and workgroup b works
     // on slab b % xcd_slabs.  Workgroups are dispatched round-robin over the
@@ -115,7 +118,7 @@ hipError_t launch_row_chunks(const RowPass &rp, const ChunkArgs &c, hipStream_t 
 // The chunks of c and the short-row pass over [p_begin, p_end) of rp: one
 // grid, except for a grid made mostly of chunks that would run fewer waves
 // per CU than the chunk kernel (then launch_row_chunks, launch_row_pass; the
-// same work and bits either way).  *launched gets the
+// same work and bits either way); rp.mixed_grid 1 or 0 forces one form.  *launched gets the
 // number of kernels launched.  The range must not be an XCD range pass.
 hipError_t launch_row_mixed(const RowPass &rp, const ChunkArgs &c, int *launched, hipStream_t stream);
 

@@ -292,6 +292,15 @@ int cbm_set_forward_split(cbm_handle *h, int32_t mode);
 int cbm_set_wave_rows(cbm_handle *h, int32_t mode);
 
 /*
+ * Long-row chunks and short rows of a gather pass: 1 runs them as one grid
+ * (the chunks' workgroups first), 0 as two launches (chunks, then short
+ * rows), -1 (the default) one grid except where a grid made mostly of chunks,
+ * in a matrix that fills the device, would run fewer waves per CU than the
+ * chunk kernel alone.  Both forms give the same bits.
+ */
+int cbm_set_mixed_grid(cbm_handle *h, int32_t mode);
+
+/*
  * XCD feature slabs (1..64, default 1 = off): the columns of the gathered
  * matrix are cut into up to `slabs` slabs and workgroup b of a gather pass
  * works on slab b % slabs.  MI355X dispatches workgroups round-robin over

@@ -74,6 +74,41 @@ def test_long_rows_integer_exact_and_deterministic(hub, cuda):
     assert np.array_equal(d1.cpu().numpy().astype(np.float64), O.adjoint(a, x, "ax"))
 
 
+@pytest.mark.parametrize("mode", ["ax", "dadx"])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("f", [24, 64, 256])
+def test_one_grid_and_two_launches_are_bit_identical(hub, cuda, mode, dtype, f):
+    """The long-row chunks and the short rows of a pass run as one grid or as
+    two launches (chunks first); every workgroup does the same work either
+    way, so forward, Delta half and adjoint give the same bits in both forms,
+    for both row mappings and both forward forms.  At F = 256 the rows take
+    64 lanes and 1024-thread workgroups."""
+    a, mats = hub
+    h = mats[mode].cusparse_extension
+    xt = torch.from_numpy(_x((HUB_N, f), 31 + f)).to(cuda).to(dtype)
+    for wave, split in ((None, None), (True, True), (False, True), (None, False)):
+        h.set_wave_rows(wave)
+        h.set_forward_split(split)
+        res = {}
+        for grid in (True, False):
+            h.set_mixed_grid(grid)
+            y = torch.full((HUB_N, f), float("nan"), device=cuda)
+            yd = torch.full((HUB_N, f), float("nan"), device=cuda)
+            dx = torch.full((HUB_N, f), float("nan"), device=cuda)
+            h.forward(xt, y)
+            h.delta_spmm(xt, yd)
+            h.matmul_t(xt, dx)
+            res[grid] = (y, yd, dx)
+        for k in range(3):
+            assert torch.equal(res[True][k], res[False][k]), (mode, f, wave, split, k)
+        assert O.max_rel_error(res[True][0].cpu().numpy(), O.forward(a, xt.float().cpu().numpy(), mode)) <= 1e-5
+        assert O.max_rel_error(res[True][2].cpu().numpy(), O.adjoint(a, xt.float().cpu().numpy(), mode)) <= 1e-5
+    assert h._lib.cbm_set_mixed_grid(h._h, 2) != 0  # out of range: rejected
+    h.set_mixed_grid(None)
+    h.set_forward_split(None)
+    h.set_wave_rows(None)
+
+
 @pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
 @pytest.mark.parametrize("dtype,f", [(torch.float32, 1), (torch.float32, 3), (torch.float32, 4), (torch.float32, 8),
                                      (torch.float32, 64), (torch.bfloat16, 16), (torch.bfloat16, 128)])
