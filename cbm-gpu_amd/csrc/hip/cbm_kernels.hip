@@ -430,9 +430,15 @@ __global__ __launch_bounds__(256) void k_row_pass(const RowPass a) {
 // A row pass over a tree tail: workgroup f finishes fragment f's segments,
 // deepest first, one group of LPR lanes per row and a workgroup barrier
 // between segments (the rows of one segment read those of the next deeper
-// one, written by this workgroup).
+// one, written by this workgroup).  Only the reverse pass runs here: each
+// row's own term s_u G_u is loaded before its gathers, since it depends on
+// nothing they produce and, loaded after them as finish_row does, it would
+// be one more dependent wait on every level of the fragment; the arithmetic
+// is finish_row's, so the bits are too.
 template <class P, int VE, int LPR>
 __global__ __launch_bounds__(256) void k_row_tail(const RowPass a, const TailArgs t) {
+    static_assert(P::kDual && P::kExtra == kExtraSelf, "the reverse pass");
+    using TE = typename P::TE;
     constexpr int kGroups = 256 / LPR;
     const int lane = int(threadIdx.x & 63u);
     const int l = lane & (LPR - 1);
@@ -445,14 +451,26 @@ __global__ __launch_bounds__(256) void k_row_tail(const RowPass a, const TailArg
             const int32_t p = t.pos[i];
             const int32_t kb = a.ptr[p], ke = a.ptr_end ? a.ptr_end[p] : a.ptr[p + 1];
             const int32_t row = a.node_of ? a.node_of[p] : p;
+            const float rs = a.rowscale ? a.rowscale[row] : 1.0f;
+            const float es = (a.escale ? a.escale[row] : 1.0f) * a.ecoef;
             for (int32_t pass = 0; pass < passes; ++pass) {
                 const int32_t c0 = pass * kSpan + l * VE;
                 const bool active = c0 < a.ncols;
+                // Lanes past the last column read column 0 (no branch around
+                // the load) and store nothing.
+                float ev[VE];
+                Load<TE, VE>::run(static_cast<const TE *>(a.e) + int64_t(row) * a.lde + (active ? c0 : 0), ev);
                 float acc[VE];
 #pragma unroll
                 for (int q = 0; q < VE; ++q) acc[q] = 0.0f;
                 gather_sum<P, VE, LPR>(a, p, kb, ke, l, c0, active, acc);
-                if (active) finish_row<P, VE>(a, p, row, c0, acc);
+                if (active) {
+#pragma unroll
+                    for (int q = 0; q < VE; ++q) acc[q] *= rs;
+#pragma unroll
+                    for (int q = 0; q < VE; ++q) acc[q] = fmaf(ev[q], es, acc[q]);
+                    store_row<VE>(a.y + int64_t(row) * a.ldy + c0, acc);
+                }
             }
         }
         __syncthreads();
