@@ -817,6 +817,8 @@ hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
         bool wave = false;
         if constexpr (LPR < 64) wave = wave_per_row<LPR>(rp);
         const int64_t pass_blocks = (wave ? (n + kWaves - 1) / kWaves : (n + kRows - 1) / kRows) * ns;
+        *lk.launched = 0;
+        if (chunk_blocks + pass_blocks == 0) return hipSuccess;
         // One grid gains the wait between the two launches and lets the
         // short rows fill the chip while the chunks' longer chains finish.
         // It runs with the registers of the larger part, so a grid made
