@@ -74,6 +74,35 @@ def test_long_rows_integer_exact_and_deterministic(hub, cuda):
     assert np.array_equal(d1.cpu().numpy().astype(np.float64), O.adjoint(a, x, "ax"))
 
 
+@pytest.mark.parametrize("f", [16, 64, 256])
+def test_every_row_long(cuda, f):
+    """A complete graph on 300 nodes with alpha above every degree: no tree
+    edge passes the rule, so Delta = A and every row (299 entries) is longer
+    than the short-row limit in both directions.  The gather passes then have
+    chunks and no short rows; with integer inputs both storage types and both
+    grid forms give the oracle's values exactly."""
+    n = 300
+    r, c = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    keep = r != c
+    ei = torch.from_numpy(np.stack([r[keep], c[keep]]).astype(np.int32))
+    a = O.csr_of(ei.numpy(), n)
+    h = cbm4mm(ei, torch.ones(ei.shape[1]), alpha=n, device=cuda).cusparse_extension
+    assert h.plan_stats["depth"] == 1
+    x = np.random.default_rng(17).integers(-3, 4, size=(n, f)).astype(np.float32)
+    want_y, want_dx = O.forward(a, x, "ax"), O.adjoint(a, x, "ax")
+    for dtype in (torch.float32, torch.bfloat16):
+        xt = torch.from_numpy(x).to(cuda).to(dtype)
+        for grid in (True, False, None):
+            h.set_mixed_grid(grid)
+            y = torch.full((n, f), float("nan"), device=cuda)
+            dx = torch.full((n, f), float("nan"), device=cuda)
+            h.forward(xt, y)
+            h.matmul_t(xt, dx)
+            assert np.array_equal(y.cpu().numpy().astype(np.float64), want_y), (dtype, grid)
+            assert np.array_equal(dx.cpu().numpy().astype(np.float64), want_dx), (dtype, grid)
+    h.set_mixed_grid(None)
+
+
 @pytest.mark.parametrize("mode", ["ax", "dadx"])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
 @pytest.mark.parametrize("f", [24, 64, 256])
