@@ -1,0 +1,12 @@
+// Instantiates the row-pass kernels of the pass shapes PassF32, PassF32Parent
+// (row_pass.hpp); one translation unit per few shapes keeps the build parallel.
+#include "row_pass.hpp"
+
+namespace cbmk {
+namespace rp {
+
+template hipError_t dispatch_ve<PassF32>(const RowPass &, const Launch &, hipStream_t);
+template hipError_t dispatch_ve<PassF32Parent>(const RowPass &, const Launch &, hipStream_t);
+
+}  // namespace rp
+}  // namespace cbmk
