@@ -65,6 +65,20 @@ def csr_of(edge_index, n):
     return a
 
 
+def csr_of_sorted(edge_index, n):
+    """The same binary CSR for an edge list already sorted by (row, col)
+    without duplicates (what the copying-model generator returns), built
+    without the COO sort: checked, then indptr by counting.  Raises
+    ValueError for input that is not sorted and unique."""
+    ei = np.asarray(edge_index, dtype=np.int64)
+    key = ei[0] * n + ei[1]
+    if key.size > 1 and not bool(np.all(key[1:] > key[:-1])):
+        raise ValueError("edge list is not sorted by (row, col) without duplicates")
+    indptr = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(ei[0], minlength=n), out=indptr[1:])
+    return sp.csr_matrix((np.ones(ei.shape[1]), ei[1].astype(np.int32), indptr), shape=(n, n))
+
+
 def degree_scale(a):
     """D^-1/2 from integer row degrees in FP64; 0 for degree-0 rows."""
     deg = np.diff(a.indptr).astype(np.float64)

@@ -94,8 +94,10 @@ def test_integer_x_is_bitwise_exact(mats, graph1, cuda, f):
 
 @pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
 def test_reference_protocol_vs_rocsparse(mats, cuda, mode):
-    """benchmark/compare_matmul_cuda.py: X ~ U(0,1), CBM vs CSR vs torch.sparse,
-    assert_close(rtol=1e-5, atol=0); every failure is counted."""
+    """benchmark/compare_matmul_cuda.py:41-62: X ~ U(0,1); CBM, rocSPARSE CSR
+    and torch.sparse CSR (``csr.a @ x``) compared pairwise with
+    assert_close(rtol=1e-5, atol=0); every failure is counted (the reference
+    adds 0 on failure, :62)."""
     m, c = mats[mode]
     failed = 0
     for it in range(5):
@@ -104,12 +106,14 @@ def test_reference_protocol_vs_rocsparse(mats, cuda, mode):
         y_csr = torch.empty(N1, 64, device=cuda)
         m.matmul(x, y_cbm)
         c.matmul(x, y_csr)
+        y_torch = c.a @ x  # the torch.sparse leg (compare_matmul_cuda.py:53)
         torch.cuda.synchronize()
-        try:
-            torch.testing.assert_close(y_csr, y_cbm, rtol=1e-5, atol=0)
-        except AssertionError:
-            failed += 1
-    assert failed == 0
+        for ref, got in ((y_csr, y_cbm), (y_torch, y_cbm), (y_torch, y_csr)):
+            try:
+                torch.testing.assert_close(ref, got, rtol=1e-5, atol=0)
+            except AssertionError:
+                failed += 1
+    assert failed == 0, f"{failed} of 15 comparisons failed"
 
 
 @pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])

@@ -230,3 +230,19 @@ def test_planted_mistakes_are_caught(golden):
     assert not np.allclose(O.cbm_forward(a, swapped, x, "ax"), case["products"]["ax"])
     # 4. the transposed operand in the adjoint breaks A D (not symmetric)
     assert not np.allclose(O.forward(a, x, "adx"), case["products"]["adx_t"])
+
+
+def test_sorted_edge_list_csr_equals_coo_build():
+    """csr_of_sorted (counting build for sorted unique edges) gives csr_of's
+    matrix, and refuses unsorted or duplicated input."""
+    rng = np.random.default_rng(5)
+    n = 300
+    pairs = np.unique(rng.integers(0, n, size=(2000, 2)), axis=0)
+    ei = pairs.T.copy()
+    a, b = O.csr_of(ei, n), O.csr_of_sorted(ei, n)
+    assert np.array_equal(a.indptr, b.indptr) and np.array_equal(a.indices, b.indices)
+    assert np.array_equal(a.data, b.data)
+    with pytest.raises(ValueError):
+        O.csr_of_sorted(ei[:, ::-1], n)
+    with pytest.raises(ValueError):
+        O.csr_of_sorted(np.concatenate([ei, ei[:, :1]], axis=1), n)
