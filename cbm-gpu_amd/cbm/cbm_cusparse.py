@@ -135,6 +135,7 @@ class cusparseCBM:
                        tf_frags=max(0, int(self._dev["tf_frag"].numel()) - 1),
                        tr_frags=max(0, int(self._dev["tr_frag"].numel()) - 1),
                        d=self._d.data_ptr() if self._d is not None else None, **ptrs)
+        self._reserved = 0
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             rc = self._lib.cbm_create(ctypes.byref(desc), current_stream_handle(self.device), ctypes.byref(h))
@@ -191,6 +192,17 @@ class cusparseCBM:
         if y.stride(1) != 1 or y.stride(0) < n_cols:
             raise ValueError(f"{name} must have unit column stride")
 
+    def _ensure_width(self, n_cols):
+        """Reserves the native scratch for n_cols columns if the handle serves
+        fewer (the native product calls never allocate).  Under HIP-graph
+        capture that allocation is not allowed: reserve before capturing."""
+        if n_cols <= self._reserved:
+            return
+        if torch.cuda.is_current_stream_capturing():
+            raise RuntimeError(f"CBM product of {n_cols} columns under graph capture: call reserve({n_cols}) "
+                               f"before capturing (reserved: {self._reserved})")
+        self.reserve(n_cols)
+
     # --------------------------------------------------------------- products
     def forward(self, rhs, res):
         """res = M rhs for this handle's product M (A, A D or D A D)."""
@@ -199,6 +211,7 @@ class cusparseCBM:
         self._check_out(res, f)
         if f == 0:
             return 0
+        self._ensure_width(f)
         rc = self._lib.cbm_matmul(self._h, code, x.data_ptr(), x.stride(0), res.data_ptr(), res.stride(0), f,
                                   current_stream_handle(self.device))
         _lib.check_hip(rc, "matmul")
@@ -231,6 +244,7 @@ class cusparseCBM:
             workspace = torch.empty((self.num_rows, f), dtype=torch.float32, device=self.device)
         else:
             self._check_out(workspace, f, "workspace")
+        self._ensure_width(f)
         rc = self._lib.cbm_matmul_t(self._h, code, g.data_ptr(), g.stride(0), out.data_ptr(), out.stride(0),
                                     workspace.data_ptr(), workspace.stride(0), f, current_stream_handle(self.device))
         _lib.check_hip(rc, "matmul_t")
@@ -243,6 +257,7 @@ class cusparseCBM:
         f = int(g.size(1))
         self._check_out(out, f, "out")
         self._check_out(workspace, f, "workspace")
+        self._ensure_width(f)
         st = current_stream_handle(self.device)
         _lib.check_hip(self._lib.cbm_adjoint_reverse(self._h, code, g.data_ptr(), g.stride(0), workspace.data_ptr(),
                                                      workspace.stride(0), f, st), "adjoint_reverse")
@@ -255,6 +270,7 @@ class cusparseCBM:
         g, code = self._check_in(grad)
         f = int(g.size(1))
         self._check_out(workspace, f, "workspace")
+        self._ensure_width(f)
         _lib.check_hip(self._lib.cbm_adjoint_reverse(self._h, code, g.data_ptr(), g.stride(0), workspace.data_ptr(),
                                                      workspace.stride(0), f, current_stream_handle(self.device)),
                        "adjoint_reverse")
@@ -263,6 +279,7 @@ class cusparseCBM:
         g, code = self._check_in(grad)
         f = int(g.size(1))
         self._check_out(out, f, "out")
+        self._ensure_width(f)
         _lib.check_hip(self._lib.cbm_adjoint_delta_t(self._h, code, g.data_ptr(), g.stride(0), workspace.data_ptr(),
                                                      workspace.stride(0), out.data_ptr(), out.stride(0), f,
                                                      current_stream_handle(self.device)), "adjoint_delta_t")
@@ -274,6 +291,7 @@ class cusparseCBM:
         self._check_out(res, f)
         if f == 0:
             return 0
+        self._ensure_width(f)
         rc = self._lib.cbm_delta_spmm(self._h, code, x.data_ptr(), x.stride(0), res.data_ptr(), res.stride(0), f,
                                       current_stream_handle(self.device))
         _lib.check_hip(rc, "delta_spmm")
@@ -285,14 +303,18 @@ class cusparseCBM:
         self._check_out(res, f)
         if f == 0:
             return 0
+        self._ensure_width(f)
         rc = self._lib.cbm_propagate(self._h, res.data_ptr(), res.stride(0), f, current_stream_handle(self.device))
         _lib.check_hip(rc, "propagate")
         return 0
 
     def reserve(self, n_cols):
-        """Pre-sizes the long-row scratch so that products of up to n_cols
-        columns allocate nothing (needed before HIP-graph capture)."""
+        """Sizes the native scratch (long-row partial sums, compensation rows,
+        XCD-range partial rows) for products of up to n_cols columns.  The
+        product methods reserve a wider width by themselves outside graph
+        capture; under capture the width must already be reserved."""
         _lib.check_hip(self._lib.cbm_reserve(self._h, int(n_cols)), "reserve")
+        self._reserved = max(self._reserved, int(n_cols))
 
     def set_kahan_depth(self, level):
         """Tree levels from `level` on propagate with an FP32 compensation

@@ -44,6 +44,7 @@ struct cbm_handle {
     int64_t partial_floats = 0;
     float *comp = nullptr;    // compensation rows of those levels
     int64_t comp_floats = 0;
+    int32_t reserved_cols = 0; // widest product the buffers serve (cbm_reserve)
     int last_launches = 0;
 };
 
@@ -200,7 +201,7 @@ int ensure_partial(cbm_handle *h, int32_t n_cols) {
     return CBM_OK;
 }
 
-int ensure_scratch(cbm_handle *h, int32_t n_cols) {
+int grow_buffers(cbm_handle *h, int32_t n_cols) {
     int rc = ensure_comp(h, n_cols);
     if (rc == CBM_OK) rc = ensure_partial(h, n_cols);
     if (rc != CBM_OK) return rc;
@@ -221,6 +222,29 @@ int ensure_scratch(cbm_handle *h, int32_t n_cols) {
     h->scratch_floats = need;
     return CBM_OK;
 }
+
+// Product calls allocate nothing: their buffers are sized by cbm_reserve (and
+// kept sized by the settings that change what a width needs), so a call
+// wider than the reserved width is refused instead of growing them -- a
+// hipMalloc in a call would synchronise the device and fails under HIP-graph
+// capture.
+int check_reserved(const cbm_handle *h, int32_t n_cols, const char *what) {
+    if (n_cols <= h->reserved_cols) return CBM_OK;
+    set_error("%s: %d columns exceed the reserved width %d; call cbm_reserve(h, %d) first", what, n_cols,
+              h->reserved_cols, n_cols);
+    return CBM_ERR_ARG;
+}
+
+// The device buffers a forward call works in: long-row partial sums,
+// compensation rows, range partial rows (the handle's own, sized by
+// cbm_reserve).
+struct Bufs {
+    float *scratch = nullptr;
+    float *comp = nullptr;
+    float *partial = nullptr;
+};
+
+Bufs handle_bufs(const cbm_handle *h) { return Bufs{h->scratch, h->comp, h->partial}; }
 
 // One ordered row range whose first n_long rows are long: the chunk kernel
 // and the short-row pass (one grid), then the reduction of multi-chunk rows.
@@ -271,19 +295,20 @@ const T *shift(const void *p, int32_t c0, int elem) {
 }
 
 // The forward Delta pass over every level; with_parent adds the parent term.
-int forward_levels(cbm_handle *h, cbmk::RowPass rp, bool with_parent, hipStream_t stream, const char *what) {
+int forward_levels(cbm_handle *h, const Bufs &b, cbmk::RowPass rp, bool with_parent, hipStream_t stream,
+                   const char *what) {
     cbmk::ChunkArgs ck;
     ck.pos = h->d.fck_pos;
     ck.meta = h->d.fck_meta;
     ck.slot = h->d.fck_slot;
-    ck.scratch = h->scratch;
+    ck.scratch = b.scratch;
     cbmk::ReduceArgs rd;
     rd.pos = h->d.fred_pos;
     rd.slot = h->d.fred_slot;
     rd.cnt = h->d.fred_cnt;
-    rd.scratch = h->scratch;
+    rd.scratch = b.scratch;
     rp.parent_pos = h->d.lvl_ppos;
-    rp.comp = h->comp;
+    rp.comp = b.comp;
     rp.comp_base = comp_base(h);
     rp.ldc = rp.ncols;
     for (int32_t l = 0; l < h->d.n_levels; ++l) {
@@ -303,7 +328,7 @@ int forward_levels(cbm_handle *h, cbmk::RowPass rp, bool with_parent, hipStream_
 // The Delta pass of every level at once (no parent term): all long-row
 // chunks with slots numbered over the levels, one short-row pass over all
 // positions that skips the long rows, then all reductions.
-int delta_flat(cbm_handle *h, cbmk::RowPass rp, hipStream_t stream, const char *what) {
+int delta_flat(cbm_handle *h, const Bufs &b, cbmk::RowPass rp, hipStream_t stream, const char *what) {
     const int32_t L = h->d.n_levels;
     cbmk::ChunkArgs ck;
     ck.pos = h->d.fck_pos;
@@ -311,14 +336,14 @@ int delta_flat(cbm_handle *h, cbmk::RowPass rp, hipStream_t stream, const char *
     ck.slot = h->d.fck_gslot;
     ck.first = 0;
     ck.count = h->fck_ptr[size_t(L)];
-    ck.scratch = h->scratch;
+    ck.scratch = b.scratch;
     cbmk::ReduceArgs rd;
     rd.pos = h->d.fred_pos;
     rd.slot = h->d.fred_gslot;
     rd.cnt = h->d.fred_cnt;
     rd.first = 0;
     rd.count = h->fred_ptr[size_t(L)];
-    rd.scratch = h->scratch;
+    rd.scratch = b.scratch;
     rp.extra = cbmk::kExtraNone;
     rp.skip_above = CBM_LONG_ROW;
     return run_range(h, rp, 0, h->d.n_rows, 0, ck, rd, stream, what);
@@ -329,11 +354,12 @@ bool tails_on(const cbm_handle *h) { return h->tree_tails != 0; }
 // In-place propagation of levels 1 .. L-1, y_v += s_v y_parent(v), with the
 // compensated form from the handle's Kahan level on: one launch per level
 // down to the tree tail, then one launch for the tail.
-int propagate_levels(cbm_handle *h, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream, const char *what) {
+int propagate_levels(cbm_handle *h, const Bufs &b, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream,
+                     const char *what) {
     const float *ps = h->d.mode == CBM_MODE_DADX ? h->ps_pos : nullptr;
     cbmk::CompArgs cm;
     cm.parent_pos = h->d.lvl_ppos;
-    cm.comp = h->comp;
+    cm.comp = b.comp;
     cm.base = comp_base(h);
     cm.ldc = n_cols;
     const int32_t L = h->d.n_levels;
@@ -380,7 +406,7 @@ void set_ranges(cbmk::RowPass &rp, const int32_t *first, int32_t n_ranges, int32
 // for the product, or over all rows at once without it for the Delta half
 // alone (the level propagation then adds the same parent terms, so the two
 // halves give the product's bits).
-int forward_ranges(cbm_handle *h, const RangeSet &rs, const cbmk::RowPass &base, bool with_parent,
+int forward_ranges(cbm_handle *h, const Bufs &b, const RangeSet &rs, const cbmk::RowPass &base, bool with_parent,
                    hipStream_t stream, const char *what) {
     cbmk::RowPass pc = base;
     pc.node_of = rs.fpc_slot;
@@ -390,25 +416,25 @@ int forward_ranges(cbm_handle *h, const RangeSet &rs, const cbmk::RowPass &base,
     pc.rowscale = nullptr;
     pc.escale = nullptr;
     pc.extra = cbmk::kExtraNone;
-    pc.y = h->partial;
+    pc.y = b.partial;
     pc.ldy = base.ncols;
     set_ranges(pc, rs.f_first, rs.n_ranges, range_phases(h, rs, base.ncols, base.g_bf16 ? 2 : 4));
     hipError_t e = cbmk::launch_row_pass(pc, stream);
     if (e != hipSuccess) return device_error(e, what);
     ++h->last_launches;
     cbmk::RowPass rr = base;
-    rr.g = h->partial;
+    rr.g = b.partial;
     rr.ldg = base.ncols;
     rr.g_bf16 = false;
     rr.parent_pos = h->d.lvl_ppos;
-    rr.comp = h->comp;
+    rr.comp = b.comp;
     rr.comp_base = comp_base(h);
     rr.ldc = base.ncols;
     cbmk::ReduceArgs rd;
     rd.pos = nullptr;
     rd.slot = rs.frs_slot;
     rd.cnt = rs.frs_cnt;
-    rd.scratch = h->partial;
+    rd.scratch = b.partial;
     const int32_t levels = with_parent ? h->d.n_levels : 1;
     for (int32_t l = 0; l < levels; ++l) {
         rr.extra = (with_parent && l > 0) ? (kahan_at(h, l) ? cbmk::kExtraParentKahan : cbmk::kExtraParent)
@@ -630,30 +656,44 @@ extern "C" int cbm_reserve(cbm_handle *h, int32_t n_cols) {
         set_error("cbm_reserve: invalid arguments");
         return CBM_ERR_ARG;
     }
-    return ensure_scratch(h, n_cols);
+    const int32_t cols = std::max(h->reserved_cols, n_cols);
+    const int rc = grow_buffers(h, cols);
+    if (rc == CBM_OK) h->reserved_cols = cols;
+    return rc;
 }
 
-extern "C" int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy,
-                          int32_t n_cols, hipStream_t stream) {
-    int rc = check_dense(h, dtype, x, ldx, y, ldy, n_cols, "cbm_matmul");
-    if (rc == CBM_OK) rc = ensure_scratch(h, n_cols);
-    if (rc != CBM_OK) return rc;
+namespace {
+
+// The forward product in buffers b; ranges: whether the XCD column ranges
+// may be used (they need b.partial).
+int forward_product(cbm_handle *h, const Bufs &b, bool ranges, int32_t dtype, const void *x, int64_t ldx, float *y,
+                    int64_t ldy, int32_t n_cols, hipStream_t stream, const char *what) {
     h->last_launches = 0;
     const int elem = dtype == CBM_BF16 ? 2 : 4;
     const bool split = use_split(h);
     return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
         const cbmk::RowPass rp = forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w);
-        if (const RangeSet *rs = pick_ranges(h, n_cols, elem)) {
+        if (const RangeSet *rs = ranges ? pick_ranges(h, n_cols, elem) : nullptr) {
             // Pieces and one reduction over all rows, then the light level
             // propagation: the bits of the per-level reduction with the
             // parent term (forward_ranges with_parent), in fewer, fuller launches.
-            const int r = forward_ranges(h, *rs, rp, false, stream, "cbm_matmul");
-            return r == CBM_OK ? propagate_levels(h, y + c0, ldy, w, stream, "cbm_matmul") : r;
+            const int r = forward_ranges(h, b, *rs, rp, false, stream, what);
+            return r == CBM_OK ? propagate_levels(h, b, y + c0, ldy, w, stream, what) : r;
         }
-        if (!split) return forward_levels(h, rp, true, stream, "cbm_matmul");
-        const int r = delta_flat(h, rp, stream, "cbm_matmul");
-        return r == CBM_OK ? propagate_levels(h, y + c0, ldy, w, stream, "cbm_matmul") : r;
+        if (!split) return forward_levels(h, b, rp, true, stream, what);
+        const int r = delta_flat(h, b, rp, stream, what);
+        return r == CBM_OK ? propagate_levels(h, b, y + c0, ldy, w, stream, what) : r;
     });
+}
+
+}  // namespace
+
+extern "C" int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy,
+                          int32_t n_cols, hipStream_t stream) {
+    int rc = check_dense(h, dtype, x, ldx, y, ldy, n_cols, "cbm_matmul");
+    if (rc == CBM_OK) rc = check_reserved(h, n_cols, "cbm_matmul");
+    if (rc != CBM_OK) return rc;
+    return forward_product(h, handle_bufs(h), true, dtype, x, ldx, y, ldy, n_cols, stream, "cbm_matmul");
 }
 
 namespace {
@@ -672,7 +712,7 @@ int check_adjoint(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, floa
         set_error("%s: workspace overlaps an operand", what);
         return CBM_ERR_ARG;
     }
-    return ensure_scratch(h, n_cols);
+    return check_reserved(h, n_cols, what);
 }
 
 // Reverse pass over the nodes with children, deepest level first:
@@ -841,7 +881,7 @@ extern "C" int cbm_adjoint_reverse(cbm_handle *h, int32_t dtype, const void *g, 
                                    int32_t n_cols, hipStream_t stream) {
     // This half reads g and writes z only.
     int rc = check_dense(h, dtype, g, ldg, z, ldz, n_cols, "cbm_adjoint_reverse");
-    if (rc == CBM_OK) rc = ensure_scratch(h, n_cols);
+    if (rc == CBM_OK) rc = check_reserved(h, n_cols, "cbm_adjoint_reverse");
     if (rc != CBM_OK) return rc;
     h->last_launches = 0;
     return adjoint_reverse(h, dtype, g, ldg, z, ldz, n_cols, stream);
@@ -858,15 +898,15 @@ extern "C" int cbm_adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, 
 extern "C" int cbm_delta_spmm(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy,
                               int32_t n_cols, hipStream_t stream) {
     int rc = check_dense(h, dtype, x, ldx, y, ldy, n_cols, "cbm_delta_spmm");
-    if (rc == CBM_OK) rc = ensure_scratch(h, n_cols);
+    if (rc == CBM_OK) rc = check_reserved(h, n_cols, "cbm_delta_spmm");
     if (rc != CBM_OK) return rc;
     h->last_launches = 0;
     const int elem = dtype == CBM_BF16 ? 2 : 4;
     return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
         const cbmk::RowPass rp = forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w);
         if (const RangeSet *rs = pick_ranges(h, n_cols, elem))
-            return forward_ranges(h, *rs, rp, false, stream, "cbm_delta_spmm");
-        return delta_flat(h, rp, stream, "cbm_delta_spmm");
+            return forward_ranges(h, handle_bufs(h), *rs, rp, false, stream, "cbm_delta_spmm");
+        return delta_flat(h, handle_bufs(h), rp, stream, "cbm_delta_spmm");
     });
 }
 
@@ -875,10 +915,10 @@ extern "C" int cbm_propagate(cbm_handle *h, float *y, int64_t ldy, int32_t n_col
         set_error("cbm_propagate: invalid arguments");
         return CBM_ERR_ARG;
     }
-    const int rc = ensure_comp(h, n_cols);
+    const int rc = check_reserved(h, n_cols, "cbm_propagate");
     if (rc != CBM_OK) return rc;
     h->last_launches = 0;
-    return propagate_levels(h, y, ldy, n_cols, stream, "cbm_propagate");
+    return propagate_levels(h, handle_bufs(h), y, ldy, n_cols, stream, "cbm_propagate");
 }
 
 extern "C" int cbm_last_launches(const cbm_handle *h) { return h ? h->last_launches : -1; }
@@ -889,7 +929,7 @@ extern "C" int cbm_set_kahan_depth(cbm_handle *h, int32_t level) {
         return CBM_ERR_ARG;
     }
     h->kahan_level = level < 0 ? -1 : level;
-    return CBM_OK;
+    return grow_buffers(h, h->reserved_cols);  // the compensated levels may need more rows
 }
 
 extern "C" int cbm_set_forward_split(cbm_handle *h, int32_t mode) {
@@ -934,7 +974,7 @@ extern "C" int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode) {
         return CBM_ERR_ARG;
     }
     h->xcd_ranges = mode;
-    return CBM_OK;
+    return grow_buffers(h, h->reserved_cols);  // a piece set needs its partial rows
 }
 
 extern "C" int cbm_set_tree_tails(cbm_handle *h, int32_t mode) {

@@ -10,8 +10,9 @@
  *
  * with A = (I - P)^-1 Delta.  X and G are row-major N x F in FP32 or BF16;
  * Y, dX and all accumulation are FP32.  Every call is asynchronous on the
- * given stream; once the handle's scratch is reserved (cbm_reserve) a call
- * allocates nothing and may be captured in a HIP graph.
+ * given stream.  Product calls allocate nothing: their scratch is sized by
+ * cbm_reserve, and a call wider than the reserved width fails with
+ * CBM_ERR_ARG, so every product call may be captured in a HIP graph.
  * Calls return CBM_OK (0) or a negative status; cbm_last_error() describes
  * the failure for the calling thread.
  *
@@ -197,10 +198,14 @@ int cbm_destroy(cbm_handle *h);
 /*
  * Sizes the handle's scratch for products of up to n_cols columns: the
  * partial sums of long rows, max(fwd_slots, bwd_slots, fwd_gslots, rev_slots) x n_cols
- * FP32, and the compensation rows of the compensated levels.  A
- * product call with more columns than reserved grows the scratch itself
- * (one hipMalloc); reserve first when the calls must not allocate (graph
- * capture).
+ * FP32, the compensation rows of the compensated levels, and the partial
+ * rows of the XCD column ranges.  Reserving never shrinks: the handle serves
+ * the widest width reserved so far.  The product calls (cbm_matmul,
+ * cbm_matmul_t, cbm_delta_spmm, cbm_propagate, cbm_adjoint_*) never allocate
+ * and fail with CBM_ERR_ARG for more columns than reserved.  The settings
+ * that change what a width needs (cbm_set_kahan_depth, cbm_set_xcd_ranges)
+ * grow the buffers for the reserved width themselves.  The reference
+ * allocates a cuSPARSE workspace on every call instead (cbm/cbm.cu:181,203).
  */
 int cbm_reserve(cbm_handle *h, int32_t n_cols);
 

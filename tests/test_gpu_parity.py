@@ -317,3 +317,61 @@ def test_graph_capture_replays(mats, cuda, ranges):
         torch.cuda.synchronize()
         assert torch.equal(y, y_ref) and torch.equal(dx, dx_ref)
     h.set_xcd_ranges(None)
+
+
+def test_product_calls_never_allocate(graph1, cuda):
+    """The native product calls refuse a width above the reserved one (they
+    would have to allocate); cbm_reserve is what sizes the scratch."""
+    import ctypes
+    from cbm import _lib
+    ei, a = graph1
+    m = cbm4mm(ei, torch.ones(ei.shape[1]), alpha=0, device=cuda)
+    h = m.cusparse_extension
+    lib = _lib.hip()
+    x = torch.from_numpy(_rand((N1, 48), 7)).to(cuda)
+    y = torch.empty(N1, 48, device=cuda)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rc = lib.cbm_matmul(h._h, 0, x.data_ptr(), 48, y.data_ptr(), 48, 48, st)
+    assert rc == _lib.CBM_ERR_ARG
+    assert b"reserved width" in lib.cbm_last_error()
+    z = torch.empty(N1, 48, device=cuda)
+    assert lib.cbm_matmul_t(h._h, 0, x.data_ptr(), 48, y.data_ptr(), 48, z.data_ptr(), 48, 48, st) == _lib.CBM_ERR_ARG
+    assert lib.cbm_reserve(h._h, 48) == 0
+    assert lib.cbm_matmul(h._h, 0, x.data_ptr(), 48, y.data_ptr(), 48, 48, st) == 0
+    assert lib.cbm_matmul(h._h, 0, x.data_ptr(), 48, y.data_ptr(), 48, 32, st) == 0  # narrower: fine
+    torch.cuda.synchronize()
+    assert O.max_rel_error(y[:, :32].cpu().numpy(), O.forward(a, x[:, :32].cpu().numpy(), "ax")) <= 1e-5
+
+
+def test_autograd_forward_and_backward_capture(graph1, cuda):
+    """The autograd product's forward and backward inside one HIP graph,
+    replayed three times with eager work between replays; each replay equals
+    the eager product and adjoint."""
+    ei, a = graph1
+    m = cbm4dad(ei, torch.ones(ei.shape[1]), alpha=0, device=cuda)
+    m.cusparse_extension.reserve(32)
+    x = torch.from_numpy(_rand((N1, 32), 21)).to(cuda).requires_grad_(True)
+    g = torch.from_numpy(_rand((N1, 32), 22)).to(cuda)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):  # warm-up outside capture
+        (m.apply(x) * g).sum().backward()
+    torch.cuda.current_stream().wait_stream(s)
+    x.grad = None
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        y = m.apply(x)
+        (y * g).sum().backward()
+    for it in range(3):
+        x.data.copy_(torch.from_numpy(_rand((N1, 32), 60 + it)).to(cuda))
+        g.copy_(torch.from_numpy(_rand((N1, 32), 70 + it)).to(cuda))
+        y_ref = torch.empty(N1, 32, device=cuda)
+        dx_ref = torch.empty(N1, 32, device=cuda)
+        m.matmul(x.detach(), y_ref)
+        m.matmul_t(g, dx_ref)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y.detach(), y_ref)
+        assert torch.equal(x.grad, dx_ref)
+        assert O.max_rel_error(x.grad.cpu().numpy(), O.adjoint(a, g.cpu().numpy(), "dadx")) <= 1e-5
