@@ -130,6 +130,7 @@ def declare(lib, sig):
     sig(lib, "cbm_destroy", c_i32, c_vp)
     sig(lib, "cbm_reserve", c_i32, c_vp, c_i32)
     sig(lib, "cbm_matmul", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
+    sig(lib, "cbm_matmul_noalloc", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
     sig(lib, "cbm_matmul_t", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
     sig(lib, "cbm_adjoint_reverse", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
     sig(lib, "cbm_adjoint_delta_t", c_i32, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp)
@@ -168,7 +169,7 @@ def declare(lib, sig):
 
 
 # Every symbol include/cbm_hip.h declares (checked by the CPU test suite).
-EXPORTED = ("cbm_last_error", "cbm_desc_size", "cbm_create", "cbm_destroy", "cbm_reserve", "cbm_matmul", "cbm_matmul_t", "cbm_adjoint_reverse",
+EXPORTED = ("cbm_last_error", "cbm_desc_size", "cbm_create", "cbm_destroy", "cbm_reserve", "cbm_matmul", "cbm_matmul_noalloc", "cbm_matmul_t", "cbm_adjoint_reverse",
             "cbm_adjoint_delta_t", "cbm_delta_spmm",
             "cbm_propagate", "cbm_last_launches", "cbm_set_slab_bytes", "cbm_xcd_ranges_active", "cbm_set_xcd_phases", "cbm_set_tree_tails", "cbm_set_kahan_depth",
             "cbm_set_forward_split", "cbm_set_wave_rows", "cbm_set_mixed_grid", "cbm_set_xcd_slabs", "cbm_set_xcd_ranges", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",

@@ -9,9 +9,18 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
+
+// Scratch of cbm_matmul_noalloc, for callers that cannot allocate: the
+// long-row partial sums and the compensation rows of one forward product.
+// Each GPU the library runs on holds its own copy until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress gives on
+// that device (the array's name is not a device address in host code).
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) float scratch[kNoallocScratchBytes / sizeof(float)];
 
 struct cbm_handle {
     cbm_desc d{};
@@ -236,8 +245,8 @@ int check_reserved(const cbm_handle *h, int32_t n_cols, const char *what) {
 }
 
 // The device buffers a forward call works in: long-row partial sums,
-// compensation rows, range partial rows (the handle's own, sized by
-// cbm_reserve).
+// compensation rows, range partial rows.  The handle's own (cbm_reserve) or
+// the library's static scratch (cbm_matmul_noalloc).
 struct Bufs {
     float *scratch = nullptr;
     float *comp = nullptr;
@@ -245,6 +254,31 @@ struct Bufs {
 };
 
 Bufs handle_bufs(const cbm_handle *h) { return Bufs{h->scratch, h->comp, h->partial}; }
+
+// Device address of `scratch` on each device, looked up once per device, by
+// the first cbm_matmul_noalloc call with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_addr[kMaxDevices] = {};
+
+int noalloc_scratch_address(float **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return device_error(e, "cbm scratch: hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("cbm scratch: device %d beyond the %d the library serves", dev, kMaxDevices);
+        return CBM_ERR_DEVICE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return device_error(e, "cbm scratch: hipGetSymbolAddress");
+        g_scratch_addr[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch_addr[dev];
+    return CBM_OK;
+}
 
 // One ordered row range whose first n_long rows are long: the chunk kernel
 // and the short-row pass (one grid), then the reduction of multi-chunk rows.
@@ -919,6 +953,30 @@ extern "C" int cbm_propagate(cbm_handle *h, float *y, int64_t ldy, int32_t n_col
     if (rc != CBM_OK) return rc;
     h->last_launches = 0;
     return propagate_levels(h, handle_bufs(h), y, ldy, n_cols, stream, "cbm_propagate");
+}
+
+extern "C" int cbm_matmul_noalloc(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy,
+                                  int32_t n_cols, hipStream_t stream) {
+    int rc = check_dense(h, dtype, x, ldx, y, ldy, n_cols, "cbm_matmul_noalloc");
+    if (rc != CBM_OK) return rc;
+    float *base = nullptr;  // this device's copy of `scratch`
+    rc = noalloc_scratch_address(&base);
+    if (rc != CBM_OK) return rc;
+    // The long-row partial sums of the forward form in use, then the
+    // compensation rows, each start 256-byte aligned.
+    const int64_t slots = use_split(h) ? h->d.fwd_gslots : h->d.fwd_slots;
+    const int64_t part = (slots * n_cols + 63) / 64 * 64;
+    const int64_t comp = int64_t(h->d.n_rows - comp_base(h)) * n_cols;
+    const int64_t need = (part + comp) * int64_t(sizeof(float));
+    if (need > int64_t(kNoallocScratchBytes)) {
+        set_error("cbm_matmul_noalloc: %d columns need %lld bytes of scratch, the static scratch holds %lld",
+                  n_cols, (long long)need, (long long)kNoallocScratchBytes);
+        return CBM_ERR_ARG;
+    }
+    Bufs b;
+    b.scratch = base;
+    b.comp = base + part;
+    return forward_product(h, b, false, dtype, x, ldx, y, ldy, n_cols, stream, "cbm_matmul_noalloc");
 }
 
 extern "C" int cbm_last_launches(const cbm_handle *h) { return h ? h->last_launches : -1; }

@@ -223,6 +223,22 @@ int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *
                hipStream_t stream);
 
 /*
+ * Forward product Y = M X that allocates nothing and needs no cbm_reserve.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP-graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and result as cbm_matmul, except that
+ * the XCD column ranges are not used (their partial rows would not fit), so
+ * where cbm_matmul would take them the bits can differ within FP32
+ * rounding.  Its long-row partial sums and compensation rows live in a
+ * static device array of 64 MiB per GPU, held until the process ends; a call
+ * that needs more fails with CBM_ERR_ARG.  Every call on a GPU shares that
+ * array: calls that overlap on one GPU (other streams, other threads, other
+ * handles) race and must be serialised by the caller.
+ */
+int cbm_matmul_noalloc(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy,
+                       int32_t n_cols, hipStream_t stream);
+
+/*
  * Adjoint dX = M^T G.  z is caller-provided FP32 workspace N x n_cols with
  * leading dimension ldz; it receives (I-P)^-T applied to the scaled G for
  * the nodes that have children (for a leaf that row is just the scaled G,
