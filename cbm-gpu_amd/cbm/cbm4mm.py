@@ -54,6 +54,9 @@ class cbm4mm:
         self.cusparse_extension = cusparse.cusparseCBM(self.deltas, self.mca_offset, self.mca_src_idx,
                                                        self.mca_dst_idx, dummy, mode=MODE_AX, n_threads=n_threads,
                                                        plan=plan)
+        # A = A^T (the build compares them exactly): the adjoint then runs
+        # through the forward structure.
+        self.cusparse_extension.set_symmetric(bool(stats.get("symmetric", 0)))
         self._scale()
 
     def _scale(self):

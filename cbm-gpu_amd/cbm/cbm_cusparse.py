@@ -116,6 +116,7 @@ class cusparseCBM:
         self._host = {k: plan.arrays[k].contiguous() for k in HOST_ARRAYS}
         self._dev = {k: plan.arrays[k].to(self.device) for k in DEVICE_ARRAYS}
         self._lib = _lib.hip()
+        self._symmetric = False
         self._bind(mode, scale)
 
     def _bind(self, mode, scale):
@@ -141,13 +142,15 @@ class cusparseCBM:
             rc = self._lib.cbm_create(ctypes.byref(desc), current_stream_handle(self.device), ctypes.byref(h))
         _lib.check_hip(rc, "cusparseCBM")
         self._h = h
+        if self._symmetric:
+            _lib.check_hip(self._lib.cbm_set_symmetric(self._h, 1), "set_symmetric")
 
     def with_scaling(self, mode, scale=None):
         """A handle for another product (A, A D or D A D) over the same device
         layout; the layout arrays are shared, not copied."""
         other = object.__new__(type(self))
         for k in ("device", "num_rows", "depth", "nnz", "n_edges", "level_sizes", "plan_stats", "_host", "_dev",
-                  "_lib"):
+                  "_lib", "_symmetric"):
             setattr(other, k, getattr(self, k))
         if mode != MODE_AX and scale is None:
             raise ValueError(f"mode {mode} needs the D^-1/2 vector as `scale`")
@@ -240,13 +243,14 @@ class cusparseCBM:
         self._check_out(out, f, "out")
         if f == 0:
             return 0
-        if workspace is None:
-            workspace = torch.empty((self.num_rows, f), dtype=torch.float32, device=self.device)
-        else:
+        if workspace is not None:
             self._check_out(workspace, f, "workspace")
+        elif not self._symmetric:  # the forward-structure adjoint needs none
+            workspace = torch.empty((self.num_rows, f), dtype=torch.float32, device=self.device)
         self._ensure_width(f)
+        ws_ptr, ws_ld = (workspace.data_ptr(), workspace.stride(0)) if workspace is not None else (None, f)
         rc = self._lib.cbm_matmul_t(self._h, code, g.data_ptr(), g.stride(0), out.data_ptr(), out.stride(0),
-                                    workspace.data_ptr(), workspace.stride(0), f, current_stream_handle(self.device))
+                                    ws_ptr, ws_ld, f, current_stream_handle(self.device))
         _lib.check_hip(rc, "matmul_t")
         return 0
 
@@ -315,6 +319,20 @@ class cusparseCBM:
         capture; under capture the width must already be reserved."""
         _lib.check_hip(self._lib.cbm_reserve(self._h, int(n_cols)), "reserve")
         self._reserved = max(self._reserved, int(n_cols))
+
+    @property
+    def symmetric(self):
+        """Whether A is declared symmetric (then matmul_t runs through the
+        forward structure)."""
+        return self._symmetric
+
+    def set_symmetric(self, symmetric):
+        """Declares A = A^T (True) or not (False): with True, ``matmul_t``
+        computes M^T G through the forward structure (M^T = M for A and
+        D A D, D A for A D) instead of the Delta^T pass.  The cbm4* classes
+        set it from the structure build's exact comparison of A with A^T."""
+        _lib.check_hip(self._lib.cbm_set_symmetric(self._h, int(bool(symmetric))), "set_symmetric")
+        self._symmetric = bool(symmetric)
 
     def set_kahan_depth(self, level):
         """Tree levels from `level` on propagate with an FP32 compensation

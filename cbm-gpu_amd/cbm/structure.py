@@ -10,7 +10,7 @@ from . import _lib
 # Array and statistic slots of include/cbm_host.h.
 (_B_DELTA_ROW, _B_DELTA_COL, _B_DELTA_VAL, _B_BRANCHES, _B_TREE_SRC, _B_TREE_DST, _B_PARENT, _B_CSR_PTR, _B_CSR_COL,
  _B_CSR_VAL) = range(10)
-_STAT_NAMES = ("n", "nnz_a", "nnz_delta", "candidates", "msa_weight", "tree_edges", "branches", "depth")
+_STAT_NAMES = ("n", "nnz_a", "nnz_delta", "candidates", "msa_weight", "tree_edges", "branches", "depth", "symmetric")
 _PLAN_NAMES = ("lvl_ptr", "lvl_node", "lvl_parent", "fwd_ptr", "fwd_col", "bwd_ptr", "bwd_row", "ch_ptr",
                "ch_node", "lvl_nlong", "fck_ptr", "fck_pos", "fck_meta", "fck_slot", "fred_ptr", "fred_pos",
                "fred_slot", "fred_cnt", "bwd_node", "bck_pos", "bck_meta", "bck_slot", "bred_pos", "bred_slot",

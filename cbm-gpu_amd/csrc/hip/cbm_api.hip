@@ -54,6 +54,7 @@ struct cbm_handle {
     float *comp = nullptr;    // compensation rows of those levels
     int64_t comp_floats = 0;
     int32_t reserved_cols = 0; // widest product the buffers serve (cbm_reserve)
+    int32_t symmetric = 0;     // A = A^T declared: the adjoint runs the forward structure
     int last_launches = 0;
 };
 
@@ -255,6 +256,15 @@ struct Bufs {
 
 Bufs handle_bufs(const cbm_handle *h) { return Bufs{h->scratch, h->comp, h->partial}; }
 
+// The scalings a forward-structure pass applies: the handle's own product
+// (CBM_MODE_*), or kFormDAX = D^-1/2 A, which is (A D^-1/2)^T for a
+// symmetric A: Delta rows scaled by D and the propagation factor D_v / D_parent,
+// with no column scale.
+constexpr int32_t kFormDAX = 3;
+
+bool form_rowscaled(int32_t form) { return form == CBM_MODE_DADX || form == kFormDAX; }
+bool form_colscaled(int32_t form) { return form == CBM_MODE_ADX || form == CBM_MODE_DADX; }
+
 // Device address of `scratch` on each device, looked up once per device, by
 // the first cbm_matmul_noalloc call with that device current.
 constexpr int kMaxDevices = 64;
@@ -388,9 +398,9 @@ bool tails_on(const cbm_handle *h) { return h->tree_tails != 0; }
 // In-place propagation of levels 1 .. L-1, y_v += s_v y_parent(v), with the
 // compensated form from the handle's Kahan level on: one launch per level
 // down to the tree tail, then one launch for the tail.
-int propagate_levels(cbm_handle *h, const Bufs &b, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream,
-                     const char *what) {
-    const float *ps = h->d.mode == CBM_MODE_DADX ? h->ps_pos : nullptr;
+int propagate_levels(cbm_handle *h, const Bufs &b, int32_t form, float *y, int64_t ldy, int32_t n_cols,
+                     hipStream_t stream, const char *what) {
+    const float *ps = form_rowscaled(form) ? h->ps_pos : nullptr;
     cbmk::CompArgs cm;
     cm.parent_pos = h->d.lvl_ppos;
     cm.comp = b.comp;
@@ -491,16 +501,17 @@ bool use_split(const cbm_handle *h) {
     return h->d.n_levels > 1;
 }
 
-cbmk::RowPass forward_pass(const cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy,
+cbmk::RowPass forward_pass(const cbm_handle *h, int32_t form, int32_t dtype, const void *x, int64_t ldx, float *y,
+                           int64_t ldy,
                            int32_t n_cols) {
     cbmk::RowPass rp;
     rp.node_of = h->d.lvl_node;
     rp.parent_of = h->d.lvl_parent;
     rp.ptr = h->d.fwd_ptr;
     rp.idx = h->d.fwd_col;
-    rp.colscale = h->d.mode == CBM_MODE_AX ? nullptr : h->d.d;
-    rp.rowscale = h->d.mode == CBM_MODE_DADX ? h->d.d : nullptr;
-    rp.escale = h->d.mode == CBM_MODE_DADX ? h->ps_pos : nullptr;
+    rp.colscale = form_colscaled(form) ? h->d.d : nullptr;
+    rp.rowscale = form_rowscaled(form) ? h->d.d : nullptr;
+    rp.escale = form_rowscaled(form) ? h->ps_pos : nullptr;
     rp.g = x;
     rp.ldg = ldx;
     rp.g_bf16 = dtype == CBM_BF16;
@@ -659,7 +670,7 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
     h->d.rev_nlong_host = h->rev_nlong.data();
     h->d.rck_ptr_host = h->rck_ptr.data();
     h->d.rred_ptr_host = h->rred_ptr.data();
-    if (d.mode == CBM_MODE_DADX) {
+    if (d.mode != CBM_MODE_AX) {  // D A D, and D A for the adjoint of A D over a symmetric A
         hipError_t e = hipMalloc(&h->ps_pos, sizeof(float) * size_t(d.n_rows));
         if (e == hipSuccess) e = cbmk::launch_dad_scales(d.n_rows, d.lvl_node, d.lvl_parent, d.d, h->ps_pos, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -700,23 +711,23 @@ namespace {
 
 // The forward product in buffers b; ranges: whether the XCD column ranges
 // may be used (they need b.partial).
-int forward_product(cbm_handle *h, const Bufs &b, bool ranges, int32_t dtype, const void *x, int64_t ldx, float *y,
-                    int64_t ldy, int32_t n_cols, hipStream_t stream, const char *what) {
+int forward_product(cbm_handle *h, const Bufs &b, bool ranges, int32_t form, int32_t dtype, const void *x,
+                    int64_t ldx, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream, const char *what) {
     h->last_launches = 0;
     const int elem = dtype == CBM_BF16 ? 2 : 4;
     const bool split = use_split(h);
     return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
-        const cbmk::RowPass rp = forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w);
+        const cbmk::RowPass rp = forward_pass(h, form, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w);
         if (const RangeSet *rs = ranges ? pick_ranges(h, n_cols, elem) : nullptr) {
             // Pieces and one reduction over all rows, then the light level
             // propagation: the bits of the per-level reduction with the
             // parent term (forward_ranges with_parent), in fewer, fuller launches.
             const int r = forward_ranges(h, b, *rs, rp, false, stream, what);
-            return r == CBM_OK ? propagate_levels(h, b, y + c0, ldy, w, stream, what) : r;
+            return r == CBM_OK ? propagate_levels(h, b, form, y + c0, ldy, w, stream, what) : r;
         }
         if (!split) return forward_levels(h, b, rp, true, stream, what);
         const int r = delta_flat(h, b, rp, stream, what);
-        return r == CBM_OK ? propagate_levels(h, b, y + c0, ldy, w, stream, what) : r;
+        return r == CBM_OK ? propagate_levels(h, b, form, y + c0, ldy, w, stream, what) : r;
     });
 }
 
@@ -727,7 +738,7 @@ extern "C" int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t l
     int rc = check_dense(h, dtype, x, ldx, y, ldy, n_cols, "cbm_matmul");
     if (rc == CBM_OK) rc = check_reserved(h, n_cols, "cbm_matmul");
     if (rc != CBM_OK) return rc;
-    return forward_product(h, handle_bufs(h), true, dtype, x, ldx, y, ldy, n_cols, stream, "cbm_matmul");
+    return forward_product(h, handle_bufs(h), true, h->d.mode, dtype, x, ldx, y, ldy, n_cols, stream, "cbm_matmul");
 }
 
 namespace {
@@ -899,6 +910,16 @@ int adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, co
 
 extern "C" int cbm_matmul_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, float *dx, int64_t lddx,
                             float *z, int64_t ldz, int32_t n_cols, hipStream_t stream) {
+    if (h && h->symmetric) {
+        // A = A^T: M^T = M for A and D A D, and (A D)^T = D A, all through
+        // the forward structure (no Delta^T pass, no reverse gather, z unused).
+        int rc = check_dense(h, dtype, g, ldg, dx, lddx, n_cols, "cbm_matmul_t");
+        if (rc == CBM_OK) rc = check_reserved(h, n_cols, "cbm_matmul_t");
+        if (rc != CBM_OK) return rc;
+        const int32_t form = h->d.mode == CBM_MODE_ADX ? kFormDAX : h->d.mode;
+        return forward_product(h, handle_bufs(h), true, form, dtype, g, ldg, dx, lddx, n_cols, stream,
+                               "cbm_matmul_t");
+    }
     int rc = check_adjoint(h, dtype, g, ldg, dx, lddx, z, ldz, n_cols, "cbm_matmul_t");
     if (rc != CBM_OK) return rc;
     h->last_launches = 0;
@@ -937,7 +958,7 @@ extern "C" int cbm_delta_spmm(cbm_handle *h, int32_t dtype, const void *x, int64
     h->last_launches = 0;
     const int elem = dtype == CBM_BF16 ? 2 : 4;
     return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
-        const cbmk::RowPass rp = forward_pass(h, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w);
+        const cbmk::RowPass rp = forward_pass(h, h->d.mode, dtype, shift<char>(x, c0, elem), ldx, y + c0, ldy, w);
         if (const RangeSet *rs = pick_ranges(h, n_cols, elem))
             return forward_ranges(h, handle_bufs(h), *rs, rp, false, stream, "cbm_delta_spmm");
         return delta_flat(h, handle_bufs(h), rp, stream, "cbm_delta_spmm");
@@ -952,7 +973,7 @@ extern "C" int cbm_propagate(cbm_handle *h, float *y, int64_t ldy, int32_t n_col
     const int rc = check_reserved(h, n_cols, "cbm_propagate");
     if (rc != CBM_OK) return rc;
     h->last_launches = 0;
-    return propagate_levels(h, handle_bufs(h), y, ldy, n_cols, stream, "cbm_propagate");
+    return propagate_levels(h, handle_bufs(h), h->d.mode, y, ldy, n_cols, stream, "cbm_propagate");
 }
 
 extern "C" int cbm_matmul_noalloc(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy,
@@ -976,7 +997,7 @@ extern "C" int cbm_matmul_noalloc(cbm_handle *h, int32_t dtype, const void *x, i
     Bufs b;
     b.scratch = base;
     b.comp = base + part;
-    return forward_product(h, b, false, dtype, x, ldx, y, ldy, n_cols, stream, "cbm_matmul_noalloc");
+    return forward_product(h, b, false, h->d.mode, dtype, x, ldx, y, ldy, n_cols, stream, "cbm_matmul_noalloc");
 }
 
 extern "C" int cbm_last_launches(const cbm_handle *h) { return h ? h->last_launches : -1; }
@@ -1033,6 +1054,15 @@ extern "C" int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode) {
     }
     h->xcd_ranges = mode;
     return grow_buffers(h, h->reserved_cols);  // a piece set needs its partial rows
+}
+
+extern "C" int cbm_set_symmetric(cbm_handle *h, int32_t symmetric) {
+    if (!h || symmetric < 0 || symmetric > 1) {
+        set_error("cbm_set_symmetric: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    h->symmetric = symmetric;
+    return CBM_OK;
 }
 
 extern "C" int cbm_set_tree_tails(cbm_handle *h, int32_t mode) {

@@ -276,7 +276,7 @@ struct ThreadScope {
 // `owned`: the library's own candidate list, released once the parents are
 // known; null when the caller supplied the list (then inconsistent weights
 // are the caller's error).
-int build_from(const Csr &a, const int32_t *csrc, const int32_t *cdst, const int32_t *cw, int64_t m,
+int build_from(const Csr &a, bool symmetric, const int32_t *csrc, const int32_t *cdst, const int32_t *cw, int64_t m,
                Candidates *owned, cbm_build **out) {
     const int32_t n = a.n;
     try {
@@ -292,6 +292,7 @@ int build_from(const Csr &a, const int32_t *csrc, const int32_t *cdst, const int
         b->stats[CBM_S_NNZ_A] = int64_t(a.col.size());
         b->stats[CBM_S_CANDIDATES] = m;
         b->stats[CBM_S_MSA_WEIGHT] = msa.weight;
+        b->stats[CBM_S_SYMMETRIC] = symmetric ? 1 : 0;
 
         std::vector<int32_t> &parent = b->arr_i32[CBM_B_PARENT];
         parent.resize(size_t(n));
@@ -434,7 +435,7 @@ extern "C" int cbm_build_create(const int32_t *row, const int32_t *col, const fl
         scan_candidates(adj.a, adj.columns(), alpha, cand);
         std::vector<int64_t>().swap(adj.at.ptr);
         std::vector<int32_t>().swap(adj.at.col);
-        return build_from(adj.a, cand.src.data(), cand.dst.data(), cand.w.data(), int64_t(cand.src.size()), &cand,
+        return build_from(adj.a, adj.symmetric, cand.src.data(), cand.dst.data(), cand.w.data(), int64_t(cand.src.size()), &cand,
                           out);
     } catch (const std::bad_alloc &) {
         set_error("init: out of host memory");
@@ -524,7 +525,7 @@ extern "C" int cbm_build_from_candidates(const cbm_adjacency *adj, const int32_t
         }
     }
     ThreadScope threads(n_threads);
-    return build_from(adj->a, src, dst, w, n_cand, nullptr, out);
+    return build_from(adj->a, adj->symmetric, src, dst, w, n_cand, nullptr, out);
 }
 
 extern "C" int64_t cbm_build_size(const cbm_build *b, int32_t which) {

@@ -243,6 +243,10 @@ int cbm_matmul_noalloc(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx,
  * leading dimension ldz; it receives (I-P)^-T applied to the scaled G for
  * the nodes that have children (for a leaf that row is just the scaled G,
  * which the kernels read from g directly; z's leaf rows are left untouched).
+ * On a handle declared symmetric (cbm_set_symmetric) the adjoint runs
+ * through the forward structure instead -- M^T = M for A and D A D, and
+ * (A D)^T = D A, i.e. the forward passes with Delta's rows scaled by D and
+ * the propagation factor D_v / D_parent -- and z may be NULL (it is unused).
  * No reference counterpart on the GPU; the CPU op s_spmm_csr_t_int32
  * (cbm/cbm_extensions.cpp:742-782) is the closest.
  */
@@ -356,6 +360,18 @@ int cbm_set_xcd_slabs(cbm_handle *h, int32_t slabs);
  * cbm_reserve sizes it for the current choice of either storage type.
  */
 int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode);
+
+/*
+ * Declares that the matrix A of the handle is symmetric (1) or not (0, the
+ * default).  With A = A^T, cbm_matmul_t computes M^T G through the forward
+ * structure: the forward's Delta pass and propagation over G (for A D, with
+ * the scalings of D A), which gathers only G rows -- no FP32 intermediate
+ * rows as the Delta^T pass does -- and needs no workspace.  Rounding differs
+ * from the Delta^T form (another summation order); both are deterministic.
+ * The cbm4* classes set it from the structure build, which compares A with
+ * A^T exactly; a wrong declaration gives a wrong adjoint.
+ */
+int cbm_set_symmetric(cbm_handle *h, int32_t symmetric);
 
 /*
  * Tree tails (1 on, the default; 0 off).  The deepest tree levels of the

@@ -86,7 +86,8 @@ enum {
     CBM_S_TREE_EDGES = 5,   /* E, edges whose parent is a real node                          */
     CBM_S_BRANCHES = 6,     /* nb, non-trivial branches                                     */
     CBM_S_DEPTH = 7,        /* levels of the forest (a forest of singletons has depth 1)    */
-    CBM_S_N_STATS = 8
+    CBM_S_SYMMETRIC = 8,    /* 1 when A = A^T (then M^T runs through the forward structure) */
+    CBM_S_N_STATS = 9
 };
 
 typedef struct cbm_build cbm_build;

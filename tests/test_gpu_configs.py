@@ -185,12 +185,21 @@ def test_c4_layout_reaches_the_size_dependent_paths(c4):
     assert h.xcd_ranges_active(torch.bfloat16, 64) == 0
 
 
+@pytest.mark.parametrize("form", ["forward_structure", "delta_t"])
 @pytest.mark.parametrize("f", [64, 256])
-def test_c4_ad_full_graph(c4, f):
+def test_c4_ad_full_graph(c4, f, form):
     """cbm4ad, N = 132,534, alpha = 32, BF16 and FP32 with the automatic XCD
-    ranges; Kahan levels 32..51 lie inside the forward tail."""
+    ranges; Kahan levels 32..51 lie inside the forward tail.  The adjoint
+    (A D)^T = D A through the forward structure (the default: A is
+    symmetric) and through the Delta^T pass."""
     wl, mat = c4
-    _check_products(wl, mat, "adx", f)
+    h = mat.cusparse_extension
+    assert h.symmetric
+    h.set_symmetric(form == "forward_structure")
+    try:
+        _check_products(wl, mat, "adx", f)
+    finally:
+        h.set_symmetric(True)
 
 
 def test_c4_integer_exact(c4):

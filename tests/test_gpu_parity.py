@@ -63,14 +63,21 @@ def test_forward_fp32_vs_oracle(mats, graph1, cuda, mode, f):
     assert err <= 1e-5, f"{mode} F={f}: rel err {err:.3e}"
 
 
+@pytest.mark.parametrize("form", ["forward_structure", "delta_t"])
 @pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
 @pytest.mark.parametrize("f", [1, 8, 64, 256])
-def test_adjoint_fp32_vs_oracle(mats, graph1, cuda, mode, f):
+def test_adjoint_fp32_vs_oracle(mats, graph1, cuda, mode, f, form):
+    """M^T G both ways: through the forward structure (A = A^T, the default
+    for these symmetric graphs) and through the Delta^T pass."""
     _, a = graph1
     m, _ = mats[mode]
+    h = m.cusparse_extension
+    assert h.symmetric  # the build found A = A^T
+    h.set_symmetric(form == "forward_structure")
     g = _rand((N1, f), 3000 + f)
     dx = torch.empty(N1, f, device=cuda)
     m.matmul_t(torch.from_numpy(g).to(cuda), dx)
+    h.set_symmetric(True)
     err = O.max_rel_error(dx.cpu().numpy(), O.adjoint(a, g, mode))
     assert err <= 1e-5, f"{mode} F={f}: rel err {err:.3e}"
 
@@ -78,7 +85,8 @@ def test_adjoint_fp32_vs_oracle(mats, graph1, cuda, mode, f):
 @pytest.mark.parametrize("f", [1, 5, 64, 128])
 def test_integer_x_is_bitwise_exact(mats, graph1, cuda, f):
     """Small-integer X makes every partial sum exact in FP32, so any correct
-    evaluation order returns the exact integer result."""
+    evaluation order returns the exact integer result (the adjoint both
+    through the forward structure and through the Delta^T pass)."""
     _, a = graph1
     m, _ = mats["ax"]
     rng = np.random.default_rng(77 + f)
@@ -87,9 +95,12 @@ def test_integer_x_is_bitwise_exact(mats, graph1, cuda, f):
     m.matmul(torch.from_numpy(x).to(cuda), y)
     exact = O.forward(a, x, "ax")
     assert np.array_equal(y.cpu().numpy().astype(np.float64), exact)
-    dx = torch.empty(N1, f, device=cuda)
-    m.matmul_t(torch.from_numpy(x).to(cuda), dx)
-    assert np.array_equal(dx.cpu().numpy().astype(np.float64), O.adjoint(a, x, "ax"))
+    for sym in (True, False):
+        m.cusparse_extension.set_symmetric(sym)
+        dx = torch.empty(N1, f, device=cuda)
+        m.matmul_t(torch.from_numpy(x).to(cuda), dx)
+        assert np.array_equal(dx.cpu().numpy().astype(np.float64), O.adjoint(a, x, "ax")), sym
+    m.cusparse_extension.set_symmetric(True)
 
 
 @pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
@@ -124,11 +135,14 @@ def test_adjoint_vs_rocsparse_transpose(mats, cuda, mode):
     absolute floor of 1e-5 * max|M^T G|."""
     m, c = mats[mode]
     g = torch.from_numpy(_rand((N1, 32), 91, "uniform")).to(cuda)
-    d_cbm = torch.empty(N1, 32, device=cuda)
     d_csr = torch.empty(N1, 32, device=cuda)
-    m.matmul_t(g, d_cbm)
     c.matmul_t(g, d_csr)
-    torch.testing.assert_close(d_csr, d_cbm, rtol=1e-5, atol=1e-5 * float(d_csr.abs().max()))
+    for sym in (True, False):
+        m.cusparse_extension.set_symmetric(sym)
+        d_cbm = torch.empty(N1, 32, device=cuda)
+        m.matmul_t(g, d_cbm)
+        torch.testing.assert_close(d_csr, d_cbm, rtol=1e-5, atol=1e-5 * float(d_csr.abs().max()))
+    m.cusparse_extension.set_symmetric(True)
 
 
 @pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])

@@ -468,3 +468,18 @@ def test_chain_and_star_tails():
     if pl.depth > 8:
         assert pl.stats["tf_level"] == 1 and pl.tf_frag.numel() == 2
         assert pl.stats["tr_level"] < pl.depth
+
+
+def test_build_reports_whether_a_is_symmetric():
+    """The build's symmetric flag (exact A == A^T) is what lets the adjoint
+    run through the forward structure: set for a symmetrised graph, clear
+    for a directed one and for a row subset."""
+    ei = structure.copying_model(500, 3, 0.8, 1.0, 0, 9)
+    b = structure.build(ei[0], ei[1], torch.ones(ei.shape[1]), 500, 0)
+    assert b.stats["symmetric"] == 1
+    keep = ei[0] != 0  # drop row 0 only: A is no longer symmetric
+    b = structure.build(ei[0][keep], ei[1][keep], torch.ones(int(keep.sum())), 500, 0)
+    assert b.stats["symmetric"] == 0
+    with structure.Adjacency(ei[0], ei[1], torch.ones(ei.shape[1]), 500) as adj:
+        src, dst, w = structure.host_candidates(adj, 0)
+        assert structure.build_from_candidates(adj, src, dst, w).stats["symmetric"] == 1
