@@ -108,6 +108,16 @@ class CbmDesc(ctypes.Structure):
         ("tr_frag", c_vp),
         ("tr_seg", c_vp),
         ("tr_pos", c_vp),
+        ("n_units", c_i32),
+        ("un_ptr", c_vp),
+        ("un_row", c_vp),
+        ("un_lpar", c_vp),
+        ("un_pc", c_vp),
+        ("un_pcb", c_vp),
+        ("un_pce", c_vp),
+        ("un_seg_ptr", c_vp),
+        ("un_seg", c_vp),
+        ("un_of", c_vp),
     ]
 
 # Device arrays of the plan that the descriptor points at.
@@ -117,7 +127,8 @@ DEVICE_ARRAYS = ("lvl_node", "lvl_parent", "lvl_ppos", "fwd_ptr", "fwd_col", "re
                  "rred_slot", "rred_cnt", "fpc_b", "fpc_e", "fpc_slot", "frs_slot", "frs_cnt", "bpc_b", "bpc_e", "bpc_mid",
                  "bpc_slot", "brs_slot", "brs_cnt", "bpc_row", "fpc4_b", "fpc4_e", "fpc4_slot", "frs4_slot", "frs4_cnt",
                  "bpc4_b", "bpc4_e", "bpc4_mid", "bpc4_slot", "brs4_slot", "brs4_cnt", "bpc4_row", "tf_frag", "tf_seg",
-                 "tf_pos", "tr_frag", "tr_seg", "tr_pos")
+                 "tf_pos", "tr_frag", "tr_seg", "tr_pos", "un_ptr", "un_row", "un_lpar", "un_pc", "un_pcb", "un_pce",
+                 "un_seg_ptr", "un_seg", "un_of")
 HOST_ARRAYS = ("lvl_ptr", "lvl_nlong", "fck_ptr", "fred_ptr", "rev_ptr", "rev_nlong", "rck_ptr", "rred_ptr", "fpc_range",
                "bpc_range", "fpc4_range", "bpc4_range")
 
@@ -142,6 +153,9 @@ def declare(lib, sig):
     sig(lib, "cbm_set_xcd_phases", c_i32, c_vp, c_i32)
     sig(lib, "cbm_set_tree_tails", c_i32, c_vp, c_i32)
     sig(lib, "cbm_set_symmetric", c_i32, c_vp, c_i32)
+    sig(lib, "cbm_set_fused", c_i32, c_vp, c_i32)
+    sig(lib, "cbm_fused_active", c_i32, c_vp, c_i32, c_i32)
+    sig(lib, "cbm_fused_errors", c_i64, c_vp)
     sig(lib, "cbm_set_kahan_depth", c_i32, c_vp, c_i32)
     sig(lib, "cbm_set_forward_split", c_i32, c_vp, c_i32)
     sig(lib, "cbm_set_wave_rows", c_i32, c_vp, c_i32)
@@ -172,7 +186,7 @@ def declare(lib, sig):
 # Every symbol include/cbm_hip.h declares (checked by the CPU test suite).
 EXPORTED = ("cbm_last_error", "cbm_desc_size", "cbm_create", "cbm_destroy", "cbm_reserve", "cbm_matmul", "cbm_matmul_noalloc", "cbm_matmul_t", "cbm_adjoint_reverse",
             "cbm_adjoint_delta_t", "cbm_delta_spmm",
-            "cbm_propagate", "cbm_last_launches", "cbm_set_slab_bytes", "cbm_xcd_ranges_active", "cbm_set_xcd_phases", "cbm_set_tree_tails", "cbm_set_symmetric", "cbm_set_kahan_depth",
+            "cbm_propagate", "cbm_last_launches", "cbm_set_slab_bytes", "cbm_xcd_ranges_active", "cbm_set_xcd_phases", "cbm_set_tree_tails", "cbm_set_symmetric", "cbm_set_fused", "cbm_fused_active", "cbm_fused_errors", "cbm_set_kahan_depth",
             "cbm_set_forward_split", "cbm_set_wave_rows", "cbm_set_mixed_grid", "cbm_set_xcd_slabs", "cbm_set_xcd_ranges", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
             "csr_last_error", "cbm_op_spmm", "cbm_op_update", "cbm_op_axpy", "cbm_op_last_error",
             "cbm_candidates_workspace_bytes", "cbm_candidates_count", "cbm_candidates_emit",

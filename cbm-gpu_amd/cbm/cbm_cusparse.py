@@ -135,6 +135,7 @@ class cusparseCBM:
                        tf_level=st.get("tf_level", self.depth), tr_level=st.get("tr_level", self.depth),
                        tf_frags=max(0, int(self._dev["tf_frag"].numel()) - 1),
                        tr_frags=max(0, int(self._dev["tr_frag"].numel()) - 1),
+                       n_units=max(0, int(self._dev["un_ptr"].numel()) - 1),
                        d=self._d.data_ptr() if self._d is not None else None, **ptrs)
         self._reserved = 0
         h = ctypes.c_void_p()
@@ -333,6 +334,30 @@ class cusparseCBM:
         set it from the structure build's exact comparison of A with A^T."""
         _lib.check_hip(self._lib.cbm_set_symmetric(self._h, int(bool(symmetric))), "set_symmetric")
         self._symmetric = bool(symmetric)
+
+    def set_fused(self, mode):
+        """The forward product in one launch (units of the plan, propagation
+        in LDS): None automatic (the default), False off, True on where
+        possible.  Rounds each row in pieces, so bits differ from the
+        multi-launch forms."""
+        m = -1 if mode is None else int(bool(mode))
+        _lib.check_hip(self._lib.cbm_set_fused(self._h, m), "set_fused")
+
+    def fused_active(self, dtype, n_cols):
+        """Whether a product with dtype storage and n_cols columns runs in
+        one launch now (the width must be reserved)."""
+        if dtype not in (torch.float32, torch.bfloat16):
+            raise TypeError(f"dtype must be float32 or bfloat16, got {dtype}")
+        code = self._lib.cbm_fused_active(self._h, BF16 if dtype == torch.bfloat16 else F32, int(n_cols))
+        _lib.check_hip(min(code, 0), "fused_active")
+        return bool(code)
+
+    def fused_errors(self):
+        """Dependency waits of the one-launch forward that hit their poll
+        limit since creation (0 in a correct run); synchronises the device."""
+        n = int(self._lib.cbm_fused_errors(self._h))
+        _lib.check_hip(min(n, 0), "fused_errors")
+        return n
 
     def set_kahan_depth(self, level):
         """Tree levels from `level` on propagate with an FP32 compensation

@@ -5,9 +5,9 @@ a file here holds everything a ``cbm4*`` object needs, so loading skips the
 candidate scan, the arborescence and the device layout.
 
 Format: one uncompressed ``.npz`` (numpy's zip of ``.npy`` arrays, read with
-``allow_pickle=False``), version 3 (version 1 lacked the layout's 4-range
-piece set, versions 1 and 2 its tree tails; such files still load, without
-them):
+``allow_pickle=False``), version 4 (version 1 lacked the layout's 4-range
+piece set, versions 1 and 2 its tree tails, versions 1 to 3 the units of the
+one-launch forward; such files still load, without them):
 
   meta               uint8   JSON text: format, version, n, alpha, the class
                              that saved it, build and layout statistics, the
@@ -34,13 +34,15 @@ import torch
 from . import structure
 
 FORMAT = "cbm-structure"
-VERSION = 3
+VERSION = 4
 # Layout arrays (and statistics) that older files do not have, by version.
 _NEW_IN_2 = ("fpc4_b", "fpc4_e", "fpc4_slot", "fpc4_range", "frs4_slot", "frs4_cnt", "bpc4_b", "bpc4_e", "bpc4_mid",
              "bpc4_slot", "bpc4_range", "brs4_slot", "brs4_cnt", "bpc4_row")
 _NEW_STATS_IN_2 = ("fwd_pieces4", "bwd_pieces4")
 _NEW_IN_3 = ("tf_frag", "tf_seg", "tf_pos", "tr_frag", "tr_seg", "tr_pos")
 _NEW_STATS_IN_3 = ("tf_level", "tr_level")  # absent: no tail (the depth)
+_NEW_IN_4 = ("un_ptr", "un_row", "un_lpar", "un_pc", "un_pcb", "un_pce", "un_seg_ptr", "un_seg", "un_of")
+_NEW_STATS_IN_4 = ("units", "unit_levels")  # absent: no units (0)
 _CORE = ("delta_crow", "delta_col", "delta_sign", "mca_offset", "mca_src_idx", "mca_dst_idx")
 _DTYPES = {"delta_crow": np.int32, "delta_col": np.int32, "delta_sign": np.int8, "mca_offset": np.int32,
            "mca_src_idx": np.int32, "mca_dst_idx": np.int32}
@@ -117,9 +119,10 @@ def load(path):
         if meta.get("format") != FORMAT:
             raise StructureFileError(f"{path}: format {meta.get('format')!r}, expected {FORMAT!r}")
         version = meta.get("version")
-        if version not in (1, 2, VERSION):
+        if version not in (1, 2, 3, VERSION):
             raise StructureFileError(f"{path}: version {version}, this library reads 1 to {VERSION}")
-        absent = (set(_NEW_IN_2) if version < 2 else set()) | (set(_NEW_IN_3) if version < 3 else set())
+        absent = ((set(_NEW_IN_2) if version < 2 else set()) | (set(_NEW_IN_3) if version < 3 else set())
+                  | (set(_NEW_IN_4) if version < 4 else set()))
         names = list(_CORE) + ["plan/" + k for k in structure.PLAN_NAMES if k not in absent]
         arrays = {"plan/" + k: np.zeros(0, np.int32) for k in absent}
         for name in names:
@@ -146,6 +149,8 @@ def load(path):
         stats.setdefault(k, 0)
     for k in _NEW_STATS_IN_3:
         stats.setdefault(k, int(meta["plan_depth"]))
+    for k in _NEW_STATS_IN_4:
+        stats.setdefault(k, 0)
     plan = structure.Plan(n=n, depth=int(meta["plan_depth"]),
                           arrays={k: t["plan/" + k] for k in structure.PLAN_NAMES}, stats=stats)
     return Stored(n=n, alpha=int(meta["alpha"]), saved_by=meta.get("saved_by", ""), delta_crow=t["delta_crow"],

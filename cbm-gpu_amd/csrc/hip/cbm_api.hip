@@ -55,6 +55,10 @@ struct cbm_handle {
     int64_t comp_floats = 0;
     int32_t reserved_cols = 0; // widest product the buffers serve (cbm_reserve)
     int32_t symmetric = 0;     // A = A^T declared: the adjoint runs the forward structure
+    int32_t fused = -1;        // one-launch forward: 1 on where possible, 0 off, -1 automatic
+    uint32_t *fused_state = nullptr; // ticket, done, epoch, failed waits (cbm_reserve)
+    uint32_t *fused_flags = nullptr; // a flag per unit and 64-column slab
+    int64_t fused_flag_count = 0;
     int last_launches = 0;
 };
 
@@ -211,8 +215,42 @@ int ensure_partial(cbm_handle *h, int32_t n_cols) {
     return CBM_OK;
 }
 
+// The state words and flags of the one-launch forward for n_cols columns
+// (all zero when allocated; the kernel keeps them consistent after that).
+int ensure_fused(cbm_handle *h, int32_t n_cols) {
+    if (h->d.n_units <= 0 || n_cols <= 0) return CBM_OK;
+    hipError_t e = hipSuccess;
+    if (!h->fused_state) {
+        e = hipMalloc(&h->fused_state, 4 * sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemset(h->fused_state, 0, 4 * sizeof(uint32_t));
+        if (e != hipSuccess) {
+            if (h->fused_state) (void)hipFree(h->fused_state);
+            h->fused_state = nullptr;
+            return device_error(e, "cbm fused state");
+        }
+    }
+    const int64_t need = int64_t(h->d.n_units) * ((n_cols + 63) / 64);
+    if (need <= h->fused_flag_count) return CBM_OK;
+    if (h->fused_flags) {
+        e = hipFree(h->fused_flags);
+        h->fused_flags = nullptr;
+        h->fused_flag_count = 0;
+        if (e != hipSuccess) return device_error(e, "cbm fused flags");
+    }
+    e = hipMalloc(&h->fused_flags, sizeof(uint32_t) * size_t(need));
+    if (e == hipSuccess) e = hipMemset(h->fused_flags, 0, sizeof(uint32_t) * size_t(need));
+    if (e != hipSuccess) {
+        if (h->fused_flags) (void)hipFree(h->fused_flags);
+        h->fused_flags = nullptr;
+        return device_error(e, "cbm fused flags");
+    }
+    h->fused_flag_count = need;
+    return CBM_OK;
+}
+
 int grow_buffers(cbm_handle *h, int32_t n_cols) {
-    int rc = ensure_comp(h, n_cols);
+    int rc = ensure_fused(h, n_cols);
+    if (rc == CBM_OK) rc = ensure_comp(h, n_cols);
     if (rc == CBM_OK) rc = ensure_partial(h, n_cols);
     if (rc != CBM_OK) return rc;
     const int64_t slots = std::max<int64_t>({h->d.fwd_slots, h->d.bwd_slots, h->d.fwd_gslots, h->d.rev_slots});
@@ -602,6 +640,11 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
             return CBM_ERR_ARG;
         }
     }
+    if (d.n_units < 0 || (d.n_units > 0 && (!d.un_ptr || !d.un_row || !d.un_lpar || !d.un_pc || !d.un_pcb ||
+                                             !d.un_pce || !d.un_seg_ptr || !d.un_seg || !d.un_of))) {
+        set_error("cbm_create: incomplete unit tables (%d units)", d.n_units);
+        return CBM_ERR_ARG;
+    }
     RangeSet sets[2];
     {
         RangeSet &a = sets[0], &b = sets[1];
@@ -686,7 +729,9 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
 extern "C" int cbm_destroy(cbm_handle *h) {
     if (!h) return CBM_OK;
     hipError_t e = hipSuccess;
-    for (float *p : {h->ps_pos, h->scratch, h->comp, h->partial}) {
+    for (void *p : {static_cast<void *>(h->ps_pos), static_cast<void *>(h->scratch), static_cast<void *>(h->comp),
+                    static_cast<void *>(h->partial), static_cast<void *>(h->fused_state),
+                    static_cast<void *>(h->fused_flags)}) {
         if (!p) continue;
         const hipError_t ei = hipFree(p);
         if (e == hipSuccess) e = ei;
@@ -709,11 +754,62 @@ extern "C" int cbm_reserve(cbm_handle *h, int32_t n_cols) {
 
 namespace {
 
+// Whether a forward-structure product of n_cols columns in this storage type
+// runs as the one-launch kernel (cbm_set_fused).  Automatic: whenever it can
+// -- it needs the plan's units, which are built only for matrices of up to
+// CBM_UNIT_MAX_ROWS rows whose Delta rows fit a unit -- and Delta holds at
+// most kFusedAutoEntries entries, where the passes are bound by the latency
+// of their launches and dependent levels rather than by bandwidth.
+constexpr int64_t kFusedAutoEntries = int64_t(1) << 22;
+
+bool fused_possible(const cbm_handle *h, int32_t dtype, int32_t n_cols, bool ranges) {
+    if (h->fused == 0 || h->d.n_units <= 0 || !h->fused_state) return false;
+    if (int64_t(h->d.n_units) * ((n_cols + 63) / 64) > h->fused_flag_count) return false;
+    const bool compensated = h->kahan_level >= 0 && std::max<int32_t>(h->kahan_level, 1) < h->d.n_levels;
+    if (compensated || h->slab_bytes > 0 || h->xcd_slabs > 1) return false;
+    if (ranges && pick_ranges(h, n_cols, elem_of(dtype))) return false;
+    return h->fused == 1 || h->d.nnz <= kFusedAutoEntries;
+}
+
 // The forward product in buffers b; ranges: whether the XCD column ranges
-// may be used (they need b.partial).
+// may be used (they need b.partial); fused: whether the one-launch kernel may
+// be used (it needs the handle's flags).
 int forward_product(cbm_handle *h, const Bufs &b, bool ranges, int32_t form, int32_t dtype, const void *x,
-                    int64_t ldx, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream, const char *what) {
+                    int64_t ldx, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream, const char *what,
+                    bool fused = true) {
     h->last_launches = 0;
+    if (fused && fused_possible(h, dtype, n_cols, ranges)) {
+        cbmk::FusedArgs a;
+        a.un_ptr = h->d.un_ptr;
+        a.un_row = h->d.un_row;
+        a.un_lpar = h->d.un_lpar;
+        a.un_pc = h->d.un_pc;
+        a.un_pcb = h->d.un_pcb;
+        a.un_pce = h->d.un_pce;
+        a.un_seg_ptr = h->d.un_seg_ptr;
+        a.un_seg = h->d.un_seg;
+        a.un_of = h->d.un_of;
+        a.node_of = h->d.lvl_node;
+        a.parent_of = h->d.lvl_parent;
+        a.parent_pos = h->d.lvl_ppos;
+        a.idx = h->d.fwd_col;
+        a.colscale = form_colscaled(form) ? h->d.d : nullptr;
+        a.rowscale = form_rowscaled(form) ? h->d.d : nullptr;
+        a.escale = form_rowscaled(form) ? h->ps_pos : nullptr;
+        a.x = x;
+        a.ldx = ldx;
+        a.y = y;
+        a.ldy = ldy;
+        a.ncols = n_cols;
+        a.n_units = h->d.n_units;
+        a.n_slabs = (n_cols + 63) / 64;
+        a.state = h->fused_state;
+        a.flags = h->fused_flags;
+        const hipError_t e = cbmk::launch_fused_forward(a, dtype == CBM_BF16, stream);
+        if (e != hipSuccess) return device_error(e, what);
+        h->last_launches = 1;
+        return CBM_OK;
+    }
     const int elem = dtype == CBM_BF16 ? 2 : 4;
     const bool split = use_split(h);
     return for_each_slab(h, n_cols, elem, [&](int32_t c0, int32_t w) {
@@ -997,7 +1093,8 @@ extern "C" int cbm_matmul_noalloc(cbm_handle *h, int32_t dtype, const void *x, i
     Bufs b;
     b.scratch = base;
     b.comp = base + part;
-    return forward_product(h, b, false, h->d.mode, dtype, x, ldx, y, ldy, n_cols, stream, "cbm_matmul_noalloc");
+    return forward_product(h, b, false, h->d.mode, dtype, x, ldx, y, ldy, n_cols, stream, "cbm_matmul_noalloc",
+                           false);
 }
 
 extern "C" int cbm_last_launches(const cbm_handle *h) { return h ? h->last_launches : -1; }
@@ -1054,6 +1151,35 @@ extern "C" int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode) {
     }
     h->xcd_ranges = mode;
     return grow_buffers(h, h->reserved_cols);  // a piece set needs its partial rows
+}
+
+extern "C" int cbm_set_fused(cbm_handle *h, int32_t mode) {
+    if (!h || mode < -1 || mode > 1) {
+        set_error("cbm_set_fused: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    h->fused = mode;
+    return CBM_OK;
+}
+
+extern "C" int cbm_fused_active(const cbm_handle *h, int32_t dtype, int32_t n_cols) {
+    if (!h || (dtype != CBM_F32 && dtype != CBM_BF16) || n_cols < 1) {
+        set_error("cbm_fused_active: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    return fused_possible(h, dtype, n_cols, true) ? 1 : 0;
+}
+
+extern "C" int64_t cbm_fused_errors(cbm_handle *h) {
+    if (!h) {
+        set_error("cbm_fused_errors: null handle");
+        return CBM_ERR_ARG;
+    }
+    if (!h->fused_state) return 0;
+    uint32_t words[4] = {};
+    const hipError_t e = hipMemcpy(words, h->fused_state, sizeof words, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return device_error(e, "cbm_fused_errors");
+    return int64_t(words[3]);
 }
 
 extern "C" int cbm_set_symmetric(cbm_handle *h, int32_t symmetric) {

@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "../../../include/cbm_host.h"
+
 namespace cbmk {
 
 // Where the optional per-row extra term of a row pass comes from.
@@ -177,6 +179,32 @@ hipError_t launch_propagate_tail(const TailArgs &t, const int32_t *node_of, cons
 // a node's internal children sit one segment deeper in its own fragment).
 // Each row is summed by one group of lanes, in entry order.
 hipError_t launch_row_tail(const RowPass &rp, const TailArgs &t, hipStream_t stream);
+
+// The forward product in one launch over the plan's units (CBM_P_UN_*):
+// Delta pieces, propagation within each unit in LDS, outside parents after
+// their unit's flag; see fused_forward.hip.  The scalings are those of a
+// RowPass (colscale by gathered row, rowscale by output row, escale by
+// position).  state: 4 uint32 in device memory, zero before the first call
+// (ticket, done, epoch, failed waits); flags: n_units x n_slabs uint32.
+struct FusedArgs {
+    const int32_t *un_ptr = nullptr, *un_row = nullptr, *un_lpar = nullptr, *un_pc = nullptr;
+    const int32_t *un_pcb = nullptr, *un_pce = nullptr, *un_seg_ptr = nullptr, *un_seg = nullptr;
+    const int32_t *un_of = nullptr;
+    const int32_t *node_of = nullptr, *parent_of = nullptr, *parent_pos = nullptr;
+    const int32_t *idx = nullptr;  // FWD_COL (packed signs)
+    const float *colscale = nullptr, *rowscale = nullptr, *escale = nullptr;
+    const void *x = nullptr;
+    int64_t ldx = 0;
+    float *y = nullptr;
+    int64_t ldy = 0;
+    int32_t ncols = 0;
+    int32_t n_units = 0;
+    int32_t n_slabs = 0;  // ceil(ncols / 64)
+    uint32_t *state = nullptr;
+    uint32_t *flags = nullptr;
+    uint32_t spin_limit = 1u << 20;
+};
+hipError_t launch_fused_forward(const FusedArgs &a, bool x_bf16, hipStream_t stream);
 
 // The reference's branch-serial propagation (cbm/cbm.cu:25-91) for the
 // operator surface: one workgroup per branch and column block walks the

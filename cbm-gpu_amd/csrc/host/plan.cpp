@@ -189,6 +189,163 @@ bool range_pieces(int32_t n, const std::vector<int32_t> &ptr, const std::vector<
 // more (many small fragments: N = 2,449,029 with 396k tree edges measured
 // 0.13 ms per-level against 0.53 ms as a tail from level 1).  No tail starts
 // above min_level; none is built (level = L) unless it saves time.
+// Units of the one-launch forward (kernel k_fused_forward): the tree is cut
+// into connected pieces ("components", a node with some of its descendants)
+// of at most CBM_UNIT_ROWS rows whose Delta rows, cut into pieces of at most
+// CBM_UNIT_PIECE entries, make at most CBM_UNIT_PIECES pieces.  Bottom-up:
+// a node keeps its children's components while they fit and cuts the
+// heaviest ones off otherwise.  A component's level is the number of cuts
+// above it; components of one level are packed into units (one workgroup
+// each) in position order, so a unit only ever waits on units of lower
+// index: the one a component root's parent row lives in.
+struct Units {
+    std::vector<int32_t> ptr, row, lpar, pc, pcb, pce, seg_ptr, seg, of;
+    int32_t levels = 0;
+};
+
+bool build_units(int32_t n, int32_t levels, const std::vector<int32_t> &parent, const std::vector<int32_t> &kptr,
+                 const std::vector<int32_t> &kid, const std::vector<int32_t> &lvl_node,
+                 const std::vector<int32_t> &lvl_ptr, const std::vector<int32_t> &pos_of_node,
+                 const std::vector<int32_t> &fptr, Units &u) {
+    auto pieces_of = [&](int32_t v) {
+        const int32_t len = fptr[size_t(pos_of_node[size_t(v)]) + 1] - fptr[size_t(pos_of_node[size_t(v)])];
+        return std::max<int32_t>(1, (len + CBM_UNIT_PIECE - 1) / CBM_UNIT_PIECE);
+    };
+    // rows / pieces still attached to each node's component, deepest level first
+    std::vector<int32_t> rows(size_t(n), 1), pcs(size_t(n), 0);
+    std::vector<char> cut(size_t(n), 0);
+    std::vector<int32_t> ch;
+    for (int32_t p = n - 1; p >= 0; --p) {
+        const int32_t v = lvl_node[size_t(p)];
+        pcs[size_t(v)] = pieces_of(v);
+        if (pcs[size_t(v)] > CBM_UNIT_PIECES) return false;  // a row longer than a unit holds
+        int32_t r = 1, q = pcs[size_t(v)];
+        ch.assign(kid.begin() + kptr[size_t(v)], kid.begin() + kptr[size_t(v) + 1]);
+        for (int32_t c : ch) r += rows[size_t(c)], q += pcs[size_t(c)];
+        if (r > CBM_UNIT_ROWS || q > CBM_UNIT_PIECES) {
+            // cut the heaviest children first (pieces, then rows, then id)
+            std::sort(ch.begin(), ch.end(), [&](int32_t a, int32_t b) {
+                if (pcs[size_t(a)] != pcs[size_t(b)]) return pcs[size_t(a)] > pcs[size_t(b)];
+                if (rows[size_t(a)] != rows[size_t(b)]) return rows[size_t(a)] > rows[size_t(b)];
+                return a < b;
+            });
+            for (int32_t c : ch) {
+                if (r <= CBM_UNIT_ROWS && q <= CBM_UNIT_PIECES) break;
+                cut[size_t(c)] = 1;
+                r -= rows[size_t(c)];
+                q -= pcs[size_t(c)];
+            }
+        }
+        rows[size_t(v)] = r;
+        pcs[size_t(v)] = q;
+    }
+    // component of each node (its nearest cut or root ancestor) and component levels, top down
+    std::vector<int32_t> comp_of(static_cast<size_t>(n)), clevel(static_cast<size_t>(n), 0);
+    int32_t max_clevel = 0;
+    for (int32_t p = 0; p < n; ++p) {
+        const int32_t v = lvl_node[size_t(p)], par = parent[size_t(v)];
+        if (par < 0) {
+            comp_of[size_t(v)] = v;
+            clevel[size_t(v)] = 0;
+        } else if (cut[size_t(v)]) {
+            comp_of[size_t(v)] = v;
+            clevel[size_t(v)] = clevel[size_t(comp_of[size_t(par)])] + 1;
+        } else {
+            comp_of[size_t(v)] = comp_of[size_t(par)];
+        }
+        max_clevel = std::max(max_clevel, clevel[size_t(comp_of[size_t(v)])]);
+    }
+    u.levels = max_clevel + 1;
+    // members per component root, in position order (the level order)
+    std::vector<int32_t> mptr(size_t(n) + 1, 0), memb(static_cast<size_t>(n));
+    for (int32_t v = 0; v < n; ++v) ++mptr[size_t(comp_of[size_t(v)]) + 1];
+    std::partial_sum(mptr.begin(), mptr.end(), mptr.begin());
+    {
+        std::vector<int32_t> fill(mptr.begin(), mptr.end() - 1);
+        for (int32_t p = 0; p < n; ++p) {
+            const int32_t v = lvl_node[size_t(p)];
+            memb[size_t(fill[size_t(comp_of[size_t(v)])]++)] = p;
+        }
+    }
+    // component roots by (component level, root position)
+    std::vector<int32_t> roots;
+    for (int32_t p = 0; p < n; ++p) {
+        const int32_t v = lvl_node[size_t(p)];
+        if (comp_of[size_t(v)] == v) roots.push_back(v);
+    }
+    std::stable_sort(roots.begin(), roots.end(),
+                     [&](int32_t a, int32_t b) { return clevel[size_t(a)] < clevel[size_t(b)]; });
+    // pack components of one level into units
+    u.ptr.assign(1, 0);
+    u.of.assign(size_t(n), -1);
+    u.row.clear();
+    int32_t unit_rows = 0, unit_pcs = 0, unit_level = -1;
+    std::vector<int32_t> cur;
+    auto close_unit = [&]() {
+        if (cur.empty()) return;
+        std::sort(cur.begin(), cur.end());  // positions ascending = level order
+        const int32_t id = int32_t(u.ptr.size()) - 1;
+        for (int32_t p : cur) {
+            u.of[size_t(p)] = id;
+            u.row.push_back(p);
+        }
+        u.ptr.push_back(int32_t(u.row.size()));
+        cur.clear();
+        unit_rows = unit_pcs = 0;
+    };
+    for (int32_t r : roots) {
+        const int32_t cr = rows[size_t(r)], cq = pcs[size_t(r)];
+        if (clevel[size_t(r)] != unit_level || unit_rows + cr > CBM_UNIT_ROWS || unit_pcs + cq > CBM_UNIT_PIECES)
+            close_unit();
+        unit_level = clevel[size_t(r)];
+        for (int32_t k = mptr[size_t(r)]; k < mptr[size_t(r) + 1]; ++k) cur.push_back(memb[size_t(k)]);
+        unit_rows += cr;
+        unit_pcs += cq;
+    }
+    close_unit();
+    const int32_t n_units = int32_t(u.ptr.size()) - 1;
+    // local parents, pieces and level segments
+    std::vector<int32_t> local(size_t(n), -1);  // index of a position within its unit
+    u.lpar.resize(size_t(n));
+    u.pc.assign(size_t(n) + 1, 0);
+    u.pcb.clear();
+    u.pce.clear();
+    u.seg_ptr.assign(1, 0);
+    u.seg.clear();
+    std::vector<int32_t> level_of_pos(static_cast<size_t>(n));
+    for (int32_t l = 0; l < levels; ++l)
+        for (int32_t p = lvl_ptr[size_t(l)]; p < lvl_ptr[size_t(l) + 1]; ++p) level_of_pos[size_t(p)] = l;
+    for (int32_t k = 0; k < n_units; ++k) {
+        const int32_t b = u.ptr[size_t(k)], e = u.ptr[size_t(k) + 1];
+        for (int32_t i = b; i < e; ++i) local[size_t(u.row[size_t(i)])] = i - b;
+        int32_t last_level = -1;
+        for (int32_t i = b; i < e; ++i) {
+            const int32_t p = u.row[size_t(i)];
+            const int32_t v = lvl_node[size_t(p)], par = parent[size_t(v)];
+            if (par < 0) {
+                u.lpar[size_t(i)] = -1;
+            } else {
+                const int32_t pp = pos_of_node[size_t(par)];
+                u.lpar[size_t(i)] = u.of[size_t(pp)] == k ? local[size_t(pp)] : -2;
+            }
+            if (level_of_pos[size_t(p)] != last_level) {
+                u.seg.push_back(i);
+                last_level = level_of_pos[size_t(p)];
+            }
+            const int32_t kb = fptr[size_t(p)], len = fptr[size_t(p) + 1] - kb;
+            const int32_t np = std::max<int32_t>(1, (len + CBM_UNIT_PIECE - 1) / CBM_UNIT_PIECE);
+            u.pc[size_t(i)] = int32_t(u.pcb.size());
+            for (int32_t q = 0; q < np; ++q) {
+                u.pcb.push_back(kb + int32_t(int64_t(len) * q / np));
+                u.pce.push_back(kb + int32_t(int64_t(len) * (q + 1) / np));
+            }
+        }
+        u.seg_ptr.push_back(int32_t(u.seg.size()));
+    }
+    u.pc[size_t(n)] = int32_t(u.pcb.size());
+    return true;
+}
+
 struct Tail {
     int32_t level = 0;
     std::vector<int32_t> frag, seg, pos;
@@ -474,6 +631,22 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
             plan->arr[CBM_P_FRED_SLOT].swap(f.red_slot);
             plan->arr[CBM_P_FRED_CNT].swap(f.red_cnt);
             plan->stat[CBM_PS_FWD_SLOTS] = max_slots;
+        }
+        if (n <= CBM_UNIT_MAX_ROWS) {
+            Units un;
+            if (build_units(n, levels, parent, kptr, kid, lvl_node, lvl_ptr, pos_of_node, fptr, un)) {
+                plan->stat[CBM_PS_UNITS] = int64_t(un.ptr.size()) - 1;
+                plan->stat[CBM_PS_UNIT_LEVELS] = un.levels;
+                plan->arr[CBM_P_UN_PTR].swap(un.ptr);
+                plan->arr[CBM_P_UN_ROW].swap(un.row);
+                plan->arr[CBM_P_UN_LPAR].swap(un.lpar);
+                plan->arr[CBM_P_UN_PC].swap(un.pc);
+                plan->arr[CBM_P_UN_PCB].swap(un.pcb);
+                plan->arr[CBM_P_UN_PCE].swap(un.pce);
+                plan->arr[CBM_P_UN_SEG_PTR].swap(un.seg_ptr);
+                plan->arr[CBM_P_UN_SEG].swap(un.seg);
+                plan->arr[CBM_P_UN_OF].swap(un.of);
+            }
         }
         const bool ranges = n <= CBM_RANGE_MAX_ROWS;
         if (ranges) {

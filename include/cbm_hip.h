@@ -177,6 +177,18 @@ typedef struct cbm_desc {
     const int32_t *tr_frag;
     const int32_t *tr_seg;
     const int32_t *tr_pos;
+    /* units of the one-launch forward (plan arrays CBM_P_UN_*, stat UNITS;
+       n_units 0 and NULL pointers when the plan has none) */
+    int32_t n_units;
+    const int32_t *un_ptr;
+    const int32_t *un_row;
+    const int32_t *un_lpar;
+    const int32_t *un_pc;
+    const int32_t *un_pcb;
+    const int32_t *un_pce;
+    const int32_t *un_seg_ptr;
+    const int32_t *un_seg;
+    const int32_t *un_of;
 } cbm_desc;
 
 typedef struct cbm_handle cbm_handle;
@@ -372,6 +384,37 @@ int cbm_set_xcd_ranges(cbm_handle *h, int32_t mode);
  * A^T exactly; a wrong declaration gives a wrong adjoint.
  */
 int cbm_set_symmetric(cbm_handle *h, int32_t symmetric);
+
+/*
+ * The forward product in one launch (-1 automatic, the default; 0 off; 1 on
+ * where possible).  With the plan's units (CBM_P_UN_*), one kernel does the
+ * Delta pass and the propagation of every level: a workgroup takes a unit of
+ * at most CBM_UNIT_ROWS rows (connected parts of the tree) and a 64-column
+ * slab, gathers its rows' Delta entries into LDS, propagates level by level
+ * in LDS, and reads a parent outside the unit from the output once that
+ * parent's unit has set its completion flag; units are dispatched so that a
+ * unit only waits on units already running.  Possible when the plan has
+ * units, no level is compensated (cbm_set_kahan_depth), no XCD ranges, slabs
+ * or column slabs are in use, and the width is reserved (the flags are sized
+ * by cbm_reserve).  Used by cbm_matmul and, on a symmetric handle, by
+ * cbm_matmul_t.  Each row sums its entries in pieces of CBM_UNIT_PIECE, so
+ * the rounding differs from the multi-launch forms; it is deterministic.
+ * Automatic: on when possible and the structure is small enough that the
+ * passes are bound by launch and dependency latency (see cbm_api.hip).
+ */
+int cbm_set_fused(cbm_handle *h, int32_t mode);
+
+/* Whether a dtype product of n_cols columns would run in one launch now
+   (1) or not (0); CBM_ERR_ARG for invalid arguments. */
+int cbm_fused_active(const cbm_handle *h, int32_t dtype, int32_t n_cols);
+
+/*
+ * The number of dependency waits of the one-launch forward that gave up
+ * after their poll limit since the handle was created (0 when every wait
+ * ended normally; a nonzero count means some results read a parent row
+ * before it was final).  Synchronises the device.
+ */
+int64_t cbm_fused_errors(cbm_handle *h);
 
 /*
  * Tree tails (1 on, the default; 0 off).  The deepest tree levels of the

@@ -269,7 +269,23 @@ enum {
     CBM_P_TR_FRAG = 72,    /* int32 [Fr+1] reverse tail: segment offsets per fragment                  */
     CBM_P_TR_SEG = 73,     /* int32 [Sr+1] position offsets per segment                                */
     CBM_P_TR_POS = 74,     /* int32 [Mr]   internal-node positions                                     */
-    CBM_P_N_ARRAYS = 75
+    /* Units of the one-launch forward (built only for n_rows <= CBM_UNIT_MAX_ROWS when every Delta row
+       fits a unit; empty otherwise).  The tree is cut into connected components of at most
+       CBM_UNIT_ROWS rows and CBM_UNIT_PIECES pieces (a row's Delta entries cut into pieces of at most
+       CBM_UNIT_PIECE entries, at least one per row); components of equal cut depth are packed into
+       units, ordered by that depth, so a unit's outside parents all lie in units of lower index.
+       Unit rows are numbered unit after unit, each unit's rows in level order. */
+    CBM_P_UN_PTR = 75,     /* int32 [U+1]  row offsets per unit                                        */
+    CBM_P_UN_ROW = 76,     /* int32 [N]    level position of each unit row                             */
+    CBM_P_UN_LPAR = 77,    /* int32 [N]    parent as a row index within the unit, -1 none, -2 in an
+                                           earlier unit (read from the output after that unit's flag)  */
+    CBM_P_UN_PC = 78,      /* int32 [N+1]  first piece of each unit row (a row's pieces are consecutive) */
+    CBM_P_UN_PCB = 79,     /* int32 [P]    piece: first entry (into FWD_COL)                           */
+    CBM_P_UN_PCE = 80,     /* int32 [P]    piece: end entry                                            */
+    CBM_P_UN_SEG_PTR = 81, /* int32 [U+1]  segment offsets per unit                                    */
+    CBM_P_UN_SEG = 82,     /* int32 [S]    first unit row of each segment (rows of one tree level)     */
+    CBM_P_UN_OF = 83,      /* int32 [N]    unit of each level position                                 */
+    CBM_P_N_ARRAYS = 84
 };
 
 /* Scalars of a cbm_plan, addressed by cbm_plan_stat(). */
@@ -286,7 +302,9 @@ enum {
     CBM_PS_BWD_PIECES4 = 9, /* backward pieces of that set                               */
     CBM_PS_TF_LEVEL = 10,  /* first level of the forward tail (L: none)                   */
     CBM_PS_TR_LEVEL = 11,  /* first level of the reverse tail (L: none)                   */
-    CBM_PS_N_STATS = 12
+    CBM_PS_UNITS = 12,     /* units of the one-launch forward (0: none built)              */
+    CBM_PS_UNIT_LEVELS = 13, /* component levels (1: no unit waits on another)            */
+    CBM_PS_N_STATS = 14
 };
 
 #define CBM_LONG_ROW 256
@@ -295,6 +313,10 @@ enum {
 #define CBM_XCD_RANGES_WIDE 4
 #define CBM_PIECE 256
 #define CBM_RANGE_MAX_ROWS 524288
+#define CBM_UNIT_ROWS 64
+#define CBM_UNIT_PIECES 256
+#define CBM_UNIT_PIECE 16
+#define CBM_UNIT_MAX_ROWS 524288
 
 typedef struct cbm_plan cbm_plan;
 

@@ -1,6 +1,9 @@
 """Rows longer than one workgroup chunk: the chunked long-row kernels and
 their in-order reduction, forward and adjoint, on a dense graph with hubs
-(rows up to ~2.4k entries, several multi-chunk rows in both directions)."""
+(rows up to ~2.4k entries, several multi-chunk rows in both directions).
+These tests exercise the multi-launch forms, so the handles here run with
+the one-launch forward off and the adjoint through the Delta^T pass
+(test_gpu_fused.py covers those two)."""
 import numpy as np
 import pytest
 import torch
@@ -16,11 +19,19 @@ pytestmark = pytest.mark.gpu
 CLASSES = {"ax": cbm4mm, "adx": cbm4ad, "dadx": cbm4dad}
 
 
+def _multi_launch(m):
+    """The multi-launch forward and the Delta^T adjoint on this matrix."""
+    m.cusparse_extension.set_fused(False)
+    m.cusparse_extension.set_symmetric(False)
+    return m
+
+
 @pytest.fixture(scope="module")
 def hub(cuda):
     ei = hub_graph()
     a = O.csr_of(ei.numpy(), HUB_N)
-    mats = {mode: cls(ei, torch.ones(ei.shape[1]), alpha=0, device=cuda) for mode, cls in CLASSES.items()}
+    mats = {mode: _multi_launch(cls(ei, torch.ones(ei.shape[1]), alpha=0, device=cuda))
+            for mode, cls in CLASSES.items()}
     st = mats["ax"].cusparse_extension.plan_stats
     assert st["fwd_slots"] > 0 and st["bwd_slots"] > 0  # multi-chunk rows in both directions
     return a, mats
@@ -86,7 +97,7 @@ def test_every_row_long(cuda, f):
     keep = r != c
     ei = torch.from_numpy(np.stack([r[keep], c[keep]]).astype(np.int32))
     a = O.csr_of(ei.numpy(), n)
-    h = cbm4mm(ei, torch.ones(ei.shape[1]), alpha=n, device=cuda).cusparse_extension
+    h = _multi_launch(cbm4mm(ei, torch.ones(ei.shape[1]), alpha=n, device=cuda)).cusparse_extension
     assert h.plan_stats["depth"] == 1
     x = np.random.default_rng(17).integers(-3, 4, size=(n, f)).astype(np.float32)
     want_y, want_dx = O.forward(a, x, "ax"), O.adjoint(a, x, "ax")
@@ -198,7 +209,8 @@ def star(cuda):
     from hubgraph import star_graph
     ei, n = star_graph()
     a = O.csr_of(ei.numpy(), n)
-    mats = {mode: cls(ei, torch.ones(ei.shape[1]), alpha=0, device=cuda) for mode, cls in CLASSES.items()}
+    mats = {mode: _multi_launch(cls(ei, torch.ones(ei.shape[1]), alpha=0, device=cuda))
+            for mode, cls in CLASSES.items()}
     assert mats["ax"].cusparse_extension.plan_stats["rev_slots"] >= 3  # a multi-chunk child list
     return a, n, mats
 

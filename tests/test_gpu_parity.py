@@ -163,16 +163,23 @@ def test_bf16_storage(mats, graph1, cuda, mode, f):
 
 
 def test_split_halves_match_fused(mats, cuda):
-    """delta_spmm followed by propagate equals the fused forward."""
+    """delta_spmm followed by propagate equals the multi-launch forward bit
+    for bit, and the one-launch forward within FP32 rounding (it sums long
+    rows in pieces)."""
     for mode in ("ax", "dadx"):
         h = mats[mode][0].cusparse_extension
         x = torch.from_numpy(_rand((N1, 64), 5)).to(cuda)
         y1 = torch.empty(N1, 64, device=cuda)
         y2 = torch.empty(N1, 64, device=cuda)
+        y3 = torch.empty(N1, 64, device=cuda)
+        h.set_fused(False)
         h.forward(x, y1)
+        h.set_fused(None)
+        h.forward(x, y3)
         h.delta_spmm(x, y2)
         h.propagate(y2)
         assert torch.equal(y1, y2)
+        assert float((y3 - y1).abs().max()) <= 1e-5 * float(y1.abs().max())
 
 
 def test_strided_operands(mats, graph1, cuda):

@@ -483,3 +483,116 @@ def test_build_reports_whether_a_is_symmetric():
     with structure.Adjacency(ei[0], ei[1], torch.ones(ei.shape[1]), 500) as adj:
         src, dst, w = structure.host_candidates(adj, 0)
         assert structure.build_from_candidates(adj, src, dst, w).stats["symmetric"] == 1
+
+
+def _check_units(pl, n):
+    """The unit tables: a partition of the positions into units of at most
+    CBM_UNIT_ROWS rows and CBM_UNIT_PIECES pieces, each unit's rows in level
+    order with level segments, local parents pointing at the parent's row, an
+    outside parent only in a unit of lower index, and pieces of at most
+    CBM_UNIT_PIECE entries tiling each row's Delta entries in order."""
+    a = {k: pl.arrays[k].numpy() for k in ("un_ptr", "un_row", "un_lpar", "un_pc", "un_pcb", "un_pce", "un_seg_ptr",
+                                            "un_seg", "un_of", "lvl_ptr", "lvl_ppos", "fwd_ptr")}
+    up = a["un_ptr"]
+    U = up.size - 1
+    assert U == pl.stats["units"] > 0
+    assert up[0] == 0 and up[-1] == n and np.all(np.diff(up) > 0)
+    assert np.array_equal(np.sort(a["un_row"]), np.arange(n))
+    level = np.zeros(n, np.int64)
+    for l in range(pl.depth):
+        level[a["lvl_ptr"][l]:a["lvl_ptr"][l + 1]] = l
+    pc = a["un_pc"]
+    assert pc[0] == 0 and pc[-1] == a["un_pcb"].size and np.all(np.diff(pc) >= 1)
+    for u in range(U):
+        r0, r1 = up[u], up[u + 1]
+        rows = a["un_row"][r0:r1]
+        assert r1 - r0 <= structure.UNIT_ROWS and pc[r1] - pc[r0] <= structure.UNIT_PIECES
+        assert np.all(np.diff(rows) > 0) and np.all(a["un_of"][rows] == u)
+        segs = a["un_seg"][a["un_seg_ptr"][u]:a["un_seg_ptr"][u + 1]]
+        assert segs[0] == r0 and np.all(np.diff(segs) > 0)
+        bounds = list(segs) + [r1]
+        for k in range(len(segs)):
+            assert np.unique(level[a["un_row"][bounds[k]:bounds[k + 1]]]).size == 1
+        assert np.all(np.diff(level[a["un_row"][segs]]) > 0)
+        for i in range(r0, r1):
+            p = a["un_row"][i]
+            pp = a["lvl_ppos"][p]
+            lp = a["un_lpar"][i]
+            if pp < 0:
+                assert lp == -1
+            elif a["un_of"][pp] == u:
+                assert lp >= 0 and a["un_row"][r0 + lp] == pp
+            else:
+                assert lp == -2 and a["un_of"][pp] < u
+            kb, ke = a["fwd_ptr"][p], a["fwd_ptr"][p + 1]
+            b, e = a["un_pcb"][pc[i]:pc[i + 1]], a["un_pce"][pc[i]:pc[i + 1]]
+            assert b[0] == kb and e[-1] == ke and np.all(b[1:] == e[:-1])
+            assert np.all(e - b <= structure.UNIT_PIECE) and np.all(e >= b)
+    return a
+
+
+def _units_forward(pl, delta_rows, x):
+    """The one-launch forward's arithmetic in FP64 from the unit tables alone:
+    per unit, piece sums, then rows in order adding the parent's row (in the
+    unit or already written by an earlier unit)."""
+    a = {k: pl.arrays[k].numpy() for k in ("un_ptr", "un_row", "un_lpar", "un_pc", "un_pcb", "un_pce", "un_of",
+                                            "lvl_node", "lvl_parent", "fwd_col")}
+    col = a["fwd_col"].astype(np.int64)
+    sign = np.where(col < 0, -1.0, 1.0)
+    col = col & 0x7FFFFFFF
+    y = np.full_like(x, np.nan, dtype=np.float64)
+    for u in range(a["un_ptr"].size - 1):
+        r0, r1 = a["un_ptr"][u], a["un_ptr"][u + 1]
+        val = {}
+        for i in range(r0, r1):
+            s = np.zeros(x.shape[1])
+            for q in range(a["un_pc"][i], a["un_pc"][i + 1]):
+                e = slice(a["un_pcb"][q], a["un_pce"][q])
+                s += (sign[e][:, None] * x[col[e]]).sum(axis=0)
+            lp = a["un_lpar"][i]
+            if lp >= 0:
+                s += val[r0 + lp]
+            elif lp == -2:
+                par = a["lvl_parent"][a["un_row"][i]]
+                assert not np.isnan(y[par]).any()  # written by an earlier unit
+                s += y[par]
+            val[i] = s
+        for i in range(r0, r1):
+            y[a["lvl_node"][a["un_row"][i]]] = val[i]
+    return y
+
+
+@pytest.mark.parametrize("which", ["c1", "hub", "star", "chain"])
+def test_units_partition_and_compute_the_product(which):
+    if which == "c1":
+        n = 2708
+        ei = structure.copying_model(n, 7, 0.8, 0.58, 0, 1001)
+    elif which == "hub":
+        n, ei = HUB_N, hub_graph()
+    elif which == "star":
+        ei, n = star_graph()
+    else:  # a chain of depth 150: components stacked many levels deep
+        m = 150
+        pairs = sorted({(i, m + j) for i in range(m) for j in range(m - i)} |
+                       {(m + j, i) for i in range(m) for j in range(m - i)})
+        ei, n = torch.tensor(pairs, dtype=torch.int32).t().contiguous(), 2 * m
+    b, pl = _plan_for(ei.numpy(), n, 0)
+    a = _check_units(pl, n)
+    if which == "chain":
+        assert pl.stats["unit_levels"] >= 150 // structure.UNIT_ROWS
+    x = np.random.default_rng(3).standard_normal((n, 5))
+    y = _units_forward(pl, None, x)
+    ref = O.forward(O.csr_of(ei.numpy(), n), x, "ax")
+    assert np.allclose(y, ref, rtol=1e-12, atol=1e-9)
+
+
+def test_no_units_when_a_row_exceeds_a_unit():
+    """A Delta row of more than CBM_UNIT_PIECES pieces fits no unit: the plan
+    then has none (the product runs the multi-launch forms)."""
+    n = structure.UNIT_PIECES * structure.UNIT_PIECE + 10
+    hub = n - 1
+    r = np.concatenate([np.full(n - 1, hub), np.arange(n - 1)])
+    c = np.concatenate([np.arange(n - 1), np.full(n - 1, hub)])
+    ei = np.stack([r, c])
+    b, pl = _plan_for(ei, n, 0)
+    assert pl.stats["units"] == 0 and pl.arrays["un_ptr"].numel() == 0

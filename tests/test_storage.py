@@ -127,16 +127,19 @@ def test_save_checks_its_input(built, tmp_path):
         _save(tmp_path / "y.npz", b, pl, delta_sign=torch.zeros_like(b.csr_val))
 
 
-@pytest.mark.parametrize("version", [1, 2])
+@pytest.mark.parametrize("version", [1, 2, 3])
 def test_older_files_load_without_the_newer_layout_parts(built, tmp_path, version):
-    """A version 1 file (no 4-range piece set, no tree tails) or version 2
-    file (no tree tails) loads with those parts empty -- no pieces, tails at
-    the depth, i.e. none -- and the rest of the layout unchanged."""
+    """A version 1 file (no 4-range piece set, no tree tails, no units), a
+    version 2 file (no tails, no units) or a version 3 file (no units) loads
+    with those parts empty -- no pieces, tails at the depth, i.e. none, no
+    units -- and the rest of the layout unchanged."""
     b, pl = built
-    path = tmp_path / "v3.npz"
+    path = tmp_path / "v4.npz"
     _save(path, b, pl)
-    gone = set(storage._NEW_IN_3) | (set(storage._NEW_IN_2) if version == 1 else set())
-    gone_stats = set(storage._NEW_STATS_IN_3) | (set(storage._NEW_STATS_IN_2) if version == 1 else set())
+    gone = (set(storage._NEW_IN_4) | (set(storage._NEW_IN_3) if version <= 2 else set())
+            | (set(storage._NEW_IN_2) if version == 1 else set()))
+    gone_stats = (set(storage._NEW_STATS_IN_4) | (set(storage._NEW_STATS_IN_3) if version <= 2 else set())
+                  | (set(storage._NEW_STATS_IN_2) if version == 1 else set()))
 
     def older(name, a):
         if name.startswith("plan/") and name[5:] in gone:

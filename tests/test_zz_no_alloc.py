@@ -1,6 +1,7 @@
 """The allocation-free forward entry point, cbm_matmul_noalloc, called
 through the C ABI: it equals cbm_matmul bit for bit (same kernels and order;
-the XCD ranges are off at these sizes) on two calls in a row, the second
+the XCD ranges are off at these sizes, and the one-launch forward, which
+needs the handle's flags, is off) on two calls in a row, the second
 with a smaller input, on graphs whose product uses the static scratch: the
 hub graph (long rows: chunk partial sums) and a chain of depth 80
 (compensated levels: compensation rows).  It also matches the FP64 oracle,
@@ -61,6 +62,7 @@ def test_noalloc_equals_cbm_matmul_on_long_rows(cuda):
     a = O.csr_of(ei.numpy(), HUB_N)
     m = cbm4dad(ei, torch.ones(ei.shape[1]), alpha=0, device=cuda)
     h = m.cusparse_extension
+    h.set_fused(False)  # cbm_matmul's multi-launch form, which the static scratch serves
     assert h.plan_stats["fwd_gslots"] > 0 and h.plan_stats["fwd_slots"] > 0  # long rows use the scratch
     lib = _lib.hip()
     for f, seed in ((64, 1), (16, 2)):  # the second call smaller
