@@ -59,21 +59,41 @@ def measured_traffic(workload, phase):
         return None
 
 
+def committed_gather_ceiling(workload, rec):
+    """The rate of the same Delta-pass kernel over uniform-random indices of
+    the workload's shape (bench.py --gather-ceiling, committed in
+    profiles/gather_ceiling.json), in gathered GB/s, or None."""
+    path = os.path.join(ROOT, "profiles", "gather_ceiling.json")
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+        c = data[workload]
+        return {"gbs": c["above_mall"]["gather_gbs"], "source": f"committed profiles/gather_ceiling.json ({c['sample']})"}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def phase_bytes(n, nnz_delta, n_edges, n_internal, depth, f, s_in, scaled):
+def phase_bytes(n, nnz_delta, n_edges, n_internal, depth, f, s_in, scaled, symmetric):
     """Algorithmic (compulsory) HBM bytes of each phase: every structure
     array read once, every input row read once, every output row written
     once; gathered rows read again, and intermediates, are not counted.
-    `scaled`: the product carries the D^-1/2 vector (4N bytes)."""
+    `scaled`: the product carries the D^-1/2 vector (4N bytes).  With a
+    symmetric A the adjoint runs through the forward structure, so its bytes
+    are the forward's (the Delta^T form's are given for its two halves).
+    `gather_*`: the bytes of the row gathers themselves (nnz(Delta) rows of
+    s_in * F bytes), which no cache on the chip holds at the largest sizes."""
     dvec = 4 * n if scaled else 0
     fwd = 4 * (n + 1) + 4 * nnz_delta + 8 * n_edges + 4 * (depth + 1) + s_in * n * f + 4 * n * f + dvec
+    delta = 4 * (n + 1) + 4 * nnz_delta + s_in * n * f + 4 * n * f + dvec
     rev = 4 * (n_internal + 1) + 4 * n_internal + 4 * n_edges + s_in * f * (n_internal + n_edges) + 4 * f * n_internal
     dlt = 4 * (n + 1) + 4 * n + 4 * nnz_delta + s_in * n * f + 4 * n_internal * f + 4 * n * f + dvec
     step_bwd = 8 * (n + 1) + 4 * nnz_delta + 4 * n_edges + 4 * (depth + 1) + s_in * n * f + 4 * n * f + dvec
-    return {"forward": fwd, "adjoint_reverse": rev, "adjoint_delta_t": dlt, "step": fwd + step_bwd}
+    return {"forward": fwd, "forward_delta_spmm": delta, "adjoint_reverse": rev, "adjoint_delta_t": dlt,
+            "step": fwd + (fwd if symmetric else step_bwd), "gather_delta": nnz_delta * s_in * f}
 
 
 def csr_bytes(n, nnz, f):
@@ -233,18 +253,41 @@ def measure(name, w, ei, m, args, dev, stream):
     ms16_graph = graph_ms(step(x16, g16))
     ms32_graph = graph_ms(step(x32, g32))
     log(f"{name}: CBM step bf16 {ms16:.3f} ms, fp32 {ms32:.3f} ms; graph-replayed {ms16_graph:.3f} / {ms32_graph:.3f}")
+    fused = {"bf16": h.fused_active(torch.bfloat16, f), "fp32": h.fused_active(torch.float32, f)}
+    # the same step with the one-launch forward off (multi-launch forms), and
+    # with the adjoint through the Delta^T pass instead of the forward structure
+    forms = {}
+    if any(fused.values()):
+        h.set_fused(False)
+        forms["multi_launch"] = {"bf16": round(time_loop(step(x16, g16), K, W, stream), 4),
+                                 "fp32": round(time_loop(step(x32, g32), K, W, stream), 4)}
+        h.set_fused(None)
+    if h.symmetric:
+        h.set_symmetric(False)
+        forms["delta_t_adjoint"] = {"bf16": round(time_loop(step(x16, g16), K, W, stream), 4),
+                                    "fp32": round(time_loop(step(x32, g32), K, W, stream), 4)}
+        h.set_symmetric(True)
+    log(f"{name}: one launch {fused}; other forms {forms}")
     ph = {
         "forward": time_loop(lambda: h.forward(x16, y), K, W, stream),
         "forward_delta_spmm": time_loop(lambda: h.delta_spmm(x16, y), K, W, stream),
         "forward_propagate": time_loop(lambda: h.propagate(y), K, W, stream),
         "adjoint": time_loop(lambda: h.matmul_t(g16, dx, workspace=z), K, W, stream),
+        # the two halves of the Delta^T form of the adjoint
         "adjoint_reverse": time_loop(lambda: h.adjoint_reverse(g16, z), K, W, stream),
         "adjoint_delta_t": time_loop(lambda: h.adjoint_delta_t(g16, z, dx), K, W, stream),
     }
-    # the other form of the forward (same bits): per-level fused Delta + parent passes
+    launches = {}
+    h.forward(x16, y)
+    launches["forward"] = h.last_launches()
+    h.matmul_t(g16, dx, workspace=z)
+    launches["adjoint"] = h.last_launches()
+    # the other multi-launch form of the forward (same bits): per-level Delta + parent passes
+    h.set_fused(False)
     h.set_forward_split(False)
     ph["forward_fused_per_level"] = time_loop(lambda: h.forward(x16, y), K, W, stream)
     h.set_forward_split(None)
+    h.set_fused(None)
     # feature slabs spread over the 8 XCDs (same bits)
     xcd = {}
     for slabs in (2, 4, 8):
@@ -318,12 +361,18 @@ def measure(name, w, ei, m, args, dev, stream):
 
     n_internal = int(h._dev["rev_node"].numel())
     scaled = w["product"] != "mm"
-    b16 = phase_bytes(n, h.nnz, h.n_edges, n_internal, h.depth, f, 2, scaled)
-    b32 = phase_bytes(n, h.nnz, h.n_edges, n_internal, h.depth, f, 4, scaled)
-    dom = "adjoint_delta_t"  # the step's largest single kernel launch is this pass's short-row kernel
+    b16 = phase_bytes(n, h.nnz, h.n_edges, n_internal, h.depth, f, 2, scaled, h.symmetric)
+    b32 = phase_bytes(n, h.nnz, h.n_edges, n_internal, h.depth, f, 4, scaled, h.symmetric)
+    # The step's largest single launch: the one-launch forward where it runs,
+    # else the forward's Delta pass (chunks and short rows in one grid); the
+    # adjoint of a symmetric A is the same kernel over G.
+    dom = "forward" if fused["bf16"] else "forward_delta_spmm"
     dom_gbs = b16[dom] / (ph[dom] * 1e-3) / 1e9
+    gen_desc = (f"copying model N={n} blocks={w['blocks']} p={w['p']} lambda={w['lam']} seed_degree={w['seed_deg']} "
+                f"seed={w['seed']}" if "blocks" in w else f"graph file N={n}")
     rec = {
-        "workload": f"{name}: {CBM_CLASSES[w['product']][1]} fwd+bwd, N={n}, nnz(A)={nnz}, alpha={w['alpha']}, F={f}",
+        "workload": f"{name}: {CBM_CLASSES[w['product']][1]} fwd+bwd, {gen_desc}, nnz(A)={nnz}, alpha={w['alpha']}, "
+                    f"F={f}",
         "nnz_delta": h.nnz, "tree_edges": h.n_edges, "internal_nodes": n_internal, "depth": h.depth,
         "compression": round(nnz / max(1, h.nnz), 3),
         "ms_step_bf16": round(ms16, 4), "ms_step_fp32": round(ms32, 4),
@@ -339,6 +388,9 @@ def measure(name, w, ei, m, args, dev, stream):
         "gbs_rocsparse_csr": round(2 * csr_bytes(n, nnz, f) / (ms_csr * 1e-3) / 1e9, 1),
         "bytes_step_bf16": b16["step"], "bytes_phase_bf16": b16,
         "dominant_phase": dom, "dominant_gbs": round(dom_gbs, 1),
+        "gather_gbs_dominant": round(b16["gather_delta"] / (ph[dom] * 1e-3) / 1e9, 1),
+        "symmetric_adjoint": h.symmetric, "one_launch_forward": fused, "launches": launches,
+        "ms_step_other_forms": forms,
         "max_rel_error_bf16_vs_fp32_cbm": vs_fp32,
         "ms_xcd_slabs_bf16": xcd,
         "ms_xcd_ranges": xr,
@@ -386,6 +438,50 @@ def gcn_step(args, dev, stream):
     print(json.dumps(out), flush=True)
 
 
+def gather_ceiling(name, w, m, args, dev, stream):
+    """The ceiling of the gather passes: the same Delta-pass kernel over
+    uniform-random column indices of the workload's shape (N rows, the mean
+    entries per row of its Delta, X in BF16 at its F), columns drawn over all
+    N rows (X beyond the 256 MB Infinity Cache at c5) and over a
+    400,000-row span (about 100 MB of X at F = 128, inside it).  Reported in
+    gathered GB/s (entries x 2 F bytes per call) beside the same kernel over
+    the workload's own Delta."""
+    import torch
+    from cbm._hip_sigs import MODE_AX
+    from cbm.cbm_cusparse import cusparseCBM
+    K, W = args.steps, args.warmup
+    n, f = w["n"], w["cols"]
+    h0 = m.cusparse_extension
+    k = max(1, round(h0.nnz / n))
+    gen = torch.Generator(device=dev).manual_seed(2005)
+    x16 = torch.randn(n, f, device=dev, generator=gen).to(torch.bfloat16)
+    y = torch.empty(n, f, device=dev)
+    h0.reserve(f)
+    ms_real = time_loop(lambda: h0.delta_spmm(x16, y), K, W, stream)
+    out = {"sample": f"N={n}, {k} uniform-random entries per row, F={f} BF16",
+           "workload_delta_pass": {"ms": round(ms_real, 4), "entries": h0.nnz,
+                                   "gather_gbs": round(h0.nnz * 2 * f / (ms_real * 1e-3) / 1e9, 1)}}
+    e0 = torch.zeros(0, dtype=torch.int32, device=dev)
+    for tag, span in (("above_mall", n), ("inside_mall", min(n, 400_000))):
+        cpu_gen = torch.Generator().manual_seed(17)
+        col = torch.randint(0, span, (n, k), generator=cpu_gen, dtype=torch.int32).sort(dim=1).values.reshape(-1)
+        crow = torch.arange(0, n * k + 1, k, dtype=torch.int32)
+        a = torch.sparse_csr_tensor(crow, col, torch.ones(n * k), (n, n)).to(dev)
+        del col, crow
+        h = cusparseCBM(a, torch.zeros(1, dtype=torch.int32, device=dev), e0, e0, torch.empty(0, device=dev),
+                        mode=MODE_AX)
+        h.reserve(f)
+        ms = time_loop(lambda: h.delta_spmm(x16, y), K, W, stream)
+        out[tag] = {"span_rows": span, "x_bytes_gathered_from": span * f * 2, "ms": round(ms, 4),
+                    "gather_gbs": round(n * k * 2 * f / (ms * 1e-3) / 1e9, 1)}
+        log(f"gather ceiling {tag}: {out[tag]}")
+        del h, a
+        torch.cuda.empty_cache()
+    out["workload_frac_of_ceiling"] = round(out["workload_delta_pass"]["gather_gbs"] / out["above_mall"]["gather_gbs"],
+                                            3)
+    print(json.dumps({name: out}), flush=True)
+
+
 def profile_steps(w, m, args, dev, stream):
     """The CBM step alone (BF16 then FP32 storage), for kernel traces and
     counter runs: args.steps timed steps after args.warmup, per dtype."""
@@ -416,9 +512,11 @@ def _host_csr(t, dtype):
                                    t.values().to(dtype), t.shape)
 
 
-def cpu_baseline(m_csr, n, f, inputs, budget_s=20.0):
+def cpu_csr_mkl(m_csr, n, f, inputs, budget_s=20.0):
     """torch.sparse CSR FP32 M @ X and M^T @ G (MKL) on the host cores, on the
-    full graph, for as many steps as fit in the budget (at most 10)."""
+    full graph, for as many steps as fit in the budget (at most 10): the
+    reference harness's CPU CSR product (benchmark/compare_matmul_cuda.py:53),
+    reported beside the oracle's time."""
     import torch
     cores = len(os.sched_getaffinity(0))
     torch.set_num_threads(cores)
@@ -436,8 +534,66 @@ def cpu_baseline(m_csr, n, f, inputs, budget_s=20.0):
             break
     ms = 1e3 * sum(times) / len(times)
     del y, dx
-    return {"value": round(ms, 3), "unit": "ms per M@X fwd+bwd (FP32)", "cores": cores, "kind": "port",
+    return {"value": round(ms, 3), "unit": "ms per M@X fwd+bwd (FP32)", "cores": cores,
+            "kind": "torch.sparse CSR (MKL)",
             "sample": f"full graph N={n} F={f}, {len(times)} steps of torch.sparse CSR M@X + M^T@G (MKL, FP32)"}
+
+
+def cpu_baseline_oracle(m, product, n, f, x32, g32, budget_s=20.0, cols=4):
+    """The oracle's CBM product timed on the host: Delta' X in FP64 (scipy,
+    one thread) followed by the reference's branch-serial propagation
+    (cbm/cbm.cu:25-91) restated in C (oracle/cbm_oracle.c, one thread), for
+    the forward and for the adjoint (for a symmetric A the same form over G;
+    D A for the A D product).  A bounded sample: `cols` columns of X and G on
+    the full graph, repeated for about budget_s seconds; the time is scaled
+    to the full F (the work is linear in the columns)."""
+    import ctypes
+    import numpy as np
+    import scipy.sparse as sp
+    import torch
+    from oracle import oracle as O
+    crow = m.deltas.crow_indices().cpu().numpy().astype(np.int64)
+    col = m.deltas.col_indices().cpu().numpy()
+    sign = np.sign(m.deltas.values().cpu().numpy()).astype(np.float64)
+    delta = sp.csr_matrix((sign, col, crow), shape=(n, n))
+    mode = {"mm": "ax", "ad": "adx", "dad": "dadx"}[product]
+    dvec = None
+    if mode != "ax":
+        dvec = m.D.cpu().numpy().astype(np.float64)  # D^-1/2 as the product uses it
+        delta_f = delta @ sp.diags(dvec)
+        delta_f = (sp.diags(dvec) @ delta_f).tocsr() if mode == "dadx" else delta_f.tocsr()
+        delta_t = (sp.diags(dvec) @ delta).tocsr()  # D Delta: the adjoint of A D is D A
+    else:
+        delta_f = delta_t = delta
+    br = np.ascontiguousarray(m.mca_offset.cpu().numpy(), np.int32)
+    src = np.ascontiguousarray(m.mca_src_idx.cpu().numpy(), np.int32)
+    dst = np.ascontiguousarray(m.mca_dst_idx.cpu().numpy(), np.int32)
+    xs = np.ascontiguousarray(x32[:, :cols].cpu().numpy(), np.float64)
+    gs = np.ascontiguousarray(g32[:, :cols].cpu().numpy(), np.float64)
+    mult_f = dvec.ctypes.data_as(ctypes.c_void_p) if mode == "dadx" else None
+    mult_t = dvec.ctypes.data_as(ctypes.c_void_p) if mode in ("adx", "dadx") else None
+    lib = O.lib()
+
+    def product(dm, v, mult):
+        y = np.ascontiguousarray(dm @ v)
+        lib.oracle_update(cols, y, len(br) - 1, br, src, dst, mult)
+        return y
+
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while True:
+        t0 = time.perf_counter()
+        product(delta_f, xs, mult_f)
+        product(delta_t, gs, mult_t)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() > t_end or len(times) >= 20:
+            break
+    ms = 1e3 * sum(times) / len(times) * (f / cols)
+    return {"value": round(ms, 3), "unit": "ms per M@X fwd+bwd (FP64 oracle, scaled to F)", "cores": 1,
+            "kind": "port",
+            "sample": f"full graph N={n}, {cols} of F={f} columns, {len(times)} steps (forward + adjoint) of the "
+                      f"oracle's CBM product: scipy Delta' X (FP64) + the branch-serial propagation in C, "
+                      f"{1e3 * sum(times) / len(times):.1f} ms per sampled step, scaled by F/{cols}"}
 
 
 def fp64_errors(m_csr, gpu_out, inputs):
@@ -482,6 +638,9 @@ def main():
                     help="run only the CBM steps (BF16, then FP32) for a profiler; prints their times")
     ap.add_argument("--build-only", action="store_true",
                     help="build the workload's structure and time the build stages only (GPU vs host candidate scan)")
+    ap.add_argument("--gather-ceiling", action="store_true",
+                    help="time the Delta-pass kernel over uniform-random indices of the workload's shape (X beyond "
+                         "and inside the Infinity Cache) beside its own Delta pass; prints one JSON line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (the calibration kernel of counter runs)")
     args = ap.parse_args()
@@ -532,14 +691,21 @@ def main():
     if args.profile:
         profile_steps(w, m, args, dev, stream)
         return
+    if args.gather_ceiling:
+        gather_ceiling(name, w, m, args, dev, stream)
+        return
     rec, gpu_out, inputs, m_csr = measure(name, w, ei, m, args, dev, stream)
     rec["build_s"] = w["build_s"]
     rec["build_stages"] = build_stages(ei, w["n"], w["alpha"], dev, stream)
     cpu = None
     if not args.no_cpu:
         t0 = time.perf_counter()
-        cpu = cpu_baseline(m_csr, w["n"], w["cols"], inputs)
-        log(f"CPU baseline {cpu['value']} ms on {cpu['cores']} cores ({time.perf_counter() - t0:.1f}s)")
+        cpu = cpu_baseline_oracle(m, w["product"], w["n"], w["cols"], inputs[0], inputs[1])
+        log(f"CPU baseline (oracle) {cpu['value']} ms on {cpu['cores']} core ({time.perf_counter() - t0:.1f}s)")
+        t0 = time.perf_counter()
+        rec["cpu_csr_mkl"] = cpu_csr_mkl(m_csr, w["n"], w["cols"], inputs)
+        log(f"CPU torch.sparse CSR {rec['cpu_csr_mkl']['value']} ms on {rec['cpu_csr_mkl']['cores']} cores "
+            f"({time.perf_counter() - t0:.1f}s)")
         rec["max_rel_error_vs_fp64"] = fp64_errors(m_csr, gpu_out, inputs)
     del gpu_out, inputs, m, m_csr
 
@@ -556,6 +722,7 @@ def main():
     ms16 = rec["ms_step_bf16"]
     dom = rec["dominant_phase"]
     traffic = None if args.graph else measured_traffic(args.workload, dom)
+    ceiling = None if args.graph else committed_gather_ceiling(args.workload, rec)
     out = {
         "metric": "A@X fwd+bwd ms & HBM GB/s (% roofline), CBM-HIP vs CSR on-GPU, FP32 & BF16",
         "value": ms16,
@@ -574,13 +741,26 @@ def main():
                    "compression": rec["compression"]},
         "roofline": {"bound": "hbm", "achieved": rec["dominant_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(rec["dominant_gbs"] / HBM_PEAK_GBS, 4),
+                     # L2-to-fabric bytes per call (FETCH_SIZE + WRITE_SIZE): they include
+                     # Infinity Cache hits, and no counter on this system separates the bytes
+                     # that came from HBM; committed from a counter run, not measured here
                      "traffic": traffic.get("bytes") if traffic else None,
+                     "traffic_kind": "L2-to-fabric bytes (Infinity Cache hits included); HBM-only bytes unmeasurable",
+                     "traffic_hbm": None,
                      "traffic_gbs": round(traffic["bytes"] / (rec["ms_phase_bf16"][dom] * 1e-3) / 1e9, 1)
                      if traffic else None,
-                     "traffic_source": traffic.get("source") if traffic else None,
-                     "kernel": f"{dom} phase (BF16): k_row_mixed (long-row chunks + short rows) + k_row_reduce, dual pass",
+                     "traffic_source": (f"committed {traffic.get('source')}" if traffic else None),
+                     "kernel": (f"{dom} (BF16): " + ("k_fused_forward (one launch: Delta pieces + LDS propagation)"
+                                                     if dom == "forward" else
+                                                     "k_row_mixed (long-row chunks + short rows) + k_row_reduce")),
                      "algorithmic_bytes": rec["bytes_phase_bf16"][dom],
-                     "ms": rec["ms_phase_bf16"][dom]},
+                     "ms": rec["ms_phase_bf16"][dom],
+                     "gather_bytes": rec["bytes_phase_bf16"]["gather_delta"],
+                     "gather_gbs": rec["gather_gbs_dominant"],
+                     "gather_ceiling_gbs": ceiling["gbs"] if ceiling else None,
+                     "frac_of_gather_ceiling": round(rec["gather_gbs_dominant"] / ceiling["gbs"], 3)
+                     if ceiling else None,
+                     "gather_ceiling_source": ceiling["source"] if ceiling else None},
         "detail": rec,
         "cpu_baseline": cpu,
         "secondary": secondary,

@@ -118,6 +118,7 @@ class CbmDesc(ctypes.Structure):
         ("un_seg_ptr", c_vp),
         ("un_seg", c_vp),
         ("un_of", c_vp),
+        ("un_waited", c_vp),
     ]
 
 # Device arrays of the plan that the descriptor points at.
@@ -128,7 +129,7 @@ DEVICE_ARRAYS = ("lvl_node", "lvl_parent", "lvl_ppos", "fwd_ptr", "fwd_col", "re
                  "bpc_slot", "brs_slot", "brs_cnt", "bpc_row", "fpc4_b", "fpc4_e", "fpc4_slot", "frs4_slot", "frs4_cnt",
                  "bpc4_b", "bpc4_e", "bpc4_mid", "bpc4_slot", "brs4_slot", "brs4_cnt", "bpc4_row", "tf_frag", "tf_seg",
                  "tf_pos", "tr_frag", "tr_seg", "tr_pos", "un_ptr", "un_row", "un_lpar", "un_pc", "un_pcb", "un_pce",
-                 "un_seg_ptr", "un_seg", "un_of")
+                 "un_seg_ptr", "un_seg", "un_of", "un_waited")
 HOST_ARRAYS = ("lvl_ptr", "lvl_nlong", "fck_ptr", "fred_ptr", "rev_ptr", "rev_nlong", "rck_ptr", "rred_ptr", "fpc_range",
                "bpc_range", "fpc4_range", "bpc4_range")
 

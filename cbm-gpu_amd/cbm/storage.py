@@ -41,7 +41,8 @@ _NEW_IN_2 = ("fpc4_b", "fpc4_e", "fpc4_slot", "fpc4_range", "frs4_slot", "frs4_c
 _NEW_STATS_IN_2 = ("fwd_pieces4", "bwd_pieces4")
 _NEW_IN_3 = ("tf_frag", "tf_seg", "tf_pos", "tr_frag", "tr_seg", "tr_pos")
 _NEW_STATS_IN_3 = ("tf_level", "tr_level")  # absent: no tail (the depth)
-_NEW_IN_4 = ("un_ptr", "un_row", "un_lpar", "un_pc", "un_pcb", "un_pce", "un_seg_ptr", "un_seg", "un_of")
+_NEW_IN_4 = ("un_ptr", "un_row", "un_lpar", "un_pc", "un_pcb", "un_pce", "un_seg_ptr", "un_seg", "un_of",
+             "un_waited")
 _NEW_STATS_IN_4 = ("units", "unit_levels")  # absent: no units (0)
 _CORE = ("delta_crow", "delta_col", "delta_sign", "mca_offset", "mca_src_idx", "mca_dst_idx")
 _DTYPES = {"delta_crow": np.int32, "delta_col": np.int32, "delta_sign": np.int8, "mca_offset": np.int32,

@@ -20,7 +20,7 @@ _PLAN_NAMES = ("lvl_ptr", "lvl_node", "lvl_parent", "fwd_ptr", "fwd_col", "bwd_p
                "bpc_row", "fpc4_b", "fpc4_e", "fpc4_slot", "fpc4_range", "frs4_slot", "frs4_cnt", "bpc4_b", "bpc4_e",
                "bpc4_mid", "bpc4_slot", "bpc4_range", "brs4_slot", "brs4_cnt", "bpc4_row", "tf_frag", "tf_seg", "tf_pos",
                "tr_frag", "tr_seg", "tr_pos", "un_ptr", "un_row", "un_lpar", "un_pc", "un_pcb", "un_pce", "un_seg_ptr",
-               "un_seg", "un_of")
+               "un_seg", "un_of", "un_waited")
 _PLAN_STATS = ("depth", "fwd_slots", "bwd_slots", "bwd_nlong", "fwd_gslots", "rev_slots", "fwd_pieces", "bwd_pieces",
                "fwd_pieces4", "bwd_pieces4", "tf_level", "tr_level", "units", "unit_levels")
 # CBM_XCD_RANGES, CBM_XCD_RANGES_WIDE, CBM_PIECE, CBM_RANGE_MAX_ROWS

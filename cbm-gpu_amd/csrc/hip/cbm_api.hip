@@ -641,7 +641,8 @@ extern "C" int cbm_create(const cbm_desc *desc, hipStream_t stream, cbm_handle *
         }
     }
     if (d.n_units < 0 || (d.n_units > 0 && (!d.un_ptr || !d.un_row || !d.un_lpar || !d.un_pc || !d.un_pcb ||
-                                             !d.un_pce || !d.un_seg_ptr || !d.un_seg || !d.un_of))) {
+                                             !d.un_pce || !d.un_seg_ptr || !d.un_seg || !d.un_of ||
+                                             !d.un_waited))) {
         set_error("cbm_create: incomplete unit tables (%d units)", d.n_units);
         return CBM_ERR_ARG;
     }
@@ -755,12 +756,18 @@ extern "C" int cbm_reserve(cbm_handle *h, int32_t n_cols) {
 namespace {
 
 // Whether a forward-structure product of n_cols columns in this storage type
-// runs as the one-launch kernel (cbm_set_fused).  Automatic: whenever it can
-// -- it needs the plan's units, which are built only for matrices of up to
-// CBM_UNIT_MAX_ROWS rows whose Delta rows fit a unit -- and Delta holds at
-// most kFusedAutoEntries entries, where the passes are bound by the latency
-// of their launches and dependent levels rather than by bandwidth.
-constexpr int64_t kFusedAutoEntries = int64_t(1) << 22;
+// runs as the one-launch kernel (cbm_set_fused).  It needs the plan's units
+// (built only for matrices of up to CBM_UNIT_MAX_ROWS rows whose Delta rows
+// fit a unit).  Automatic: only for small matrices, up to kFusedAutoRows rows
+// and kFusedAutoEntries Delta entries, where the forward is bound by the
+// latency of its launches and dependent levels.  Measured BF16 step (forward
+// + adjoint, both one launch vs multi-launch): N = 2,708 (7k entries, depth
+// 9) 0.035 vs 0.039 ms; N = 19,717 (55k entries, depth 12) 0.123 vs 0.092 ms;
+// N = 169,343 (2.3M entries, depth 5) 0.751 vs 0.259 ms.  Larger units run
+// with fewer waves per CU (64 KB of LDS each) and narrower gathers than the
+// row passes, which costs more than the launches saved.
+constexpr int32_t kFusedAutoRows = 16384;
+constexpr int64_t kFusedAutoEntries = int64_t(1) << 16;
 
 bool fused_possible(const cbm_handle *h, int32_t dtype, int32_t n_cols, bool ranges) {
     if (h->fused == 0 || h->d.n_units <= 0 || !h->fused_state) return false;
@@ -768,7 +775,7 @@ bool fused_possible(const cbm_handle *h, int32_t dtype, int32_t n_cols, bool ran
     const bool compensated = h->kahan_level >= 0 && std::max<int32_t>(h->kahan_level, 1) < h->d.n_levels;
     if (compensated || h->slab_bytes > 0 || h->xcd_slabs > 1) return false;
     if (ranges && pick_ranges(h, n_cols, elem_of(dtype))) return false;
-    return h->fused == 1 || h->d.nnz <= kFusedAutoEntries;
+    return h->fused == 1 || (h->d.n_rows <= kFusedAutoRows && h->d.nnz <= kFusedAutoEntries);
 }
 
 // The forward product in buffers b; ranges: whether the XCD column ranges
@@ -789,6 +796,7 @@ int forward_product(cbm_handle *h, const Bufs &b, bool ranges, int32_t form, int
         a.un_seg_ptr = h->d.un_seg_ptr;
         a.un_seg = h->d.un_seg;
         a.un_of = h->d.un_of;
+        a.un_waited = h->d.un_waited;
         a.node_of = h->d.lvl_node;
         a.parent_of = h->d.lvl_parent;
         a.parent_pos = h->d.lvl_ppos;

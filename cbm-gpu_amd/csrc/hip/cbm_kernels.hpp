@@ -189,7 +189,7 @@ hipError_t launch_row_tail(const RowPass &rp, const TailArgs &t, hipStream_t str
 struct FusedArgs {
     const int32_t *un_ptr = nullptr, *un_row = nullptr, *un_lpar = nullptr, *un_pc = nullptr;
     const int32_t *un_pcb = nullptr, *un_pce = nullptr, *un_seg_ptr = nullptr, *un_seg = nullptr;
-    const int32_t *un_of = nullptr;
+    const int32_t *un_of = nullptr, *un_waited = nullptr;
     const int32_t *node_of = nullptr, *parent_of = nullptr, *parent_pos = nullptr;
     const int32_t *idx = nullptr;  // FWD_COL (packed signs)
     const float *colscale = nullptr, *rowscale = nullptr, *escale = nullptr;

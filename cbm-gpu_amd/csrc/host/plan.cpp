@@ -199,7 +199,7 @@ bool range_pieces(int32_t n, const std::vector<int32_t> &ptr, const std::vector<
 // each) in position order, so a unit only ever waits on units of lower
 // index: the one a component root's parent row lives in.
 struct Units {
-    std::vector<int32_t> ptr, row, lpar, pc, pcb, pce, seg_ptr, seg, of;
+    std::vector<int32_t> ptr, row, lpar, pc, pcb, pce, seg_ptr, seg, of, waited;
     int32_t levels = 0;
 };
 
@@ -343,6 +343,11 @@ bool build_units(int32_t n, int32_t levels, const std::vector<int32_t> &parent, 
         u.seg_ptr.push_back(int32_t(u.seg.size()));
     }
     u.pc[size_t(n)] = int32_t(u.pcb.size());
+    // units holding a parent that a later unit reads: only these publish a flag
+    u.waited.assign(size_t(n_units), 0);
+    for (int32_t i = 0; i < n; ++i)
+        if (u.lpar[size_t(i)] == -2)
+            u.waited[size_t(u.of[size_t(pos_of_node[size_t(parent[size_t(lvl_node[size_t(u.row[size_t(i)])])])])])] = 1;
     return true;
 }
 
@@ -646,6 +651,7 @@ extern "C" int cbm_plan_create(int32_t n, const int32_t *crow, const int32_t *co
                 plan->arr[CBM_P_UN_SEG_PTR].swap(un.seg_ptr);
                 plan->arr[CBM_P_UN_SEG].swap(un.seg);
                 plan->arr[CBM_P_UN_OF].swap(un.of);
+                plan->arr[CBM_P_UN_WAITED].swap(un.waited);
             }
         }
         const bool ranges = n <= CBM_RANGE_MAX_ROWS;

@@ -189,6 +189,7 @@ typedef struct cbm_desc {
     const int32_t *un_seg_ptr;
     const int32_t *un_seg;
     const int32_t *un_of;
+    const int32_t *un_waited;
 } cbm_desc;
 
 typedef struct cbm_handle cbm_handle;
@@ -399,8 +400,9 @@ int cbm_set_symmetric(cbm_handle *h, int32_t symmetric);
  * by cbm_reserve).  Used by cbm_matmul and, on a symmetric handle, by
  * cbm_matmul_t.  Each row sums its entries in pieces of CBM_UNIT_PIECE, so
  * the rounding differs from the multi-launch forms; it is deterministic.
- * Automatic: on when possible and the structure is small enough that the
- * passes are bound by launch and dependency latency (see cbm_api.hip).
+ * Automatic: on when possible for matrices of at most 16,384 rows and 65,536
+ * Delta entries, whose passes are bound by launch and dependency latency;
+ * larger ones run faster as row passes over the whole device.
  */
 int cbm_set_fused(cbm_handle *h, int32_t mode);
 

@@ -285,7 +285,8 @@ enum {
     CBM_P_UN_SEG_PTR = 81, /* int32 [U+1]  segment offsets per unit                                    */
     CBM_P_UN_SEG = 82,     /* int32 [S]    first unit row of each segment (rows of one tree level)     */
     CBM_P_UN_OF = 83,      /* int32 [N]    unit of each level position                                 */
-    CBM_P_N_ARRAYS = 84
+    CBM_P_UN_WAITED = 84,  /* int32 [U]    1 if a later unit reads a parent row of this unit, else 0   */
+    CBM_P_N_ARRAYS = 85
 };
 
 /* Scalars of a cbm_plan, addressed by cbm_plan_stat(). */
