@@ -141,18 +141,16 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_forward(const FusedArgs
         for (int q = 0; q < VE; ++q) v[q] = src[q];
         store_row<VE>(a.y + int64_t(a.node_of[a.un_row[i]]) * a.ldy + c0, v);
     }
-    // Only a unit that a later unit waits on publishes: each wave waits for
-    // its stores, the barrier joins them, and one release fence (a write-back
-    // of this XCD's L2) makes them visible device-wide before the flag.
+    // Only a unit that a later unit waits on publishes.  Every wave releases
+    // its own stores at device scope (a single fence by one thread after the
+    // barrier let waiters read stale rows), then one thread sets the flag.
     const bool waited = a.un_waited[u] != 0;
-    if (waited) __builtin_amdgcn_s_waitcnt(0);  // vmcnt, expcnt, lgkmcnt all 0
+    if (waited) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) {
-        if (waited) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (waited)
             __hip_atomic_store(a.flags + int64_t(u) * a.n_slabs + slab, want, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-        }
         const uint32_t total = uint32_t(a.n_units) * uint32_t(a.n_slabs);
         if (__hip_atomic_fetch_add(a.state + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == total - 1u) {
             // every workgroup of this call has taken its ticket, read the
