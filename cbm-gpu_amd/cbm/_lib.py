@@ -13,7 +13,9 @@ import threading
 import torch  # noqa: F401  (torch's HIP runtime is loaded first, libcbm_hip.so binds to it)
 
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
-HOST_LIB = os.path.join(LIB_DIR, "libcbm_host.so")
+# CBM_HOST_LIB selects another build of the host library, e.g. the
+# AddressSanitizer/UBSan one of `make -C cbm-gpu_amd/csrc asan` (lib/asan/).
+HOST_LIB = os.environ.get("CBM_HOST_LIB") or os.path.join(LIB_DIR, "libcbm_host.so")
 HIP_LIB = os.path.join(LIB_DIR, "libcbm_hip.so")
 
 _lock = threading.Lock()
