@@ -172,6 +172,9 @@ def declare(lib, sig):
     sig(lib, "cbm_op_spmm", c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_i32, ctypes.c_float,
         c_i32, c_vp)
     sig(lib, "cbm_op_update", c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_vp)
+    sig(lib, "cbm_op_spmm_t_workspace_bytes", c_i64, c_i32, c_i32, c_i64)
+    sig(lib, "cbm_op_spmm_t", c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp,
+        c_i64, c_vp)
     sig(lib, "cbm_op_axpy", c_i32, c_i32, c_i32, ctypes.c_float, c_vp, c_i64, c_vp, c_i64, c_vp)
     sig(lib, "cbm_candidates_last_error", ctypes.c_char_p)
     sig(lib, "cbm_cluster_last_error", ctypes.c_char_p)
@@ -189,7 +192,7 @@ EXPORTED = ("cbm_last_error", "cbm_desc_size", "cbm_create", "cbm_destroy", "cbm
             "cbm_adjoint_delta_t", "cbm_delta_spmm",
             "cbm_propagate", "cbm_last_launches", "cbm_set_slab_bytes", "cbm_xcd_ranges_active", "cbm_set_xcd_phases", "cbm_set_tree_tails", "cbm_set_symmetric", "cbm_set_fused", "cbm_fused_active", "cbm_fused_errors", "cbm_set_kahan_depth",
             "cbm_set_forward_split", "cbm_set_wave_rows", "cbm_set_mixed_grid", "cbm_set_xcd_slabs", "cbm_set_xcd_ranges", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
-            "csr_last_error", "cbm_op_spmm", "cbm_op_update", "cbm_op_axpy", "cbm_op_last_error",
+            "csr_last_error", "cbm_op_spmm", "cbm_op_spmm_t_workspace_bytes", "cbm_op_spmm_t", "cbm_op_update", "cbm_op_axpy", "cbm_op_last_error",
             "cbm_candidates_workspace_bytes", "cbm_candidates_count", "cbm_candidates_emit",
             "cbm_candidates_last_error", "cbm_cluster_column_counts", "cbm_cluster_fingerprints",
             "cbm_cluster_last_error")

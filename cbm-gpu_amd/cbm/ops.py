@@ -121,26 +121,26 @@ def s_spmm_csr_int32(lhs_row_ptr_b, lhs_row_ptr_e, lhs_col_idx, lhs_values, rhs,
 
 def s_spmm_csr_t_int32(lhs_row_ptr_b, lhs_row_ptr_e, lhs_col_idx, lhs_values, rhs, dst):
     """dst = A^T rhs (:742-782), with A given by rows; A has dst.size(0)
-    columns and rhs.size(0) rows.  Runs rocSPARSE's transposed SpMM."""
-    from .csr_cusparse import cusparseCSR
+    columns and rhs.size(0) rows.  A^T is sorted out on the device (stable
+    radix sort by column) in a workspace from torch's allocator, then summed
+    by the row-pass kernel (cbm_op_spmm_t)."""
     row_b, row_e, col, val = _csr_args(lhs_row_ptr_b, lhs_row_ptr_e, lhs_col_idx, lhs_values)
     _dense(rhs, "rhs")
     _dense(dst, "dst")
-    n_rows, n_cols = int(rhs.size(0)), int(dst.size(0))
-    b, e = row_b[:n_rows].long(), row_e[:n_rows].long()
-    lengths = e - b
-    crow = torch.zeros(n_rows + 1, dtype=torch.int64, device=rhs.device)
-    crow[1:] = torch.cumsum(lengths, 0)
-    total = int(crow[-1])
-    if total:
-        starts = torch.repeat_interleave(b - crow[:-1], lengths)
-        pos = torch.arange(total, device=rhs.device) + starts
-        col3, val3 = col.long()[pos].to(torch.int32), val[pos]
-    else:
-        col3 = torch.zeros(0, dtype=torch.int32, device=rhs.device)
-        val3 = torch.zeros(0, dtype=torch.float32, device=rhs.device)
-    a = torch.sparse_csr_tensor(crow.to(torch.int32), col3, val3, (n_rows, n_cols))
-    cusparseCSR(a).matmul_t(rhs, dst)
+    n_rows, n_cols, f = int(rhs.size(0)), int(dst.size(0)), int(rhs.size(1))
+    if int(dst.size(1)) != f or row_b.numel() < n_rows or row_e.numel() < n_rows:
+        raise ValueError("row pointer arrays or dst do not match the operand shapes")
+    nnz = int((row_e[:n_rows].long() - row_b[:n_rows].long()).clamp_min(0).sum()) if n_rows else 0
+    lib = _lib.hip()
+    ws_bytes = int(lib.cbm_op_spmm_t_workspace_bytes(n_rows, n_cols, nnz))
+    if ws_bytes < 0:
+        raise ValueError(f"no transpose workspace for {n_rows} x {n_cols} with {nnz} entries")
+    ws = torch.empty(ws_bytes + 256, dtype=torch.uint8, device=dst.device)
+    ws_ptr = (ws.data_ptr() + 255) // 256 * 256
+    rc = lib.cbm_op_spmm_t(n_rows, n_cols, row_b.data_ptr() or None, row_e.data_ptr() or None,
+                           col.data_ptr() or None, val.data_ptr() or None, nnz, rhs.data_ptr(), rhs.stride(0),
+                           dst.data_ptr(), dst.stride(0), f, ws_ptr, ws_bytes, current_stream_handle(dst.device))
+    _ok(rc, "spmm_t")
 
 
 def s_update_csr_int32(mca_branches, mca_row_idx, mca_col_idx, dst):

@@ -588,6 +588,21 @@ int cbm_op_spmm(int32_t n_rows, const int32_t *row_b, const int32_t *row_e, cons
 int cbm_op_update(int32_t n_branches, const int32_t *branches, const int32_t *src, const int32_t *dst,
                   const float *mult, float *y, int64_t ldy, int32_t n_cols, hipStream_t stream);
 
+/*
+ * dst = A^T rhs for the n_rows x n_cols matrix A given by rows (row_b /
+ * row_e as above, nnz = sum of the row lengths, column indices in
+ * [0, n_cols)); rhs is n_rows x n_feat, dst n_cols x n_feat.  Backs
+ * s_spmm_csr_t_int32 (:742-782).  A^T is formed on the device in the
+ * caller's workspace (256-byte aligned, cbm_op_spmm_t_workspace_bytes
+ * bytes) by a stable radix sort of the entries by column, so every column
+ * lists its rows in ascending order and the result is deterministic; the
+ * row-pass kernel then sums each column.  Allocates nothing.
+ */
+int64_t cbm_op_spmm_t_workspace_bytes(int32_t n_rows, int32_t n_cols, int64_t nnz);
+int cbm_op_spmm_t(int32_t n_rows, int32_t n_cols, const int32_t *row_b, const int32_t *row_e, const int32_t *col,
+                  const float *val, int64_t nnz, const float *rhs, int64_t ldr, float *dst, int64_t ldd,
+                  int32_t n_feat, void *workspace, int64_t workspace_bytes, hipStream_t stream);
+
 /* y += coef * x over n_rows x n_cols (the cblas_saxpy of :491,590). */
 int cbm_op_axpy(int32_t n_rows, int32_t n_cols, float coef, const float *x, int64_t ldx, float *y, int64_t ldy,
                 hipStream_t stream);

@@ -97,3 +97,18 @@ def test_a_file_graph_builds_like_its_edge_index(tmp_path):
     b = cbm_mkl_cpp.init(ei[0], ei[1], torch.ones(ei.shape[1]), 800, 4)
     for x, y in zip(a, b):
         assert torch.equal(x, y)
+
+
+def test_largest_workload_links_ignore_the_communities():
+    """The largest workload's graph (N = 2,449,029, 47 blocks): the share of
+    its entries whose two ends lie in the same block is that of uniformly
+    random links (1/47, about 2%), so no ordering of rows or columns by block
+    gives its gathers locality."""
+    import bench
+    w = bench.WORKLOADS["c5"]
+    ei = structure.copying_model(w["n"], w["blocks"], w["p"], w["lam"], w["seed_deg"], w["seed"]).numpy()
+    bsize = w["n"] // w["blocks"]
+    blk = lambda v: np.minimum(v // bsize, w["blocks"] - 1)
+    inside = float(np.mean(blk(ei[0]) == blk(ei[1])))
+    assert ei.shape[1] > 100_000_000
+    assert 0.01 < inside < 0.05, inside

@@ -118,3 +118,43 @@ def test_ops_reject_host_tensors_and_wrong_dtypes(setup, cuda):
         cbm_mkl_cpp.s_spmm_csr_int32(d["row_b"].cpu(), d["row_e"], d["col"], d["val"], x, y)
     with pytest.raises(RuntimeError, match="float32"):
         cbm_mkl_cpp.s_spmm_csr_int32(d["row_b"], d["row_e"], d["col"], d["val"], x.double(), y)
+
+
+def test_transposed_spmm_rectangular_gapped_and_deterministic(cuda):
+    """s_spmm_csr_t_int32 on a rectangular operand given as a 4-array CSR with
+    gaps between rows and an empty row: equal to FP64 A^T rhs, the same bits
+    on every call (the transpose is a stable sort), and the MKL-style column
+    count N = dst.size(0)."""
+    rng = np.random.default_rng(9)
+    n_rows, n_cols, f = 700, 450, 20
+    lens = rng.integers(0, 30, size=n_rows)
+    lens[5] = 0
+    gaps = rng.integers(0, 4, size=n_rows)
+    row_b = np.zeros(n_rows, np.int64)
+    pos = 0
+    cols, vals = [], []
+    for r in range(n_rows):
+        pos += gaps[r]
+        row_b[r] = pos
+        cols.append(np.sort(rng.choice(n_cols, size=lens[r], replace=False)))
+        vals.append(rng.standard_normal(lens[r]))
+        pos += lens[r]
+    total = pos
+    col = np.zeros(total, np.int32)
+    val = np.zeros(total, np.float32)
+    for r in range(n_rows):
+        col[row_b[r]:row_b[r] + lens[r]] = cols[r]
+        val[row_b[r]:row_b[r] + lens[r]] = vals[r]
+    row_e = row_b + lens
+    a = np.zeros((n_rows, n_cols))
+    for r in range(n_rows):
+        a[r, cols[r]] = vals[r].astype(np.float32)
+    x = rng.standard_normal((n_rows, f)).astype(np.float32)
+    t = lambda v: torch.from_numpy(np.ascontiguousarray(v)).to(cuda)
+    dst = torch.empty(n_cols, f, device=cuda)
+    cbm_mkl_cpp.s_spmm_csr_t_int32(t(row_b.astype(np.int32)), t(row_e.astype(np.int32)), t(col), t(val), t(x), dst)
+    ref = a.T @ x.astype(np.float64)
+    assert O.max_rel_error(dst.cpu().numpy(), ref) <= 1e-5
+    again = torch.empty_like(dst)
+    cbm_mkl_cpp.s_spmm_csr_t_int32(t(row_b.astype(np.int32)), t(row_e.astype(np.int32)), t(col), t(val), t(x), again)
+    assert torch.equal(dst, again)
