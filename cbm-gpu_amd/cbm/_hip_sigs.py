@@ -109,6 +109,7 @@ class CbmDesc(ctypes.Structure):
         ("tr_seg", c_vp),
         ("tr_pos", c_vp),
         ("n_units", c_i32),
+        ("unit_levels", c_i32),
         ("un_ptr", c_vp),
         ("un_row", c_vp),
         ("un_lpar", c_vp),

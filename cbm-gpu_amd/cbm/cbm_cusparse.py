@@ -136,6 +136,7 @@ class cusparseCBM:
                        tf_frags=max(0, int(self._dev["tf_frag"].numel()) - 1),
                        tr_frags=max(0, int(self._dev["tr_frag"].numel()) - 1),
                        n_units=max(0, int(self._dev["un_ptr"].numel()) - 1),
+                       unit_levels=int(st.get("unit_levels", 0)),
                        d=self._d.data_ptr() if self._d is not None else None, **ptrs)
         self._reserved = 0
         h = ctypes.c_void_p()
@@ -337,9 +338,9 @@ class cusparseCBM:
 
     def set_fused(self, mode):
         """The forward product in one launch (units of the plan, propagation
-        in LDS): None automatic (the default), False off, True on where
-        possible.  Rounds each row in pieces, so bits differ from the
-        multi-launch forms."""
+        in LDS): True on where possible, False or None (automatic, the
+        default) off -- the multi-launch forms measured faster.  Rounds each
+        row in pieces, so bits differ from the multi-launch forms."""
         m = -1 if mode is None else int(bool(mode))
         _lib.check_hip(self._lib.cbm_set_fused(self._h, m), "set_fused")
 

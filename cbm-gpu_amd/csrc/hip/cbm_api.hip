@@ -55,7 +55,7 @@ struct cbm_handle {
     int64_t comp_floats = 0;
     int32_t reserved_cols = 0; // widest product the buffers serve (cbm_reserve)
     int32_t symmetric = 0;     // A = A^T declared: the adjoint runs the forward structure
-    int32_t fused = -1;        // one-launch forward: 1 on where possible, 0 off, -1 automatic
+    int32_t fused = -1;        // one-launch forward: 1 on where possible, 0 off, -1 automatic (off)
     uint32_t *fused_state = nullptr; // ticket, done, epoch, failed waits (cbm_reserve)
     uint32_t *fused_flags = nullptr; // a flag per unit and 64-column slab
     int64_t fused_flag_count = 0;
@@ -758,24 +758,21 @@ namespace {
 // Whether a forward-structure product of n_cols columns in this storage type
 // runs as the one-launch kernel (cbm_set_fused).  It needs the plan's units
 // (built only for matrices of up to CBM_UNIT_MAX_ROWS rows whose Delta rows
-// fit a unit).  Automatic: only for small matrices, up to kFusedAutoRows rows
-// and kFusedAutoEntries Delta entries, where the forward is bound by the
-// latency of its launches and dependent levels.  Measured BF16 step (forward
-// + adjoint, both one launch vs multi-launch): N = 2,708 (7k entries, depth
-// 9) 0.035 vs 0.039 ms; N = 19,717 (55k entries, depth 12) 0.123 vs 0.092 ms;
-// N = 169,343 (2.3M entries, depth 5) 0.751 vs 0.259 ms.  Larger units run
-// with fewer waves per CU (64 KB of LDS each) and narrower gathers than the
-// row passes, which costs more than the launches saved.
-constexpr int32_t kFusedAutoRows = 16384;
-constexpr int64_t kFusedAutoEntries = int64_t(1) << 16;
-
+// fit a unit) and is used only when asked for (fused == 1): measured forward
+// times (BF16, one launch vs the multi-launch forms, microseconds): N = 2,708
+// (3 unit levels) 20.8 vs 18.4; N = 19,717 at alpha = 0 (5 unit levels) 74
+// vs 46, at alpha = 8 (no waits) 28.6 vs 26.8; N = 169,343 182 vs 129.  Its
+// units run with fewer waves per CU (64 KB of LDS each) and narrower gathers
+// than the row passes, and every wait between units costs a device-scope
+// release (an L2 write-back) and a poll, which together cost more than the
+// launches they save.
 bool fused_possible(const cbm_handle *h, int32_t dtype, int32_t n_cols, bool ranges) {
     if (h->fused == 0 || h->d.n_units <= 0 || !h->fused_state) return false;
     if (int64_t(h->d.n_units) * ((n_cols + 63) / 64) > h->fused_flag_count) return false;
     const bool compensated = h->kahan_level >= 0 && std::max<int32_t>(h->kahan_level, 1) < h->d.n_levels;
     if (compensated || h->slab_bytes > 0 || h->xcd_slabs > 1) return false;
     if (ranges && pick_ranges(h, n_cols, elem_of(dtype))) return false;
-    return h->fused == 1 || (h->d.n_rows <= kFusedAutoRows && h->d.nnz <= kFusedAutoEntries);
+    return h->fused == 1;
 }
 
 // The forward product in buffers b; ranges: whether the XCD column ranges
@@ -811,6 +808,7 @@ int forward_product(cbm_handle *h, const Bufs &b, bool ranges, int32_t form, int
         a.ncols = n_cols;
         a.n_units = h->d.n_units;
         a.n_slabs = (n_cols + 63) / 64;
+        a.ordered = h->d.unit_levels > 1 ? 1 : 0;
         a.state = h->fused_state;
         a.flags = h->fused_flags;
         const hipError_t e = cbmk::launch_fused_forward(a, dtype == CBM_BF16, stream);

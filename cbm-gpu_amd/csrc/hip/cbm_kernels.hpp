@@ -200,6 +200,10 @@ struct FusedArgs {
     int32_t ncols = 0;
     int32_t n_units = 0;
     int32_t n_slabs = 0;  // ceil(ncols / 64)
+    // Whether units wait on one another (more than one component level):
+    // only then are units taken by ticket and the state words used; without
+    // waits workgroup b takes unit b / n_slabs and no atomic is issued.
+    int32_t ordered = 1;
     uint32_t *state = nullptr;
     uint32_t *flags = nullptr;
     uint32_t spin_limit = 1u << 20;

@@ -37,9 +37,15 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_forward(const FusedArgs
     constexpr int kGroups = kFusedThreads / LPR;
     __shared__ float lds[CBM_UNIT_PIECES * kSlabCols];
     __shared__ uint32_t s_ticket, s_epoch;
-    if (threadIdx.x == 0) {
-        s_epoch = __hip_atomic_load(a.state + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_ticket = __hip_atomic_fetch_add(a.state, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (a.ordered) {
+        if (threadIdx.x == 0) {
+            s_epoch = __hip_atomic_load(a.state + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_ticket = __hip_atomic_fetch_add(a.state, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+    } else if (threadIdx.x == 0) {
+        s_epoch = 0;
+        s_ticket = blockIdx.x;
     }
     __syncthreads();
     const uint32_t ticket = s_ticket, want = s_epoch + 1u;
@@ -144,8 +150,9 @@ __global__ __launch_bounds__(kFusedThreads) void k_fused_forward(const FusedArgs
     // Only a unit that a later unit waits on publishes.  Every wave releases
     // its own stores at device scope (a single fence by one thread after the
     // barrier let waiters read stale rows), then one thread sets the flag.
-    const bool waited = a.un_waited[u] != 0;
+    const bool waited = a.ordered && a.un_waited[u] != 0;
     if (waited) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (!a.ordered) return;  // no flags, no counters
     __syncthreads();
     if (threadIdx.x == 0) {
         if (waited)

@@ -177,9 +177,10 @@ typedef struct cbm_desc {
     const int32_t *tr_frag;
     const int32_t *tr_seg;
     const int32_t *tr_pos;
-    /* units of the one-launch forward (plan arrays CBM_P_UN_*, stat UNITS;
-       n_units 0 and NULL pointers when the plan has none) */
+    /* units of the one-launch forward (plan arrays CBM_P_UN_*, stats UNITS
+       and UNIT_LEVELS; n_units 0 and NULL pointers when the plan has none) */
     int32_t n_units;
+    int32_t unit_levels;
     const int32_t *un_ptr;
     const int32_t *un_row;
     const int32_t *un_lpar;
@@ -400,9 +401,9 @@ int cbm_set_symmetric(cbm_handle *h, int32_t symmetric);
  * by cbm_reserve).  Used by cbm_matmul and, on a symmetric handle, by
  * cbm_matmul_t.  Each row sums its entries in pieces of CBM_UNIT_PIECE, so
  * the rounding differs from the multi-launch forms; it is deterministic.
- * Automatic: on when possible for matrices of at most 16,384 rows and 65,536
- * Delta entries, whose passes are bound by launch and dependency latency;
- * larger ones run faster as row passes over the whole device.
+ * Automatic (-1) leaves it off: on MI355X the multi-launch forms measured
+ * faster at every size tried (a unit's 64 KB of LDS halves the waves per CU,
+ * and each wait between units costs a device-scope release and a poll).
  */
 int cbm_set_fused(cbm_handle *h, int32_t mode);
 

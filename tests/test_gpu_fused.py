@@ -1,5 +1,5 @@
 """The forward product in one launch (k_fused_forward over the plan's
-units): Delta pieces gathered into LDS, propagation level by level in LDS,
+units; opt-in, forced on here): Delta pieces gathered into LDS, propagation level by level in LDS,
 outside parents read after their unit's flag.  Checked against the FP64
 oracle (normwise relative error <= 1e-5, FP32 and BF16 storage against the
 rounded inputs), integer X bitwise exact, equal within FP32 rounding to the
@@ -138,6 +138,7 @@ def test_fused_graph_replay_and_launch_count(cuda):
     h = mats["dadx"].cusparse_extension
     f = 64
     h.reserve(f)
+    h.set_fused(True)
     x = torch.from_numpy(_rand((n, f), 1)).to(cuda)
     g = torch.from_numpy(_rand((n, f), 2)).to(cuda)
     y = torch.empty(n, f, device=cuda)
@@ -168,6 +169,7 @@ def test_fused_graph_replay_and_launch_count(cuda):
         graph.replay()
         torch.cuda.synchronize()
         assert torch.equal(y, y_ref) and torch.equal(dx, dx_ref), it
+    h.set_fused(None)
     assert h.fused_errors() == 0
 
 
