@@ -8,6 +8,18 @@ from ._hip_sigs import MODE_AX
 from .autograd import cbm_apply
 
 
+# The uncompressed form is kept beside the CBM structure for matrices up to
+# this size (its layout costs about as much memory as A's CSR), when the
+# constructor's `uncompressed` is True or None with this default True.
+UNCOMPRESSED = True
+UNCOMPRESSED_MAX_ROWS = 1 << 18
+UNCOMPRESSED_MAX_NNZ = 1 << 24
+
+
+def _wants_uncompressed(num_rows, edge_index):
+    return num_rows <= UNCOMPRESSED_MAX_ROWS and int(edge_index.shape[1]) <= UNCOMPRESSED_MAX_NNZ
+
+
 def _num_rows(edge_index, num_nodes):
     if num_nodes is not None:
         return int(num_nodes)
@@ -24,12 +36,17 @@ class cbm4mm:
     ``cusparse_extension``.  ``matmul(x, y)`` writes ``y = A x`` in place.
     Additions: ``x @``-style autograd product via :meth:`apply`, BF16 ``x``,
     ``num_nodes=``, the candidate-edge scan on the GPU (``candidates_on``),
-    the build statistics in ``build_stats``, and :meth:`save` / :meth:`load`
-    of the built structure.
+    the build statistics in ``build_stats``, :meth:`save` / :meth:`load`
+    of the built structure, and ``uncompressed``: for matrices of up to
+    UNCOMPRESSED_MAX_ROWS rows the same product over A itself is kept beside
+    the CBM form and the handle runs it where a cost model predicts it
+    faster (small graphs, whose tree levels are dependent launches);
+    False keeps the CBM form only.  Loaded structures have no A, so they
+    run the CBM form.
     """
 
     def __init__(self, edge_index, edge_values, alpha=0, *, num_nodes=None, device="cuda", n_threads=0,
-                 candidates_on="device"):
+                 candidates_on="device", uncompressed=None):
         num_rows = _num_rows(edge_index, num_nodes)
         # The candidate-edge scan of the build runs on the target GPU
         # ("device"), or on the host (None); the structure is the same.
@@ -38,8 +55,12 @@ class cbm4mm:
         built = cbm_mkl.build(edge_index[0], edge_index[1], edge_values, num_rows, alpha, n_threads=n_threads,
                               candidates_on=candidates_on)
         self.alpha = int(alpha)
+        if uncompressed is None:
+            uncompressed = UNCOMPRESSED
+        self._plain_edges = edge_index if uncompressed and _wants_uncompressed(num_rows, edge_index) else None
         self._assemble(num_rows, built.csr_ptr, built.csr_col, built.csr_val, built.branches, built.tree_src,
                        built.tree_dst, built.stats, device, n_threads)
+        self._plain_edges = None
 
     def _assemble(self, num_rows, crow, col, val, branches, tree_src, tree_dst, stats, device, n_threads, plan=None):
         self.build_stats = stats
@@ -57,6 +78,16 @@ class cbm4mm:
         # A = A^T (the build compares them exactly): the adjoint then runs
         # through the forward structure.
         self.cusparse_extension.set_symmetric(bool(stats.get("symmetric", 0)))
+        edges = getattr(self, "_plain_edges", None)
+        if edges is not None:
+            # The same product over A itself, which the handle runs where the
+            # tree's dependent launches would cost more than Delta saves.
+            from .csr4mm import csr_from_edge_index
+            a = csr_from_edge_index(edges, num_rows, device)
+            e0 = torch.zeros(0, dtype=torch.int32, device=device)
+            plain = cusparse.cusparseCBM(a, torch.zeros(1, dtype=torch.int32, device=device), e0, e0, dummy,
+                                         mode=MODE_AX, n_threads=n_threads)
+            self.cusparse_extension.attach_uncompressed(plain)
         self._scale()
 
     def _scale(self):

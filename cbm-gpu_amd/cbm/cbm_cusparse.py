@@ -117,6 +117,8 @@ class cusparseCBM:
         self._dev = {k: plan.arrays[k].to(self.device) for k in DEVICE_ARRAYS}
         self._lib = _lib.hip()
         self._symmetric = False
+        self._plain = None
+        self._plain_mode = -1
         self._bind(mode, scale)
 
     def _bind(self, mode, scale):
@@ -152,17 +154,42 @@ class cusparseCBM:
         layout; the layout arrays are shared, not copied."""
         other = object.__new__(type(self))
         for k in ("device", "num_rows", "depth", "nnz", "n_edges", "level_sizes", "plan_stats", "_host", "_dev",
-                  "_lib", "_symmetric"):
+                  "_lib", "_symmetric", "_plain_mode"):
             setattr(other, k, getattr(self, k))
         if mode != MODE_AX and scale is None:
             raise ValueError(f"mode {mode} needs the D^-1/2 vector as `scale`")
+        other._plain = None
         other._bind(mode, scale)
+        if self._plain is not None:
+            other.attach_uncompressed(self._plain.with_scaling(mode, scale), self._plain_mode)
         return other
+
+    def attach_uncompressed(self, plain, mode=None):
+        """Attaches ``plain``, a :class:`cusparseCBM` of the same product over
+        A itself (no tree), and lets the product calls run over it: ``mode``
+        None (automatic, by the launch/bytes cost model of
+        cbm_attach_uncompressed), True always, False never."""
+        code = -1 if mode is None or mode == -1 else int(bool(mode))
+        if plain is not None and (plain.num_rows != self.num_rows or plain.mode != self.mode):
+            raise ValueError("the uncompressed handle must have the same rows and product")
+        _lib.check_hip(self._lib.cbm_attach_uncompressed(self._h, plain._h if plain is not None else None, code),
+                       "attach_uncompressed")
+        self._plain, self._plain_mode = plain, code
+        if plain is not None:
+            plain._symmetric = self._symmetric
+            plain._reserved = max(plain._reserved, self._reserved)
+
+    def uncompressed_active(self, dtype, n_cols):
+        """Whether a product with dtype storage and n_cols columns runs over
+        the attached uncompressed handle now."""
+        code = self._lib.cbm_uncompressed_active(self._h, BF16 if dtype == torch.bfloat16 else F32, int(n_cols))
+        _lib.check_hip(min(code, 0), "uncompressed_active")
+        return bool(code)
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            self._lib.cbm_destroy(h)
+            self._lib.cbm_destroy(h)  # before the attached uncompressed handle it borrows
             self._h = None
 
     @property
@@ -321,6 +348,8 @@ class cusparseCBM:
         capture; under capture the width must already be reserved."""
         _lib.check_hip(self._lib.cbm_reserve(self._h, int(n_cols)), "reserve")
         self._reserved = max(self._reserved, int(n_cols))
+        if self._plain is not None:
+            self._plain._reserved = max(self._plain._reserved, int(n_cols))
 
     @property
     def symmetric(self):
@@ -335,6 +364,8 @@ class cusparseCBM:
         set it from the structure build's exact comparison of A with A^T."""
         _lib.check_hip(self._lib.cbm_set_symmetric(self._h, int(bool(symmetric))), "set_symmetric")
         self._symmetric = bool(symmetric)
+        if self._plain is not None:
+            self._plain._symmetric = bool(symmetric)
 
     def set_fused(self, mode):
         """The forward product in one launch (units of the plan, propagation

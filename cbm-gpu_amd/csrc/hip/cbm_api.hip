@@ -55,6 +55,8 @@ struct cbm_handle {
     int64_t comp_floats = 0;
     int32_t reserved_cols = 0; // widest product the buffers serve (cbm_reserve)
     int32_t symmetric = 0;     // A = A^T declared: the adjoint runs the forward structure
+    cbm_handle *plain = nullptr; // the same product over A itself (no tree), borrowed (cbm_attach_uncompressed)
+    int32_t plain_mode = -1;     // products over `plain`: 1 always, 0 never, -1 by the cost model
     int32_t fused = -1;        // one-launch forward: 1 on where possible, 0 off, -1 automatic (off)
     uint32_t *fused_state = nullptr; // ticket, done, epoch, failed waits (cbm_reserve)
     uint32_t *fused_flags = nullptr; // a flag per unit and 64-column slab
@@ -748,9 +750,83 @@ extern "C" int cbm_reserve(cbm_handle *h, int32_t n_cols) {
         return CBM_ERR_ARG;
     }
     const int32_t cols = std::max(h->reserved_cols, n_cols);
-    const int rc = grow_buffers(h, cols);
+    int rc = grow_buffers(h, cols);
     if (rc == CBM_OK) h->reserved_cols = cols;
+    if (rc == CBM_OK && h->plain) rc = cbm_reserve(h->plain, cols);
     return rc;
+}
+
+namespace {
+
+// Launches of one forward product over this handle's layout (split form):
+// the Delta pass (one grid, two where chunks and short rows stay apart), the
+// reduction of multi-chunk rows, and the propagation (per-level launches down
+// to the tree tail, then the tail).
+int64_t forward_launches(const cbm_handle *h) {
+    const int32_t L = h->d.n_levels;
+    int64_t n = 1 + (h->fred_ptr[size_t(L)] > 0 ? 1 : 0);
+    if (L > 1) {
+        const bool tail = tails_on(h) && h->d.tf_level < L;
+        n += (tail ? h->d.tf_level - 1 : L - 1) + (tail ? 1 : 0);
+    }
+    return n;
+}
+
+// The cost model that picks between the compressed product and the same
+// product over A (cbm_attach_uncompressed): t = launches x kLaunchUs + the
+// bytes the gathers and the output move / kGatherBytesPerUs.  A launch costs
+// about 8 us on MI355X whatever its size (a 2,708-row row pass and a 2,708-
+// row propagation level alike), and random row gathers run at about 7 TB/s
+// (the gather ceiling of the Delta pass).  The tree's launches are worth it
+// only when the gathers that Delta saves take longer than they do: with
+// these constants N = 2,708 (depth 9, nnz 11k -> 7k) and N = 19,717 (depth
+// 12, nnz 92k -> 55k) run over A, while N = 132,534 (nnz 80M -> 41M) and
+// N = 2,449,029 (nnz 122M -> 117M) run compressed.
+constexpr double kLaunchUs = 8.0;
+constexpr double kGatherBytesPerUs = 7.0e6;
+
+double forward_cost_us(const cbm_handle *h, int32_t n_cols, int elem) {
+    const double bytes = double(h->d.nnz) * n_cols * elem + double(h->d.n_rows) * n_cols * 4.0;
+    return double(forward_launches(h)) * kLaunchUs + bytes / kGatherBytesPerUs;
+}
+
+// The handle a product call of this width and storage type runs on.
+cbm_handle *route(cbm_handle *h, int32_t dtype, int32_t n_cols) {
+    if (!h->plain || h->plain_mode == 0) return h;
+    if (h->plain_mode == 1) return h->plain;
+    const int elem = dtype == CBM_BF16 ? 2 : 4;
+    return forward_cost_us(h->plain, n_cols, elem) < forward_cost_us(h, n_cols, elem) ? h->plain : h;
+}
+
+}  // namespace
+
+extern "C" int cbm_attach_uncompressed(cbm_handle *h, cbm_handle *plain, int32_t mode) {
+    if (!h || mode < -1 || mode > 1 || plain == h) {
+        set_error("cbm_attach_uncompressed: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    if (plain && (plain->d.n_rows != h->d.n_rows || plain->d.n_levels != 1 || plain->d.n_edges != 0 ||
+                  plain->d.mode != h->d.mode || plain->plain)) {
+        set_error("cbm_attach_uncompressed: the uncompressed handle must have %d rows, no tree edges, mode %d and "
+                  "no uncompressed handle of its own",
+                  h->d.n_rows, h->d.mode);
+        return CBM_ERR_ARG;
+    }
+    h->plain = plain;
+    h->plain_mode = mode;
+    if (plain) {
+        plain->symmetric = h->symmetric;
+        return cbm_reserve(plain, h->reserved_cols);
+    }
+    return CBM_OK;
+}
+
+extern "C" int cbm_uncompressed_active(cbm_handle *h, int32_t dtype, int32_t n_cols) {
+    if (!h || (dtype != CBM_F32 && dtype != CBM_BF16) || n_cols < 1) {
+        set_error("cbm_uncompressed_active: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    return route(h, dtype, n_cols) != h ? 1 : 0;
 }
 
 namespace {
@@ -840,6 +916,11 @@ extern "C" int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t l
     int rc = check_dense(h, dtype, x, ldx, y, ldy, n_cols, "cbm_matmul");
     if (rc == CBM_OK) rc = check_reserved(h, n_cols, "cbm_matmul");
     if (rc != CBM_OK) return rc;
+    if (cbm_handle *p = route(h, dtype, n_cols); p != h) {
+        rc = cbm_matmul(p, dtype, x, ldx, y, ldy, n_cols, stream);
+        h->last_launches = p->last_launches;
+        return rc;
+    }
     return forward_product(h, handle_bufs(h), true, h->d.mode, dtype, x, ldx, y, ldy, n_cols, stream, "cbm_matmul");
 }
 
@@ -1012,6 +1093,13 @@ int adjoint_delta_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, co
 
 extern "C" int cbm_matmul_t(cbm_handle *h, int32_t dtype, const void *g, int64_t ldg, float *dx, int64_t lddx,
                             float *z, int64_t ldz, int32_t n_cols, hipStream_t stream) {
+    if (h && n_cols >= 1 && (dtype == CBM_F32 || dtype == CBM_BF16) && n_cols <= h->reserved_cols) {
+        if (cbm_handle *p = route(h, dtype, n_cols); p != h) {
+            const int rc = cbm_matmul_t(p, dtype, g, ldg, dx, lddx, z, ldz, n_cols, stream);
+            h->last_launches = p->last_launches;
+            return rc;
+        }
+    }
     if (h && h->symmetric) {
         // A = A^T: M^T = M for A and D A D, and (A D)^T = D A, all through
         // the forward structure (no Delta^T pass, no reverse gather, z unused).
@@ -1082,6 +1170,7 @@ extern "C" int cbm_matmul_noalloc(cbm_handle *h, int32_t dtype, const void *x, i
                                   int32_t n_cols, hipStream_t stream) {
     int rc = check_dense(h, dtype, x, ldx, y, ldy, n_cols, "cbm_matmul_noalloc");
     if (rc != CBM_OK) return rc;
+    if (cbm_handle *p = route(h, dtype, n_cols); p != h) return cbm_matmul_noalloc(p, dtype, x, ldx, y, ldy, n_cols, stream);
     float *base = nullptr;  // this device's copy of `scratch`
     rc = noalloc_scratch_address(&base);
     if (rc != CBM_OK) return rc;
@@ -1194,6 +1283,7 @@ extern "C" int cbm_set_symmetric(cbm_handle *h, int32_t symmetric) {
         return CBM_ERR_ARG;
     }
     h->symmetric = symmetric;
+    if (h->plain) h->plain->symmetric = symmetric;
     return CBM_OK;
 }
 

@@ -420,6 +420,27 @@ int cbm_fused_active(const cbm_handle *h, int32_t dtype, int32_t n_cols);
 int64_t cbm_fused_errors(cbm_handle *h);
 
 /*
+ * Attaches `plain`, a handle of the same product over A itself (a layout
+ * with no tree edges, so Delta = A; created by the caller and borrowed until
+ * both are destroyed), and chooses when the product calls run over it
+ * instead: mode 1 always, 0 never, -1 (automatic) when a cost model --
+ * launches per product x 8 us plus gathered and written bytes / 7 TB/s --
+ * predicts it faster.  On small graphs the tree's dependent launches cost
+ * more than the gathers Delta saves (N = 2,708 and 19,717 run over A), on
+ * large ones the reverse.  cbm_matmul, cbm_matmul_t and cbm_matmul_noalloc
+ * follow the choice; cbm_delta_spmm, cbm_propagate and the adjoint halves
+ * always run the compressed form.  cbm_reserve reserves both handles, and
+ * cbm_set_symmetric applies to both.  plain = NULL detaches.  No reference
+ * counterpart: the reference always runs the compressed product.
+ */
+int cbm_attach_uncompressed(cbm_handle *h, cbm_handle *plain, int32_t mode);
+
+/* Whether a dtype product of n_cols columns runs over the attached
+   uncompressed handle now (1) or compressed (0); CBM_ERR_ARG for invalid
+   arguments. */
+int cbm_uncompressed_active(cbm_handle *h, int32_t dtype, int32_t n_cols);
+
+/*
  * Tree tails (1 on, the default; 0 off).  The deepest tree levels of the
  * forward propagation and of the adjoint's reverse pass, each of which would
  * be a short dependent launch, are finished in one launch: one workgroup per
