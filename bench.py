@@ -254,20 +254,32 @@ def measure(name, w, ei, m, args, dev, stream):
     ms32_graph = graph_ms(step(x32, g32))
     log(f"{name}: CBM step bf16 {ms16:.3f} ms, fp32 {ms32:.3f} ms; graph-replayed {ms16_graph:.3f} / {ms32_graph:.3f}")
     fused = {"bf16": h.fused_active(torch.bfloat16, f), "fp32": h.fused_active(torch.float32, f)}
-    # the same step with the one-launch forward off (multi-launch forms), and
-    # with the adjoint through the Delta^T pass instead of the forward structure
+    over_a = {"bf16": h.uncompressed_active(torch.bfloat16, f), "fp32": h.uncompressed_active(torch.float32, f)}
+    # the same step in the other forms: compressed where the cost model runs
+    # it over A, the one-launch forward (forced on where the plan has units),
+    # and the adjoint through the Delta^T pass instead of the forward structure
     forms = {}
-    if any(fused.values()):
-        h.set_fused(False)
-        forms["multi_launch"] = {"bf16": round(time_loop(step(x16, g16), K, W, stream), 4),
-                                 "fp32": round(time_loop(step(x32, g32), K, W, stream), 4)}
+    if any(over_a.values()):
+        h.attach_uncompressed(h._plain, False)
+        forms["compressed"] = {"bf16": round(time_loop(step(x16, g16), K, W, stream), 4),
+                               "fp32": round(time_loop(step(x32, g32), K, W, stream), 4)}
+        h.attach_uncompressed(h._plain, None)
+    if h.plan_stats.get("units", 0) > 0:
+        h.set_fused(True)
+        if h._plain is not None:
+            h.attach_uncompressed(h._plain, False)
+        if h.fused_active(torch.bfloat16, f):
+            forms["one_launch_forward"] = {"bf16": round(time_loop(step(x16, g16), K, W, stream), 4),
+                                           "fp32": round(time_loop(step(x32, g32), K, W, stream), 4)}
+        if h._plain is not None:
+            h.attach_uncompressed(h._plain, None)
         h.set_fused(None)
     if h.symmetric:
         h.set_symmetric(False)
         forms["delta_t_adjoint"] = {"bf16": round(time_loop(step(x16, g16), K, W, stream), 4),
                                     "fp32": round(time_loop(step(x32, g32), K, W, stream), 4)}
         h.set_symmetric(True)
-    log(f"{name}: one launch {fused}; other forms {forms}")
+    log(f"{name}: over A {over_a}; one launch {fused}; other forms {forms}")
     ph = {
         "forward": time_loop(lambda: h.forward(x16, y), K, W, stream),
         "forward_delta_spmm": time_loop(lambda: h.delta_spmm(x16, y), K, W, stream),
@@ -389,7 +401,7 @@ def measure(name, w, ei, m, args, dev, stream):
         "bytes_step_bf16": b16["step"], "bytes_phase_bf16": b16,
         "dominant_phase": dom, "dominant_gbs": round(dom_gbs, 1),
         "gather_gbs_dominant": round(b16["gather_delta"] / (ph[dom] * 1e-3) / 1e9, 1),
-        "symmetric_adjoint": h.symmetric, "one_launch_forward": fused, "launches": launches,
+        "symmetric_adjoint": h.symmetric, "one_launch_forward": fused, "runs_over_a": over_a, "launches": launches,
         "ms_step_other_forms": forms,
         "max_rel_error_bf16_vs_fp32_cbm": vs_fp32,
         "ms_xcd_slabs_bf16": xcd,
