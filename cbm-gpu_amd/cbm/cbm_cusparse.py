@@ -432,8 +432,17 @@ class cusparseCBM:
 
     def set_tree_tails(self, on):
         """Tree tails in one launch each (True, the default) or per level
-        (False); the forward's bits do not change."""
-        _lib.check_hip(self._lib.cbm_set_tree_tails(self._h, int(bool(on))), "set_tree_tails")
+        (False); ``"unstaged"`` and ``"all_staged"`` keep the tails with no
+        forward fragment, or every one that fits, staged in LDS (the default
+        stages those of at least 8 levels).  The forward's bits do not
+        change."""
+        codes = {"unstaged": 2, "all_staged": 3}
+        code = codes[on] if isinstance(on, str) and on in codes else None
+        if code is None:
+            if isinstance(on, str):
+                raise ValueError(f"set_tree_tails: a bool, 'unstaged' or 'all_staged', got {on!r}")
+            code = int(bool(on))
+        _lib.check_hip(self._lib.cbm_set_tree_tails(self._h, code), "set_tree_tails")
 
     def set_xcd_phases(self, phases):
         """Column phases of the XCD range passes: None automatic, 1, 2 or 4

@@ -37,7 +37,10 @@ struct cbm_handle {
     int32_t xcd_slabs = 1;    // feature slabs spread over the XCDs (1 = off)
     int32_t xcd_ranges = -1;  // column ranges per XCD: 1 on, 0 off, -1 by size, 8 / 4 that piece set
     int32_t xcd_phases = -1;  // column phases of a range pass: 1, 2, 4, or -1 automatic
-    int32_t tree_tails = 1;   // tree tails in one launch each (1) or per level (0)
+    int32_t tree_tails = 1;   // tree tails: 1 deep fragments staged, 2 none staged, 3 all that fit, 0 per level
+    int32_t *tf_order = nullptr;  // forward tail fragments: the stageable ones, deepest first, then the rest
+    int32_t tf_staged_deep = 0, tf_rows_deep = 0;  // the stageable ones of at least kTailStagedMinSegs levels
+    int32_t tf_staged_all = 0, tf_rows_all = 0;    // every stageable one
     // Column-range piece sets: [0] 8 ranges (one per XCD), [1] 4 ranges (each
     // on two XCDs, one column half each).
     struct RangeSet {
@@ -435,6 +438,31 @@ int delta_flat(cbm_handle *h, const Bufs &b, cbmk::RowPass rp, hipStream_t strea
 
 bool tails_on(const cbm_handle *h) { return h->tree_tails != 0; }
 
+// Forward tail fragments of at least this many levels are staged in LDS by
+// default (their chains of dependent levels are the tail's critical path;
+// the short ones cost one or two global round trips either way).
+constexpr int32_t kTailStagedMinSegs = 8;
+
+// The forward tail of the handle's tail mode: which fragments are staged.
+cbmk::TailArgs forward_tail(const cbm_handle *h) {
+    cbmk::TailArgs t;
+    t.frag = h->d.tf_frag;
+    t.seg = h->d.tf_seg;
+    t.pos = h->d.tf_pos;
+    t.n_frag = h->d.tf_frags;
+    if (h->tf_order && (h->tree_tails == 1 || h->tree_tails == 3)) {
+        t.order = h->tf_order;
+        t.n_staged = h->tree_tails == 1 ? h->tf_staged_deep : h->tf_staged_all;
+        t.staged_rows = h->tree_tails == 1 ? h->tf_rows_deep : h->tf_rows_all;
+    }
+    return t;
+}
+
+int32_t tail_launches(const cbmk::TailArgs &t) {
+    const int32_t staged = t.order ? t.n_staged : 0;
+    return (staged > 0 ? 1 : 0) + (staged < t.n_frag ? 1 : 0);
+}
+
 // In-place propagation of levels 1 .. L-1, y_v += s_v y_parent(v), with the
 // compensated form from the handle's Kahan level on: one launch per level
 // down to the tree tail, then one launch for the tail.
@@ -458,16 +486,12 @@ int propagate_levels(cbm_handle *h, const Bufs &b, int32_t form, float *y, int64
         ++h->last_launches;
     }
     if (tail) {
-        cbmk::TailArgs t;
-        t.frag = h->d.tf_frag;
-        t.seg = h->d.tf_seg;
-        t.pos = h->d.tf_pos;
-        t.n_frag = h->d.tf_frags;
+        const cbmk::TailArgs t = forward_tail(h);
         const bool comp = h->kahan_level >= 0 && std::max<int32_t>(h->kahan_level, 1) < L;
         const hipError_t err = cbmk::launch_propagate_tail(t, h->d.lvl_node, h->d.lvl_parent, ps, cm,
                                                            comp ? cm.base : INT32_MAX, y, ldy, n_cols, stream);
         if (err != hipSuccess) return device_error(err, what);
-        ++h->last_launches;
+        h->last_launches += tail_launches(t);
     }
     return CBM_OK;
 }
@@ -566,6 +590,65 @@ cbmk::RowPass forward_pass(const cbm_handle *h, int32_t form, int32_t dtype, con
     rp.mixed_grid = h->mixed_grid;
     rp.xcd_slabs = h->xcd_slabs;
     return rp;
+}
+
+// Orders the forward tail's fragments for staging: those whose rows fit the
+// LDS of a staged workgroup (with compensation rows, the larger case) first,
+// most levels first, then the rest in plan order; counts the ones to stage
+// by default (at least kTailStagedMinSegs levels) and in all, with the rows
+// the largest of each set needs (a multiple of 4, for 16-byte LDS rows).
+// Runs once, at the first reservation (it reads the plan's tail tables back
+// from the device, so it waits for the device).
+int stage_forward_tail(cbm_handle *h) {
+    const int32_t nf = h->d.tf_frags;
+    std::vector<int32_t> frag(size_t(nf) + 1);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(frag.data(), h->d.tf_frag, sizeof(int32_t) * frag.size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return device_error(e, "cbm_reserve (tree tail)");
+    bool ok = frag[0] == 0;
+    for (int32_t f = 0; ok && f < nf; ++f) ok = frag[size_t(f) + 1] >= frag[size_t(f)];
+    if (!ok) {
+        set_error("cbm_reserve: forward tree tail fragments are not ascending");
+        return CBM_ERR_ARG;
+    }
+    std::vector<int32_t> seg(size_t(frag[size_t(nf)]) + 1);
+    e = hipMemcpy(seg.data(), h->d.tf_seg, sizeof(int32_t) * seg.size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return device_error(e, "cbm_reserve (tree tail)");
+    for (size_t k = 0; ok && k + 1 < seg.size(); ++k) ok = seg[k + 1] >= seg[k];
+    if (!ok || seg[0] != 0) {
+        set_error("cbm_reserve: forward tree tail segments are not ascending");
+        return CBM_ERR_ARG;
+    }
+    auto round4 = [](int32_t r) { return (r + 3) / 4 * 4; };
+    std::vector<int32_t> fit, rest;
+    for (int32_t f = 0; f < nf; ++f) {
+        const int32_t rows = seg[size_t(frag[size_t(f) + 1])] - seg[size_t(frag[size_t(f)])];
+        const bool fits = rows > 0 && cbmk::tail_staged_bytes(round4(rows), true) <= cbmk::kTailStagedMaxBytes;
+        (fits ? fit : rest).push_back(f);
+    }
+    auto levels = [&](int32_t f) { return frag[size_t(f) + 1] - frag[size_t(f)]; };
+    auto rows_of = [&](int32_t f) { return seg[size_t(frag[size_t(f) + 1])] - seg[size_t(frag[size_t(f)])]; };
+    std::stable_sort(fit.begin(), fit.end(), [&](int32_t a, int32_t b) { return levels(a) > levels(b); });
+    h->tf_staged_all = int32_t(fit.size());
+    h->tf_staged_deep = 0;
+    h->tf_rows_all = h->tf_rows_deep = 0;
+    for (const int32_t f : fit) {
+        h->tf_rows_all = std::max(h->tf_rows_all, round4(rows_of(f)));
+        if (levels(f) >= kTailStagedMinSegs) {
+            ++h->tf_staged_deep;
+            h->tf_rows_deep = std::max(h->tf_rows_deep, round4(rows_of(f)));
+        }
+    }
+    fit.insert(fit.end(), rest.begin(), rest.end());
+    int32_t *order = nullptr;
+    e = hipMalloc(&order, sizeof(int32_t) * size_t(std::max(nf, 1)));
+    if (e == hipSuccess) e = hipMemcpy(order, fit.data(), sizeof(int32_t) * fit.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (order) (void)hipFree(order);
+        return device_error(e, "cbm_reserve (tree tail)");
+    }
+    h->tf_order = order;
+    return CBM_OK;
 }
 
 }  // namespace
@@ -734,7 +817,7 @@ extern "C" int cbm_destroy(cbm_handle *h) {
     hipError_t e = hipSuccess;
     for (void *p : {static_cast<void *>(h->ps_pos), static_cast<void *>(h->scratch), static_cast<void *>(h->comp),
                     static_cast<void *>(h->partial), static_cast<void *>(h->fused_state),
-                    static_cast<void *>(h->fused_flags)}) {
+                    static_cast<void *>(h->fused_flags), static_cast<void *>(h->tf_order)}) {
         if (!p) continue;
         const hipError_t ei = hipFree(p);
         if (e == hipSuccess) e = ei;
@@ -749,8 +832,11 @@ extern "C" int cbm_reserve(cbm_handle *h, int32_t n_cols) {
         set_error("cbm_reserve: invalid arguments");
         return CBM_ERR_ARG;
     }
+    int rc = CBM_OK;
+    if (!h->tf_order && h->d.tf_level < h->d.n_levels) rc = stage_forward_tail(h);
+    if (rc != CBM_OK) return rc;
     const int32_t cols = std::max(h->reserved_cols, n_cols);
-    int rc = grow_buffers(h, cols);
+    rc = grow_buffers(h, cols);
     if (rc == CBM_OK) h->reserved_cols = cols;
     if (rc == CBM_OK && h->plain) rc = cbm_reserve(h->plain, cols);
     return rc;
@@ -767,7 +853,7 @@ int64_t forward_launches(const cbm_handle *h) {
     int64_t n = 1 + (h->fred_ptr[size_t(L)] > 0 ? 1 : 0);
     if (L > 1) {
         const bool tail = tails_on(h) && h->d.tf_level < L;
-        n += (tail ? h->d.tf_level - 1 : L - 1) + (tail ? 1 : 0);
+        n += (tail ? h->d.tf_level - 1 : L - 1) + (tail ? tail_launches(forward_tail(h)) : 0);
     }
     return n;
 }
@@ -1288,7 +1374,7 @@ extern "C" int cbm_set_symmetric(cbm_handle *h, int32_t symmetric) {
 }
 
 extern "C" int cbm_set_tree_tails(cbm_handle *h, int32_t mode) {
-    if (!h || mode < 0 || mode > 1) {
+    if (!h || mode < 0 || mode > 3) {
         set_error("cbm_set_tree_tails: invalid arguments");
         return CBM_ERR_ARG;
     }

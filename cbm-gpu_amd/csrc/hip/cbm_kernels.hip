@@ -24,14 +24,11 @@ using namespace rp;
 
 // y[node_of[p]] += parscale[p] * y[parent_of[p]], the columns of lane l of
 // a group of LPR lanes.
+// The row's parent par (< 0: none), row and scale ps given.
 template <int VE, int LPR>
-__device__ inline void propagate_row(int32_t p, int l, const int32_t *__restrict__ node_of,
-                                     const int32_t *__restrict__ parent_of, const float *__restrict__ parscale,
-                                     float *y, int64_t ldy, int32_t ncols) {
-    const int32_t par = parent_of[p];
+__device__ inline void propagate_row_at(int32_t par, int32_t row, float ps, int l, float *y, int64_t ldy,
+                                        int32_t ncols) {
     if (par < 0) return;
-    const int32_t row = node_of[p];
-    const float ps = parscale ? parscale[p] : 1.0f;
     for (int32_t c0 = l * VE; c0 < ncols; c0 += LPR * VE) {
         float a[VE], b[VE];
         Load<float, VE>::run(y + int64_t(par) * ldy + c0, a);
@@ -40,6 +37,15 @@ __device__ inline void propagate_row(int32_t p, int l, const int32_t *__restrict
         for (int q = 0; q < VE; ++q) b[q] = fmaf(a[q], ps, b[q]);
         store_row<VE>(y + int64_t(row) * ldy + c0, b);
     }
+}
+
+template <int VE, int LPR>
+__device__ inline void propagate_row(int32_t p, int l, const int32_t *__restrict__ node_of,
+                                     const int32_t *__restrict__ parent_of, const float *__restrict__ parscale,
+                                     float *y, int64_t ldy, int32_t ncols) {
+    const int32_t par = parent_of[p];
+    if (par < 0) return;
+    propagate_row_at<VE, LPR>(par, node_of[p], parscale ? parscale[p] : 1.0f, l, y, ldy, ncols);
 }
 
 template <int VE, int LPR>
@@ -80,14 +86,11 @@ __global__ __launch_bounds__(256) void k_propagate_comp(int32_t p_begin, int32_t
 
 // The compensated form: (y, comp) of position p from the parent's y and
 // comp rows; positions from c.base on own a comp row.
+// The compensated form with position p's parent par, row, scale es and the
+// parent's position pp given.
 template <int VE, int LPR>
-__device__ inline void propagate_row_comp(int32_t p, int l, const int32_t *__restrict__ node_of,
-                                          const int32_t *__restrict__ parent_of, const float *__restrict__ parscale,
-                                          const CompArgs &c, float *y, int64_t ldy, int32_t ncols) {
-    const int32_t par = parent_of[p];
-    const int32_t row = node_of[p];
-    const float es = parscale ? parscale[p] : 1.0f;
-    const int32_t pp = c.parent_pos[p];
+__device__ inline void propagate_row_comp_at(int32_t p, int32_t par, int32_t row, float es, int32_t pp, int l,
+                                             const CompArgs &c, float *y, int64_t ldy, int32_t ncols) {
     for (int32_t c0 = l * VE; c0 < ncols; c0 += LPR * VE) {
         float comp[VE];
 #pragma unroll
@@ -109,21 +112,63 @@ __device__ inline void propagate_row_comp(int32_t p, int l, const int32_t *__res
     }
 }
 
+template <int VE, int LPR>
+__device__ inline void propagate_row_comp(int32_t p, int l, const int32_t *__restrict__ node_of,
+                                          const int32_t *__restrict__ parent_of, const float *__restrict__ parscale,
+                                          const CompArgs &c, float *y, int64_t ldy, int32_t ncols) {
+    propagate_row_comp_at<VE, LPR>(p, parent_of[p], node_of[p], parscale ? parscale[p] : 1.0f, c.parent_pos[p], l, c,
+                                   y, ldy, ncols);
+}
+
+// Rows of a tail fragment whose position, row, parent, scale and parent
+// position the unstaged tail kernel holds in LDS (larger fragments read them
+// from the plan's arrays level by level).
+constexpr int kTailMetaRows = 512;
+
 // The forward propagation of a tree tail: workgroup f finishes fragment f's
 // segments, root first, one group of LPR lanes per row and a workgroup
 // barrier between segments; positions from comp_first on take the
-// compensated form.
+// compensated form.  The rows' indices and scales are read into LDS first,
+// so a level waits only for its rows' loads and stores.
 template <int VE, int LPR>
 __global__ __launch_bounds__(256) void k_propagate_tail(const TailArgs t, const int32_t *__restrict__ node_of,
                                                         const int32_t *__restrict__ parent_of,
                                                         const float *__restrict__ parscale, const CompArgs c,
                                                         int32_t comp_first, float *y, int64_t ldy, int32_t ncols) {
     constexpr int kGroups = 256 / LPR;
+    __shared__ int32_t m_pos[kTailMetaRows], m_row[kTailMetaRows], m_par[kTailMetaRows], m_pp[kTailMetaRows];
+    __shared__ float m_es[kTailMetaRows];
     const int lane = int(threadIdx.x & 63u);
     const int l = lane & (LPR - 1);
     const int32_t group = int32_t(threadIdx.x / LPR);
-    const int32_t f = int32_t(blockIdx.x);
-    for (int32_t s = t.frag[f]; s < t.frag[f + 1]; ++s) {
+    const int32_t f = t.order ? t.order[t.n_staged + int32_t(blockIdx.x)] : int32_t(blockIdx.x);
+    const int32_t s0 = t.frag[f], s1 = t.frag[f + 1];
+    const int32_t rb = t.seg[s0], n = t.seg[s1] - rb;
+    if (n >= 0 && n <= kTailMetaRows) {
+        for (int32_t i = int32_t(threadIdx.x); i < n; i += 256) {
+            const int32_t p = t.pos[rb + i];
+            m_pos[i] = p;
+            m_row[i] = node_of[p];
+            m_par[i] = parent_of[p];
+            m_es[i] = parscale ? parscale[p] : 1.0f;
+            m_pp[i] = p >= comp_first ? c.parent_pos[p] : -1;
+        }
+        __syncthreads();
+        for (int32_t s = s0; s < s1; ++s) {
+            const int32_t ie = t.seg[s + 1] - rb;
+            for (int32_t i = t.seg[s] - rb + group; i < ie; i += kGroups) {
+                const int32_t p = m_pos[i];
+                if (p >= comp_first) {
+                    propagate_row_comp_at<VE, LPR>(p, m_par[i], m_row[i], m_es[i], m_pp[i], l, c, y, ldy, ncols);
+                } else {
+                    propagate_row_at<VE, LPR>(m_par[i], m_row[i], m_es[i], l, y, ldy, ncols);
+                }
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    for (int32_t s = s0; s < s1; ++s) {
         for (int32_t i = t.seg[s] + group; i < t.seg[s + 1]; i += kGroups) {
             const int32_t p = t.pos[i];
             if (p >= comp_first) {
@@ -133,6 +178,142 @@ __global__ __launch_bounds__(256) void k_propagate_tail(const TailArgs t, const 
             }
         }
         __syncthreads();
+    }
+}
+
+// The same propagation for the staged fragments, whose chains are long: a
+// workgroup takes fragment t.order[blockIdx.x / n_slabs] and kTailSlab columns
+// of it.  The fragment's rows (Delta-pass values), positions, parent slots
+// and scales are loaded into LDS at once -- a row finds its parent's slot by
+// a binary search, since the fragment's positions ascend (segments are tree
+// levels, positions are in level order) -- so the levels then run from LDS
+// alone, with a workgroup barrier between them, and the rows go out at the
+// end.  A parent outside the fragment (the root's, finished by the launches
+// before the tail) is read from y and comp.  Every value goes through the
+// arithmetic of propagate_row / propagate_row_comp, so the bits are those of
+// the unstaged tail.
+template <int VE>
+__global__ __launch_bounds__(256) void k_propagate_tail_staged(const TailArgs t, const int32_t *__restrict__ node_of,
+                                                               const int32_t *__restrict__ parent_of,
+                                                               const float *__restrict__ parscale, const CompArgs c,
+                                                               int32_t comp_first, float *y, int64_t ldy,
+                                                               int32_t ncols, int32_t n_slabs) {
+    constexpr int LPR = kTailSlab / VE;
+    constexpr int kGroups = 256 / LPR;
+    constexpr int32_t kNoParent = -2, kOutside = -1;
+    extern __shared__ int4 tail_lds[];
+    const int32_t cap = t.staged_rows;
+    // per row: x position, y parent's slot (kOutside, kNoParent), z parent's
+    // position, w the scale's bits; then the rows, then their compensation rows
+    int4 *s_meta = tail_lds;
+    float *s_y = reinterpret_cast<float *>(s_meta + cap);
+    float *s_c = s_y + int64_t(cap) * kTailSlab;  // (comp_first < INT32_MAX)
+    const int32_t f = t.order[int32_t(blockIdx.x) / n_slabs];
+    const int32_t slab = int32_t(blockIdx.x) % n_slabs;
+    const int l = int(threadIdx.x) % LPR, group = int(threadIdx.x) / LPR;
+    const int lc = l * VE;
+    const int32_t c0 = slab * kTailSlab + lc;
+    const bool active = c0 < ncols;
+    const int32_t s0 = t.frag[f], s1 = t.frag[f + 1];
+    const int32_t rb = t.seg[s0], n = t.seg[s1] - rb;
+    if (n > cap || n < 0) return;  // the host stages only fragments that fit
+
+    for (int32_t i = group; i < n; i += kGroups) {
+        const int32_t p = t.pos[rb + i];
+        if (l == 0) {
+            const bool has = parent_of[p] >= 0;
+            s_meta[i] = make_int4(p, has ? kOutside : kNoParent, has ? c.parent_pos[p] : -1,
+                                  __float_as_int(parscale ? parscale[p] : 1.0f));
+        }
+        float v[VE];
+        if (active) {
+            Load<float, VE>::run(y + int64_t(node_of[p]) * ldy + c0, v);
+        } else {
+#pragma unroll
+            for (int q = 0; q < VE; ++q) v[q] = 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < VE; ++q) s_y[i * kTailSlab + lc + q] = v[q];
+    }
+    __syncthreads();
+    for (int32_t i = int32_t(threadIdx.x); i < n; i += 256) {
+        const int32_t pp = s_meta[i].z;
+        if (s_meta[i].y == kNoParent) continue;
+        int32_t lo = 0, hi = i;  // a parent in the fragment precedes its child
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (s_meta[mid].x < pp) {
+                lo = mid + 1;
+            } else {
+                hi = mid;
+            }
+        }
+        if (lo < i && s_meta[lo].x == pp) s_meta[i].y = lo;
+    }
+    __syncthreads();
+
+    for (int32_t s = s0; s < s1; ++s) {
+        const int32_t ie = t.seg[s + 1] - rb;
+        for (int32_t i = t.seg[s] - rb + group; i < ie; i += kGroups) {
+            const int4 m = s_meta[i];
+            const int32_t p = m.x, ps = m.y, pp = m.z;
+            const bool comp_row = p >= comp_first;
+            float *yi = s_y + i * kTailSlab + lc;
+            if (ps == kNoParent) {
+                if (comp_row)
+#pragma unroll
+                    for (int q = 0; q < VE; ++q) s_c[i * kTailSlab + lc + q] = 0.0f;
+                continue;
+            }
+            const float es = __int_as_float(m.w);
+            const bool parent_comp = comp_row && pp >= c.base;  // the parent has a compensation row
+            float acc[VE], v[VE], cp[VE];
+#pragma unroll
+            for (int q = 0; q < VE; ++q) {
+                acc[q] = yi[q];
+                cp[q] = 0.0f;
+            }
+            if (ps >= 0) {
+#pragma unroll
+                for (int q = 0; q < VE; ++q) v[q] = s_y[ps * kTailSlab + lc + q];
+                if (parent_comp)
+#pragma unroll
+                    for (int q = 0; q < VE; ++q) cp[q] = s_c[ps * kTailSlab + lc + q];
+            } else if (active) {
+                Load<float, VE>::run(y + int64_t(parent_of[p]) * ldy + c0, v);
+                if (parent_comp) Load<float, VE>::run(c.comp + int64_t(pp - c.base) * c.ldc + c0, cp);
+            } else {
+#pragma unroll
+                for (int q = 0; q < VE; ++q) v[q] = 0.0f;
+            }
+            if (comp_row) {
+                float cm[VE];
+                comp_add<VE>(acc, es, v, cp, cm);
+#pragma unroll
+                for (int q = 0; q < VE; ++q) s_c[i * kTailSlab + lc + q] = cm[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], es, acc[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < VE; ++q) yi[q] = acc[q];
+        }
+        __syncthreads();
+    }
+
+    for (int32_t i = group; active && i < n; i += kGroups) {
+        const int4 m = s_meta[i];
+        float v[VE];
+        if (m.y != kNoParent) {
+#pragma unroll
+            for (int q = 0; q < VE; ++q) v[q] = s_y[i * kTailSlab + lc + q];
+            store_row<VE>(y + int64_t(node_of[m.x]) * ldy + c0, v);
+        }
+        if (m.x >= comp_first) {
+#pragma unroll
+            for (int q = 0; q < VE; ++q) v[q] = s_c[i * kTailSlab + lc + q];
+            store_row<VE>(c.comp + int64_t(m.x - c.base) * c.ldc + c0, v);
+        }
     }
 }
 
@@ -245,12 +426,31 @@ hipError_t launch_propagate_tail(const TailArgs &t, const int32_t *node_of, cons
     const bool comp = comp_first < INT32_MAX;
     const bool vec = (ncols % 4) == 0 && aligned16(y, ldy, 4) && (!comp || aligned16(c.comp, c.ldc, 4));
     const int ve = vec ? 4 : 1;
+    const int32_t n_staged = t.order ? t.n_staged : 0;
+    if (n_staged < 0 || n_staged > t.n_frag) return hipErrorInvalidValue;
+    if (n_staged > 0) {
+        const int64_t lds = tail_staged_bytes(t.staged_rows, comp);
+        const int32_t n_slabs = (ncols + kTailSlab - 1) / kTailSlab;
+        if (t.staged_rows <= 0 || lds > kTailStagedMaxBytes || int64_t(n_staged) * n_slabs > int64_t(INT32_MAX))
+            return hipErrorInvalidValue;
+        const dim3 grid(static_cast<unsigned>(int64_t(n_staged) * n_slabs)), block(256);
+        if (vec) {
+            k_propagate_tail_staged<4><<<grid, block, size_t(lds), stream>>>(t, node_of, parent_of, parscale, c,
+                                                                             comp_first, y, ldy, ncols, n_slabs);
+        } else {
+            k_propagate_tail_staged<1><<<grid, block, size_t(lds), stream>>>(t, node_of, parent_of, parscale, c,
+                                                                             comp_first, y, ldy, ncols, n_slabs);
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess || n_staged == t.n_frag) return e;
+    }
+    const unsigned rest = unsigned(t.n_frag - n_staged);
     const int lpr = lanes_per_row(ncols, ve);
-#define CBM_TAIL_CASE(V, L)                                                                                     \
-    if (ve == V && lpr == L) {                                                                                  \
-        k_propagate_tail<V, L><<<dim3(unsigned(t.n_frag)), dim3(256), 0, stream>>>(t, node_of, parent_of, parscale, \
-                                                                                  c, comp_first, y, ldy, ncols);  \
-        return hipGetLastError();                                                                               \
+#define CBM_TAIL_CASE(V, L)                                                                                          \
+    if (ve == V && lpr == L) {                                                                                       \
+        k_propagate_tail<V, L><<<dim3(rest), dim3(256), 0, stream>>>(t, node_of, parent_of, parscale, c, comp_first, \
+                                                                     y, ldy, ncols);                                 \
+        return hipGetLastError();                                                                                    \
     }
     CBM_TAIL_CASE(4, 1) CBM_TAIL_CASE(4, 2) CBM_TAIL_CASE(4, 4) CBM_TAIL_CASE(4, 8) CBM_TAIL_CASE(4, 16)
     CBM_TAIL_CASE(4, 32) CBM_TAIL_CASE(4, 64)

@@ -166,11 +166,29 @@ struct TailArgs {
     const int32_t *seg = nullptr;
     const int32_t *pos = nullptr;
     int32_t n_frag = 0;
+    // Optional order of the fragments: order[0 .. n_staged) are finished with
+    // their rows staged in LDS (at most staged_rows rows each, kTailSlab
+    // columns per workgroup), order[n_staged .. n_frag) as above.  Without an
+    // order, workgroup f takes fragment f.
+    const int32_t *order = nullptr;
+    int32_t n_staged = 0;
+    int32_t staged_rows = 0;
 };
+
+// Columns of one workgroup of a staged tail, and the LDS such a workgroup may
+// hold: a fragment is staged only if its rows fit (tail_staged_bytes).
+constexpr int kTailSlab = 16;
+constexpr int64_t kTailStagedMaxBytes = 65536;
+inline int64_t tail_staged_bytes(int32_t rows, bool comp) {
+    // position, parent slot, parent position and scale, then the row (and
+    // its compensation row)
+    return int64_t(rows) * (4 * int64_t(sizeof(int32_t)) + int64_t(kTailSlab) * int64_t(sizeof(float)) * (comp ? 2 : 1));
+}
 
 // The forward propagation of a tail, root segments first: each position as in
 // launch_propagate, or as in launch_propagate_comp from position comp_first
-// on (the same arithmetic, so the same bits as the per-level launches).
+// on (the same arithmetic, so the same bits as the per-level launches).  The
+// staged fragments (t.order) run first, in a launch of their own.
 hipError_t launch_propagate_tail(const TailArgs &t, const int32_t *node_of, const int32_t *parent_of,
                                  const float *parscale, const CompArgs &c, int32_t comp_first, float *y, int64_t ldy,
                                  int32_t ncols, hipStream_t stream);

@@ -218,8 +218,11 @@ int cbm_destroy(cbm_handle *h);
  * cbm_matmul_t, cbm_delta_spmm, cbm_propagate, cbm_adjoint_*) never allocate
  * and fail with CBM_ERR_ARG for more columns than reserved.  The settings
  * that change what a width needs (cbm_set_kahan_depth, cbm_set_xcd_ranges)
- * grow the buffers for the reserved width themselves.  The reference
- * allocates a cuSPARSE workspace on every call instead (cbm/cbm.cu:181,203).
+ * grow the buffers for the reserved width themselves.  The first reservation
+ * also orders the forward tail's fragments for staging in LDS (it reads the
+ * tail tables back, waiting for the device; see cbm_set_tree_tails).  The
+ * reference allocates a cuSPARSE workspace on every call instead
+ * (cbm/cbm.cu:181,203).
  */
 int cbm_reserve(cbm_handle *h, int32_t n_cols);
 
@@ -441,15 +444,18 @@ int cbm_attach_uncompressed(cbm_handle *h, cbm_handle *plain, int32_t mode);
 int cbm_uncompressed_active(cbm_handle *h, int32_t dtype, int32_t n_cols);
 
 /*
- * Tree tails (1 on, the default; 0 off).  The deepest tree levels of the
- * forward propagation and of the adjoint's reverse pass, each of which would
- * be a short dependent launch, are finished in one launch: one workgroup per
- * fragment (a node of the tail's first level with its descendants) works
- * through the fragment's levels with a workgroup barrier between them.  The
- * plan chooses each tail's first level by a cost model (cbm_plan_create).
- * The forward gives the same bits either way; the reverse pass sums each row
- * in entry order by one group of lanes, which can round differently from the
- * per-level mapping.
+ * Tree tails (1 on, the default; 0 off; 2 and 3 on with other staging).  The
+ * deepest tree levels of the forward propagation and of the adjoint's reverse
+ * pass, each of which would be a short dependent launch, are finished in one
+ * launch: one workgroup per fragment (a node of the tail's first level with
+ * its descendants) works through the fragment's levels with a workgroup
+ * barrier between them.  The plan chooses each tail's first level by a cost
+ * model (cbm_plan_create).  In the forward tail, fragments of at least 8
+ * levels whose rows fit 64 KiB of LDS in 16-column slabs run first, in a
+ * launch of their own, with their rows staged in LDS (mode 1); mode 2 stages
+ * none, mode 3 every fragment that fits.  The forward gives the same bits in
+ * every mode; the reverse pass sums each row in entry order by one group of
+ * lanes, which can round differently from the per-level mapping.
  */
 int cbm_set_tree_tails(cbm_handle *h, int32_t mode);
 

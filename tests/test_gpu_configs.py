@@ -202,6 +202,28 @@ def test_c4_ad_full_graph(c4, f, form):
         h.set_symmetric(True)
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_c4_tail_staging_same_bits(c4, dtype):
+    """C4's forward tail (fragments of up to 41 levels and 372 rows, Kahan
+    levels inside it) staged in LDS for its deep fragments (the default), for
+    every fragment that fits, and for none, and per level: one set of bits."""
+    wl, mat = c4
+    h = mat.cusparse_extension
+    f = 256
+    x = torch.from_numpy(_randn(wl.n, f, 77)).to(wl.cuda).to(dtype)
+    out = {}
+    try:
+        for mode in (True, "all_staged", "unstaged", False):
+            h.set_tree_tails(mode)
+            y = torch.full((wl.n, f), float("nan"), device=x.device)
+            h.forward(x, y)
+            out[mode] = y
+    finally:
+        h.set_tree_tails(True)
+    for mode in (True, "all_staged", "unstaged"):
+        assert torch.equal(out[mode], out[False]), mode
+
+
 def test_c4_integer_exact(c4):
     """The plain product A over C4's structure (rows of up to thousands of
     entries, depth 52, with XCD ranges at FP32 F = 64): integer X exact."""

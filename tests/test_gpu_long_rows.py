@@ -455,9 +455,10 @@ def test_xcd_ranges_with_column_slabs(hub, cuda, dtype):
 def test_tree_tails(hub, cuda, dtype, f):
     """The deepest levels in one launch (one workgroup per fragment): the
     forward gives the per-level launches' bits, plain and compensated (Kahan
-    from level 2, inside the tail); the reverse pass (each row summed in
-    entry order) stays within FP32 rounding of them and of FP64, and
-    integer-valued inputs are exact."""
+    from level 2, inside the tail), with the deep fragments staged in LDS
+    (the default), none staged, or every one staged; the reverse pass (each
+    row summed in entry order) stays within FP32 rounding of them and of
+    FP64, and integer-valued inputs are exact."""
     a, mats = hub
     for mode, m in mats.items():
         h = m.cusparse_extension
@@ -467,7 +468,7 @@ def test_tree_tails(hub, cuda, dtype, f):
         for depth in (None, 2):
             h.set_kahan_depth(depth)
             res = {}
-            for on in (True, False):
+            for on in (True, "unstaged", "all_staged", False):
                 h.set_tree_tails(on)
                 y = torch.full((HUB_N, f), float("nan"), device=cuda)
                 dx = torch.full((HUB_N, f), float("nan"), device=cuda)
@@ -475,8 +476,15 @@ def test_tree_tails(hub, cuda, dtype, f):
                 launches = h.last_launches()
                 h.matmul_t(xt, dx)
                 res[on] = (y, dx, launches)
-            assert torch.equal(res[True][0], res[False][0]), (mode, depth)
-            assert res[True][2] < res[False][2]
+            for on in (True, "unstaged", "all_staged"):
+                assert torch.equal(res[on][0], res[False][0]), (mode, depth, on)
+                assert res[on][2] < res[False][2]
+            frag, seg = h._dev["tf_frag"].cpu().numpy(), h._dev["tf_seg"].cpu().numpy()
+            rows = seg[frag[1:]] - seg[frag[:-1]]
+            fit = int((rows <= 452).sum())  # 452 rows of 16 columns (+ compensation, 16 bytes each) fit 64 KiB
+            assert 0 < fit
+            # the staged fragments take a launch of their own, the others one more
+            assert res["all_staged"][2] == res["unstaged"][2] + (1 if fit < len(rows) else 0)
             assert O.max_rel_error(res[True][1].cpu().numpy(), res[False][1].cpu().numpy()) <= 1e-6, (mode, depth)
             assert O.max_rel_error(res[True][0].cpu().numpy(), O.forward(a, xe, mode)) <= 1e-5, (mode, depth)
             assert O.max_rel_error(res[True][1].cpu().numpy(), O.adjoint(a, xe, mode)) <= 1e-5, (mode, depth)
