@@ -23,7 +23,22 @@ def _require_cuda(t):
         raise RuntimeError("tensor is not a CUDA tensor")
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def current_stream_handle(device=None):
+    """torch's current stream on ``device`` as a HIP stream handle.  Where
+    torch offers the raw handle it is read directly (about 0.1 us instead of
+    2 us for building a Stream object: the product calls are a few
+    microseconds each on small graphs)."""
+    if _raw_stream is not None:
+        if isinstance(device, torch.device) and device.index is not None:
+            index = device.index
+        elif isinstance(device, int):
+            index = device
+        else:
+            index = torch.cuda.current_device()
+        return ctypes.c_void_p(_raw_stream(index))
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
@@ -206,7 +221,7 @@ class cusparseCBM:
         _require_cuda(x)
         if x.dim() != 2 or int(x.size(0)) != self.num_rows:
             raise ValueError(f"operand must be {self.num_rows} x F, got {tuple(x.shape)}")
-        if x.device != self.device:
+        if x.get_device() != self.device.index:
             raise ValueError(f"operand is on {x.device}, the matrix on {self.device}")
         code = _dtype_code(x)
         if x.stride(1) != 1 or x.stride(0) < x.size(1):
@@ -219,7 +234,7 @@ class cusparseCBM:
             raise TypeError(f'"{name}" is not a tensor of type "torch.float32"')
         if y.dim() != 2 or tuple(y.shape) != (self.num_rows, n_cols):
             raise ValueError(f"{name} must be {self.num_rows} x {n_cols}, got {tuple(y.shape)}")
-        if y.device != self.device:
+        if y.get_device() != self.device.index:
             raise ValueError(f"{name} is on {y.device}, the matrix on {self.device}")
         if y.stride(1) != 1 or y.stride(0) < n_cols:
             raise ValueError(f"{name} must have unit column stride")
