@@ -47,9 +47,10 @@ ROCSPARSE_ALGS = {"default": 0, "csr_row_split": 4, "csr_nnz_split": 5, "csr_mer
 
 
 def measured_traffic(workload, phase):
-    """HBM bytes per call of a phase from the committed counter summary
-    (profiles/hbm_traffic.json, written from rocprofv3 FETCH_SIZE and
-    WRITE_SIZE runs scaled by a copy of known size), or None."""
+    """Memory bytes per call of a phase from the committed counter summary
+    (profiles/hbm_traffic.json, written from rocprofv3 runs of the 32-byte
+    DRAM read-request counter and WRITE_SIZE, each checked against a copy of
+    known size): bytes that leave the L2s, or None."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(path) as fh:
@@ -764,7 +765,8 @@ def main():
                      "traffic_source": (f"committed {traffic.get('source')}" if traffic else None),
                      "kernel": (f"{dom} (BF16): " + ("k_fused_forward (one launch: Delta pieces + LDS propagation)"
                                                      if dom == "forward" else
-                                                     "k_row_mixed (long-row chunks + short rows) + k_row_reduce")),
+                                                     "the Delta pass (k_row_mixed: long-row chunks + short rows, or "
+                                                     "k_row_pass over XCD range pieces) + k_row_reduce")),
                      "algorithmic_bytes": rec["bytes_phase_bf16"][dom],
                      "ms": rec["ms_phase_bf16"][dom],
                      "gather_bytes": rec["bytes_phase_bf16"]["gather_delta"],
