@@ -440,8 +440,12 @@ bool tails_on(const cbm_handle *h) { return h->tree_tails != 0; }
 
 // Forward tail fragments of at least this many levels are staged in LDS by
 // default (their chains of dependent levels are the tail's critical path;
-// the short ones cost one or two global round trips either way).
+// the short ones cost one or two global round trips either way), provided
+// the deepest of them has kTailStagedMinChain levels: the staged launch is
+// one launch more, which a chain of 41 levels repays (N = 132,534: 66 ->
+// 52 us per tail) and one of 19 does not (N = 12,008: 3.5 us lost).
 constexpr int32_t kTailStagedMinSegs = 8;
+constexpr int32_t kTailStagedMinChain = 24;
 
 // The forward tail of the handle's tail mode: which fragments are staged.
 cbmk::TailArgs forward_tail(const cbm_handle *h) {
@@ -632,9 +636,10 @@ int stage_forward_tail(cbm_handle *h) {
     h->tf_staged_all = int32_t(fit.size());
     h->tf_staged_deep = 0;
     h->tf_rows_all = h->tf_rows_deep = 0;
+    const bool chain = !fit.empty() && levels(fit.front()) >= kTailStagedMinChain;
     for (const int32_t f : fit) {
         h->tf_rows_all = std::max(h->tf_rows_all, round4(rows_of(f)));
-        if (levels(f) >= kTailStagedMinSegs) {
+        if (chain && levels(f) >= kTailStagedMinSegs) {
             ++h->tf_staged_deep;
             h->tf_rows_deep = std::max(h->tf_rows_deep, round4(rows_of(f)));
         }
@@ -860,7 +865,7 @@ int64_t forward_launches(const cbm_handle *h) {
 
 // The cost model that picks between the compressed product and the same
 // product over A (cbm_attach_uncompressed): t = launches x kLaunchUs + the
-// bytes the gathers and the output move / kGatherBytesPerUs.  A launch costs
+// bytes the gathers and the output move / the gather rate.  A launch costs
 // about 8 us on MI355X whatever its size (a 2,708-row row pass and a 2,708-
 // row propagation level alike), and random row gathers run at about 7 TB/s
 // (the gather ceiling of the Delta pass).  The tree's launches are worth it
@@ -868,6 +873,13 @@ int64_t forward_launches(const cbm_handle *h) {
 // these constants N = 2,708 (depth 9, nnz 11k -> 7k) and N = 19,717 (depth
 // 12, nnz 92k -> 55k) run over A, while N = 132,534 (nnz 80M -> 41M) and
 // N = 2,449,029 (nnz 122M -> 117M) run compressed.
+// One rate for all gathers is a simplification: gathers whose rows the L2s
+// hold run faster (12-18 TB/s on matrices of 8-68 MB), but so do the small
+// levels' launches (4-5 us).  A rate that follows the L2 fit alone measured
+// worse on graphs of N = 16,000 (depth 31, F = 256 BF16: it chose A, 0.533
+// ms per step, where the compressed product takes 0.459) than this constant
+// pair, whose one miss among eight measured cases (N = 12,008, F = 512 BF16:
+// compressed 0.208 against 0.193 ms) costs 8%.
 constexpr double kLaunchUs = 8.0;
 constexpr double kGatherBytesPerUs = 7.0e6;
 

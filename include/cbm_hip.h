@@ -452,8 +452,9 @@ int cbm_uncompressed_active(cbm_handle *h, int32_t dtype, int32_t n_cols);
  * barrier between them.  The plan chooses each tail's first level by a cost
  * model (cbm_plan_create).  In the forward tail, fragments of at least 8
  * levels whose rows fit 64 KiB of LDS in 16-column slabs run first, in a
- * launch of their own, with their rows staged in LDS (mode 1); mode 2 stages
- * none, mode 3 every fragment that fits.  The forward gives the same bits in
+ * launch of their own, with their rows staged in LDS (mode 1), when the
+ * deepest of them has 24 levels or more; mode 2 stages none, mode 3 every
+ * fragment that fits.  The forward gives the same bits in
  * every mode; the reverse pass sums each row in entry order by one group of
  * lanes, which can round differently from the per-level mapping.
  */
