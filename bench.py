@@ -280,7 +280,13 @@ def measure(name, w, ei, m, args, dev, stream):
         forms["delta_t_adjoint"] = {"bf16": round(time_loop(step(x16, g16), K, W, stream), 4),
                                     "fp32": round(time_loop(step(x32, g32), K, W, stream), 4)}
         h.set_symmetric(True)
-    log(f"{name}: over A {over_a}; one launch {fused}; other forms {forms}")
+    ordered = {"bf16": h.order_active(torch.bfloat16, f), "fp32": h.order_active(torch.float32, f)}
+    if getattr(h, "_order", None) is not None:
+        h.set_order(False)
+        forms["position_order"] = {"bf16": round(time_loop(step(x16, g16), K, W, stream), 4),
+                                   "fp32": round(time_loop(step(x32, g32), K, W, stream), 4)}
+        h.set_order(None)
+    log(f"{name}: over A {over_a}; one launch {fused}; gather order {ordered}; other forms {forms}")
     ph = {
         "forward": time_loop(lambda: h.forward(x16, y), K, W, stream),
         "forward_delta_spmm": time_loop(lambda: h.delta_spmm(x16, y), K, W, stream),
@@ -403,6 +409,7 @@ def measure(name, w, ei, m, args, dev, stream):
         "dominant_phase": dom, "dominant_gbs": round(dom_gbs, 1),
         "gather_gbs_dominant": round(b16["gather_delta"] / (ph[dom] * 1e-3) / 1e9, 1),
         "symmetric_adjoint": h.symmetric, "one_launch_forward": fused, "runs_over_a": over_a, "launches": launches,
+        "gather_order": ordered,
         "ms_step_other_forms": forms,
         "max_rel_error_bf16_vs_fp32_cbm": vs_fp32,
         "ms_xcd_slabs_bf16": xcd,

@@ -87,6 +87,7 @@ def host():
             _sig(lib, "cbm_plan_depth", c_i32, c_vp)
             _sig(lib, "cbm_plan_stat", c_i64, c_vp, c_i32)
             _sig(lib, "cbm_plan_free", None, c_vp)
+            _sig(lib, "cbm_gather_order", c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, ctypes.POINTER(c_i32))
             _host = lib
     return _host
 

@@ -16,6 +16,14 @@ UNCOMPRESSED_MAX_ROWS = 1 << 18
 UNCOMPRESSED_MAX_NNZ = 1 << 24
 
 
+# The gather order of the Delta pass is computed for matrices of more than
+# this many rows when the constructor's `gather_order` is None: there the
+# gathered matrix is far larger than the L2s, so every gather an L2 serves is
+# a fabric request saved (the partner scan and the forest walk cost about a
+# second per million rows, once).
+ORDER_MIN_ROWS = 1 << 19
+
+
 def _wants_uncompressed(num_rows, edge_index):
     return num_rows <= UNCOMPRESSED_MAX_ROWS and int(edge_index.shape[1]) <= UNCOMPRESSED_MAX_NNZ
 
@@ -42,11 +50,14 @@ class cbm4mm:
     the CBM form and the handle runs it where a cost model predicts it
     faster (small graphs, whose tree levels are dependent launches);
     False keeps the CBM form only.  Loaded structures have no A, so they
-    run the CBM form.
+    run the CBM form.  ``gather_order``: the Delta pass sums the short rows
+    in an order that puts rows sharing columns together, each XCD taking one
+    stretch of it (same results, more gathers served by the L2s): None for
+    matrices of more than ORDER_MIN_ROWS rows, True always, False never.
     """
 
     def __init__(self, edge_index, edge_values, alpha=0, *, num_nodes=None, device="cuda", n_threads=0,
-                 candidates_on="device", uncompressed=None):
+                 candidates_on="device", uncompressed=None, gather_order=None):
         num_rows = _num_rows(edge_index, num_nodes)
         # The candidate-edge scan of the build runs on the target GPU
         # ("device"), or on the host (None); the structure is the same.
@@ -58,6 +69,7 @@ class cbm4mm:
         if uncompressed is None:
             uncompressed = UNCOMPRESSED
         self._plain_edges = edge_index if uncompressed and _wants_uncompressed(num_rows, edge_index) else None
+        self._order_wanted = num_rows > ORDER_MIN_ROWS if gather_order is None else bool(gather_order)
         self._assemble(num_rows, built.csr_ptr, built.csr_col, built.csr_val, built.branches, built.tree_src,
                        built.tree_dst, built.stats, device, n_threads)
         self._plain_edges = None
@@ -78,6 +90,11 @@ class cbm4mm:
         # A = A^T (the build compares them exactly): the adjoint then runs
         # through the forward structure.
         self.cusparse_extension.set_symmetric(bool(stats.get("symmetric", 0)))
+        order = getattr(self, "_stored_order", None)
+        if order is not None or getattr(self, "_order_wanted", False):
+            if torch.device(device).type != "cuda":
+                raise RuntimeError("tensor is not a CUDA tensor")
+            self.cusparse_extension.attach_order(order)
         edges = getattr(self, "_plain_edges", None)
         if edges is not None:
             # The same product over A itself, which the handle runs where the

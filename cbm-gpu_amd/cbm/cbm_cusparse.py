@@ -163,6 +163,7 @@ class cusparseCBM:
         self._h = h
         if self._symmetric:
             _lib.check_hip(self._lib.cbm_set_symmetric(self._h, 1), "set_symmetric")
+        self._bind_order()
 
     def with_scaling(self, mode, scale=None):
         """A handle for another product (A, A D or D A D) over the same device
@@ -171,6 +172,9 @@ class cusparseCBM:
         for k in ("device", "num_rows", "depth", "nnz", "n_edges", "level_sizes", "plan_stats", "_host", "_dev",
                   "_lib", "_symmetric", "_plain_mode"):
             setattr(other, k, getattr(self, k))
+        for k in ("_order", "_order_dev", "_order_mode"):
+            if hasattr(self, k):
+                setattr(other, k, getattr(self, k))
         if mode != MODE_AX and scale is None:
             raise ValueError(f"mode {mode} needs the D^-1/2 vector as `scale`")
         other._plain = None
@@ -193,6 +197,47 @@ class cusparseCBM:
         if plain is not None:
             plain._symmetric = self._symmetric
             plain._reserved = max(plain._reserved, self._reserved)
+
+    def attach_order(self, order=None, mode=None):
+        """Runs the Delta pass over the short rows in a gather order
+        (``cbm_attach_order``): ``order`` the positions of this layout's short
+        rows (e.g. from a structure file), or None to compute it (the partner
+        scan on this GPU, then the forest walk on the host).  ``mode``: None
+        (automatic), True, False.  Returns the order (int32, host)."""
+        short = self._dev["fwd_ptr"].diff() <= structure.LONG_ROW
+        if order is None:
+            order = structure.device_gather_order(self._dev["fwd_ptr"], self._dev["fwd_col"], self.device)
+        else:
+            order = order.to("cpu", torch.int32).contiguous()
+            n_short = int(short.sum())
+            if (order.numel() != n_short or (n_short and (int(order.min()) < 0 or int(order.max()) >= self.num_rows))
+                    or not bool(short.cpu()[order.long()].all()) or int(torch.unique(order).numel()) != n_short):
+                raise ValueError("the given order is not an order of this matrix's short rows")
+        self._order = order
+        self._order_dev = (order if order.numel() else torch.zeros(1, dtype=torch.int32)).to(self.device)
+        self._order_mode = -1 if mode is None else int(bool(mode))
+        self._bind_order()
+        return order
+
+    def _bind_order(self):
+        order = getattr(self, "_order", None)
+        if order is None:
+            return
+        _lib.check_hip(self._lib.cbm_attach_order(self._h, int(order.numel()), self._order_dev.data_ptr(),
+                                                  self._order_mode), "attach_order")
+
+    def set_order(self, mode):
+        """Switches the gather order of the Delta pass: True, False or None
+        (automatic)."""
+        self._order_mode = -1 if mode is None else int(bool(mode))
+        _lib.check_hip(self._lib.cbm_set_order(self._h, self._order_mode), "set_order")
+
+    def order_active(self, dtype, n_cols):
+        """Whether a forward product with dtype storage and n_cols columns
+        runs its Delta pass in the gather order now."""
+        code = self._lib.cbm_order_active(self._h, BF16 if dtype == torch.bfloat16 else F32, int(n_cols))
+        _lib.check_hip(min(code, 0), "order_active")
+        return bool(code)
 
     def uncompressed_active(self, dtype, n_cols):
         """Whether a product with dtype storage and n_cols columns runs over

@@ -288,3 +288,78 @@ def plan(n_rows, crow, col, val, tree_src, tree_dst, n_threads=0):
         return Plan(n=n_rows, depth=int(lib.cbm_plan_depth(p)), arrays=arrays, stats=stats)
     finally:
         lib.cbm_plan_free(p)
+
+
+# Gather order of the Delta pass (cbm_partners, cbm_gather_order).
+LONG_ROW = 256        # CBM_LONG_ROW
+# Columns held by more rows than this are not counted by the partner scan
+# (each adds one to all of them and separates little, at a quadratic cost).
+PARTNER_CAP = 256
+
+
+def device_partners(fwd_ptr, fwd_col, device, cap=PARTNER_CAP, n_workgroups=0):
+    """The two best-overlapping partner rows of every short row of a forward
+    row list (``cbm_partners``, HIP): ``(partner, overlap)``, int32 ``[n, 2]``
+    on ``device``; -1 / 0 where there is none.  Long rows are left out of the
+    column lists, so they are nobody's partner."""
+    dev = torch.device(device)
+    if dev.type != "cuda":
+        raise RuntimeError("tensor is not a CUDA tensor")
+    hip = _lib.hip()
+    with torch.cuda.device(dev):
+        ptr = fwd_ptr.to(dev, torch.int32).contiguous()
+        n = int(ptr.numel()) - 1
+        col = fwd_col.to(dev, torch.int32).contiguous()
+        lens = ptr.diff()
+        rows = torch.repeat_interleave(torch.arange(n, dtype=torch.int32, device=dev), lens.long())
+        keep = (lens <= LONG_ROW)[rows.long()]
+        cols = (col[keep] & 0x7fffffff).long()
+        order = torch.argsort(cols, stable=True)
+        t_col = rows[keep][order].contiguous()
+        t_ptr = torch.zeros(n + 1, dtype=torch.int32, device=dev)
+        t_ptr[1:] = torch.cumsum(torch.bincount(cols, minlength=n), 0).to(torch.int32)
+        del rows, keep, cols, order
+        if col.numel() == 0:
+            col = torch.zeros(1, dtype=torch.int32, device=dev)
+        if t_col.numel() == 0:
+            t_col = torch.zeros(1, dtype=torch.int32, device=dev)
+        ws_bytes = int(hip.cbm_partners_workspace_bytes(n, int(n_workgroups)))
+        if ws_bytes < 0:
+            raise ValueError(f"no partner workspace for {n} rows")
+        ws = torch.empty(ws_bytes + 256, dtype=torch.uint8, device=dev)
+        ws_ptr = (ws.data_ptr() + 255) // 256 * 256
+        partner = torch.empty((n, 2), dtype=torch.int32, device=dev)
+        overlap = torch.empty((n, 2), dtype=torch.int32, device=dev)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        _lib.check_candidates(hip.cbm_partners(n, ptr.data_ptr(), col.data_ptr(), n, t_ptr.data_ptr(),
+                                               t_col.data_ptr(), int(cap), int(n_workgroups), partner.data_ptr(),
+                                               overlap.data_ptr(), ws_ptr, ws_bytes, stream), "partners")
+        torch.cuda.current_stream(dev).synchronize()
+        del ws
+        return partner, overlap
+
+
+def gather_order(fwd_ptr, partner, overlap):
+    """The short rows of a forward row list in the gather order
+    (``cbm_gather_order``): int32 positions, rows that share columns next to
+    one another, from each row's partner rows (``[n, k]`` int32, -1 for none)."""
+    ptr = _cpu_contig(fwd_ptr, torch.int32, "fwd_ptr")
+    n = int(ptr.numel()) - 1
+    partner = _cpu_contig(partner, torch.int32, "partner").reshape(n, -1)
+    overlap = _cpu_contig(overlap, torch.int32, "overlap").reshape(n, -1)
+    if partner.shape != overlap.shape:
+        raise ValueError("partner and overlap differ in shape")
+    out = torch.empty(max(n, 1), dtype=torch.int32)
+    k = ctypes.c_int32()
+    lib = _lib.host()
+    rc = lib.cbm_gather_order(n, ptr.data_ptr(), partner.data_ptr(), overlap.data_ptr(), int(partner.shape[1]),
+                              out.data_ptr(), ctypes.byref(k))
+    _lib.check_host(rc, "gather order")
+    return out[:k.value].clone()
+
+
+def device_gather_order(fwd_ptr, fwd_col, device):
+    """The gather order of a forward row list held on ``device``: the partner
+    scan there, the forest walk on the host."""
+    partner, overlap = device_partners(fwd_ptr, fwd_col, device)
+    return gather_order(fwd_ptr.cpu(), partner.cpu(), overlap.cpu())

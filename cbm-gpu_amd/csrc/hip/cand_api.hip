@@ -331,6 +331,98 @@ __global__ __launch_bounds__(kListThreads) void k_cand_lists(const CandArgs a) {
     }
 }
 
+// Partner scan of the gather order (cbm_partners): for every row r of an
+// ordered row list, the two rows sharing the most columns with it.  The same
+// counting as k_cand_lists, over every other row instead of the rows above
+// r, and only through columns held by at most `cap` rows (a column held by
+// many rows adds one to each of them and tells little about which share
+// most, at a quadratic cost).  The counts are walked once for the best two
+// (most overlap, then lowest row), combined over the workgroup by a
+// butterfly and through LDS.
+struct PartnerArgs {
+    int32_t n = 0;
+    int32_t n_cols = 0;
+    int32_t cap = 0;
+    const int32_t *ptr = nullptr;    // rows: entry offsets and columns (bit 31 a sign)
+    const int32_t *col = nullptr;
+    const int32_t *t_ptr = nullptr;  // per column: the rows holding it
+    const int32_t *t_col = nullptr;
+    int32_t *partner = nullptr;      // [2 n]
+    int32_t *overlap = nullptr;      // [2 n]
+    uint32_t *next_row = nullptr;
+    uint32_t *slabs = nullptr;       // 2 n words per workgroup (counters, touched list)
+};
+
+// Ranking key of a (count, row) pair: more overlap first, then the lower row.
+__device__ inline uint64_t partner_key(uint32_t ov, int32_t row) {
+    return ov ? (uint64_t(ov) << 32) | uint64_t(uint32_t(INT32_MAX - row)) : 0ull;
+}
+
+// The best two of the pairs (a1 >= a2) and (b1 >= b2) of distinct rows.
+__device__ inline void top2_merge(uint64_t &a1, uint64_t &a2, uint64_t b1, uint64_t b2) {
+    const uint64_t hi = a1 > b1 ? a1 : b1, lo = a1 > b1 ? b1 : a1;
+    const uint64_t rest = a2 > b2 ? a2 : b2;
+    a1 = hi;
+    a2 = lo > rest ? lo : rest;
+}
+
+__global__ __launch_bounds__(kListThreads) void k_partner_lists(const PartnerArgs a) {
+    constexpr int kWaves = kListThreads / 64;
+    __shared__ uint64_t best[2 * kWaves];
+    __shared__ int32_t s_row, s_touched;
+    uint32_t *cnt = a.slabs + size_t(blockIdx.x) * 2 * size_t(a.n);
+    int32_t *touched = reinterpret_cast<int32_t *>(cnt + a.n);
+    const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    for (;;) {
+        if (tid == 0) s_touched = 0;
+        if (tid == 0) s_row = int32_t(min(atomicAdd(a.next_row, 1u), uint32_t(INT32_MAX)));
+        __syncthreads();
+        const int32_t r = s_row;
+        if (r >= a.n) return;  // the same row for every thread: the workgroup leaves together
+        const int32_t rb = a.ptr[r], re = a.ptr[r + 1];
+        for (int32_t i = rb + wave; i < re; i += kWaves) {
+            const int32_t k = a.col[i] & 0x7fffffff;
+            if (k >= a.n_cols) continue;
+            const int32_t kb = a.t_ptr[k], ke = a.t_ptr[k + 1];
+            if (ke - kb > a.cap) continue;
+            for (int32_t q = kb + lane; q < ke; q += 64) {
+                const int32_t c = a.t_col[q];
+                if (c == r || uint32_t(c) >= uint32_t(a.n)) continue;
+                if (atomicAdd(&cnt[c], 1u) == 0u) touched[atomicAdd(&s_touched, 1)] = c;
+            }
+        }
+        __syncthreads();
+        const int32_t nt = s_touched;
+        uint64_t k1 = 0, k2 = 0;
+        for (int32_t i = tid; i < nt; i += kListThreads) {
+            const int32_t c = touched[i];
+            const uint32_t ov = __hip_atomic_load(&cnt[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&cnt[c], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            top2_merge(k1, k2, partner_key(ov, c), 0ull);
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint64_t o1 = __shfl_xor(k1, off), o2 = __shfl_xor(k2, off);
+            top2_merge(k1, k2, o1, o2);
+        }
+        if (lane == 0) {
+            best[2 * wave] = k1;
+            best[2 * wave + 1] = k2;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint64_t b1 = 0, b2 = 0;
+            for (int w = 0; w < kWaves; ++w) top2_merge(b1, b2, best[2 * w], best[2 * w + 1]);
+            const uint64_t kk[2] = {b1, b2};
+            for (int j = 0; j < 2; ++j) {
+                a.partner[2 * int64_t(r) + j] = kk[j] ? INT32_MAX - int32_t(uint32_t(kk[j])) : -1;
+                a.overlap[2 * int64_t(r) + j] = int32_t(kk[j] >> 32);
+            }
+        }
+        __syncthreads();  // s_row, s_touched and best are rewritten for the next row
+    }
+}
+
 __global__ void k_zero_words(uint32_t *p, int64_t n) {
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
@@ -430,6 +522,54 @@ int run_pass(bool emit, int32_t n_rows, const int64_t *ptr, const int32_t *col, 
 }  // namespace
 
 extern "C" const char *cbm_candidates_last_error(void) { return g_error.c_str(); }
+
+extern "C" int64_t cbm_partners_workspace_bytes(int32_t n_rows, int32_t n_workgroups) {
+    Shape s;
+    return resolve(n_rows, CBM_CAND_LISTS, n_workgroups, s) ? s.bytes : -1;
+}
+
+extern "C" int cbm_partners(int32_t n_rows, const int32_t *ptr, const int32_t *col, int32_t n_cols,
+                            const int32_t *t_ptr, const int32_t *t_col, int32_t cap, int32_t n_workgroups,
+                            int32_t *partner, int32_t *overlap, void *workspace, int64_t workspace_bytes,
+                            hipStream_t stream) {
+    Shape s;
+    if (!resolve(n_rows, CBM_CAND_LISTS, n_workgroups, s) || n_cols < 0 || cap < 0 || !ptr || !t_ptr ||
+        !partner || !overlap || !workspace) {
+        set_error("cbm_partners: invalid arguments (n_rows=%d n_cols=%d cap=%d)", n_rows, n_cols, cap);
+        return CBM_ERR_ARG;
+    }
+    if ((reinterpret_cast<uintptr_t>(workspace) & 255u) != 0) {
+        set_error("cbm_partners: the workspace must be 256-byte aligned");
+        return CBM_ERR_ARG;
+    }
+    if (workspace_bytes < s.bytes) {
+        set_error("cbm_partners: workspace of %lld bytes, %lld needed", (long long)workspace_bytes,
+                  (long long)s.bytes);
+        return CBM_ERR_ARG;
+    }
+    PartnerArgs a;
+    a.n = n_rows;
+    a.n_cols = n_cols;
+    a.cap = cap;
+    a.ptr = ptr;
+    a.col = col;
+    a.t_ptr = t_ptr;
+    a.t_col = t_col;
+    a.partner = partner;
+    a.overlap = overlap;
+    a.next_row = static_cast<uint32_t *>(workspace);
+    a.slabs = reinterpret_cast<uint32_t *>(static_cast<char *>(workspace) + kHeaderBytes);
+    const int64_t words = s.bytes / 4;
+    k_zero_words<<<dim3(unsigned(std::min<int64_t>((words + 255) / 256, 8192))), dim3(256), 0, stream>>>(
+        static_cast<uint32_t *>(workspace), words);
+    k_partner_lists<<<dim3(unsigned(s.wgs)), dim3(kListThreads), 0, stream>>>(a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("cbm_partners: %s", hipGetErrorString(e));
+        return CBM_ERR_DEVICE;
+    }
+    return CBM_OK;
+}
 
 extern "C" int64_t cbm_candidates_workspace_bytes(int32_t n_rows, int32_t path, int32_t n_workgroups) {
     Shape s;

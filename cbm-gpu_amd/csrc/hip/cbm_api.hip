@@ -64,6 +64,9 @@ struct cbm_handle {
     uint32_t *fused_state = nullptr; // ticket, done, epoch, failed waits (cbm_reserve)
     uint32_t *fused_flags = nullptr; // a flag per unit and 64-column slab
     int64_t fused_flag_count = 0;
+    const int32_t *order = nullptr; // gather order of the short rows, borrowed (cbm_attach_order)
+    int32_t n_order = 0;
+    int32_t order_mode = 0;    // Delta pass in the gather order: 1 on, 0 off, -1 automatic
     int last_launches = 0;
 };
 
@@ -412,6 +415,13 @@ int forward_levels(cbm_handle *h, const Bufs &b, cbmk::RowPass rp, bool with_par
     return CBM_OK;
 }
 
+// Whether the Delta pass of every level at once runs in the attached gather
+// order (cbm_attach_order): an order attached and not switched off, and no
+// column slabs (their workgroup mapping is another one).
+bool order_on(const cbm_handle *h) {
+    return h->order_mode != 0 && h->n_order > 0 && h->slab_bytes == 0 && h->xcd_slabs <= 1;
+}
+
 // The Delta pass of every level at once (no parent term): all long-row
 // chunks with slots numbered over the levels, one short-row pass over all
 // positions that skips the long rows, then all reductions.
@@ -433,6 +443,12 @@ int delta_flat(cbm_handle *h, const Bufs &b, cbmk::RowPass rp, hipStream_t strea
     rd.scratch = b.scratch;
     rp.extra = cbmk::kExtraNone;
     rp.skip_above = CBM_LONG_ROW;
+    if (order_on(h)) {
+        // The short rows in the gather order, XCD by XCD; the long rows by
+        // their chunks as ever.
+        rp.order = h->order;
+        return run_range(h, rp, 0, h->n_order, 0, ck, rd, stream, what);
+    }
     return run_range(h, rp, 0, h->d.n_rows, 0, ck, rd, stream, what);
 }
 
@@ -927,6 +943,26 @@ extern "C" int cbm_uncompressed_active(cbm_handle *h, int32_t dtype, int32_t n_c
     return route(h, dtype, n_cols) != h ? 1 : 0;
 }
 
+extern "C" int cbm_attach_order(cbm_handle *h, int32_t n_order, const int32_t *order, int32_t mode) {
+    if (!h || n_order < 0 || n_order > h->d.n_rows || mode < -1 || mode > 1 || (n_order > 0 && !order)) {
+        set_error("cbm_attach_order: invalid arguments (n_order=%d mode=%d)", n_order, mode);
+        return CBM_ERR_ARG;
+    }
+    h->order = n_order > 0 ? order : nullptr;
+    h->n_order = n_order;
+    h->order_mode = n_order > 0 ? mode : 0;
+    return CBM_OK;
+}
+
+extern "C" int cbm_set_order(cbm_handle *h, int32_t mode) {
+    if (!h || mode < -1 || mode > 1) {
+        set_error("cbm_set_order: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    h->order_mode = h->n_order > 0 ? mode : 0;
+    return CBM_OK;
+}
+
 namespace {
 
 // Whether a forward-structure product of n_cols columns in this storage type
@@ -1001,13 +1037,26 @@ int forward_product(cbm_handle *h, const Bufs &b, bool ranges, int32_t form, int
             const int r = forward_ranges(h, b, *rs, rp, false, stream, what);
             return r == CBM_OK ? propagate_levels(h, b, form, y + c0, ldy, w, stream, what) : r;
         }
-        if (!split) return forward_levels(h, b, rp, true, stream, what);
+        // A one-level product (no tree) with a gather order runs the split
+        // form too: it is the same Delta pass, with nothing to propagate.
+        const bool ordered = h->d.n_levels == 1 && order_on(h);
+        if (!split && !ordered) return forward_levels(h, b, rp, true, stream, what);
         const int r = delta_flat(h, b, rp, stream, what);
         return r == CBM_OK ? propagate_levels(h, b, form, y + c0, ldy, w, stream, what) : r;
     });
 }
 
 }  // namespace
+
+extern "C" int cbm_order_active(cbm_handle *h, int32_t dtype, int32_t n_cols) {
+    if (!h || (dtype != CBM_F32 && dtype != CBM_BF16) || n_cols < 1) {
+        set_error("cbm_order_active: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    const cbm_handle *r = route(h, dtype, n_cols);
+    if (!order_on(r) || fused_possible(r, dtype, n_cols, true) || pick_ranges(r, n_cols, elem_of(dtype))) return 0;
+    return use_split(r) || r->d.n_levels == 1 ? 1 : 0;
+}
 
 extern "C" int cbm_matmul(cbm_handle *h, int32_t dtype, const void *x, int64_t ldx, float *y, int64_t ldy,
                           int32_t n_cols, hipStream_t stream) {

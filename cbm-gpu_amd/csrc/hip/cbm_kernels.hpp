@@ -96,6 +96,13 @@ struct RowPass {
     int32_t range_phases = 1;
     int32_t range_blocks = 0;
     int32_t range_first[9] = {};
+    // Row order of the short-row pass: when set, row slot i of [p_begin,
+    // p_end) is position order[i] (-1: none), and the pass's order_blocks
+    // workgroups (set by the launcher) take the row blocks XCD by XCD:
+    // workgroup b (dispatched to XCD b % 8) takes block b / 8 of XCD b % 8's
+    // contiguous eighth, so each XCD walks one stretch of the order.
+    const int32_t *order = nullptr;
+    int32_t order_blocks = 0;
 };
 
 // Wave slots of the device (256 CUs x 4 SIMDs x 8 waves): a pass with fewer

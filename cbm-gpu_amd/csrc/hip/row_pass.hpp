@@ -378,6 +378,10 @@ __device__ __forceinline__ void row_pass_block(const RowPass &a, uint32_t bid) {
                 p_end = a.range_first[i + 1];
             }
         }
+    } else if (a.order) {
+        const uint32_t x = bid & 7u, nb = uint32_t(a.order_blocks), base = nb >> 3, rem = nb & 7u;
+        slab = 0;
+        blk = int64_t(x) * base + min(x, rem) + int64_t(bid >> 3);
     } else {
         slab = int32_t(bid % uint32_t(a.xcd_slabs));
         blk = int64_t(bid / uint32_t(a.xcd_slabs));
@@ -386,7 +390,8 @@ __device__ __forceinline__ void row_pass_block(const RowPass &a, uint32_t bid) {
                           lane / kLanesPerRow;
     const int64_t p64 = int64_t(p_begin) + group;
     if (p64 >= int64_t(p_end)) return;  // a row's lanes leave together
-    const int32_t p = int32_t(p64);
+    const int32_t p = a.order ? a.order[p64] : int32_t(p64);
+    if (p < 0) return;
     const int32_t kb = a.ptr[p], ke = a.ptr_end ? a.ptr_end[p] : a.ptr[p + 1];
     if (ke - kb > a.skip_above) return;  // a long row, done by the chunk kernel
     const int32_t row = a.node_of ? a.node_of[p] : p;
@@ -673,27 +678,31 @@ hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
             return launch_one<P, VE, LPR>(rp, Launch{Kind::kPass, nullptr, nullptr}, stream);
         }
         *lk.launched = 1;
+        RowPass r = rp;
+        r.order_blocks = int32_t(pass_blocks);
         if constexpr (LPR < 64) {
             if (wave) {
                 k_row_mixed<P, VE, LPR, kWaveS, kBs><<<dim3(unsigned(chunk_blocks + pass_blocks)), dim3(kBs), 0,
-                                                       stream>>>(rp, *lk.chunk, uint32_t(chunk_blocks));
+                                                       stream>>>(r, *lk.chunk, uint32_t(chunk_blocks));
                 return hipGetLastError();
             }
         }
         k_row_mixed<P, VE, LPR, 1, kBs><<<dim3(unsigned(chunk_blocks + pass_blocks)), dim3(kBs), 0, stream>>>(
-            rp, *lk.chunk, uint32_t(chunk_blocks));
+            r, *lk.chunk, uint32_t(chunk_blocks));
     } else if (lk.kind == Kind::kPass) {
         const int64_t n = int64_t(rp.p_end) - rp.p_begin;
         if (n <= 0) return hipSuccess;
         const int64_t ns = rp.xcd_slabs;
+        RowPass r = rp;
         if constexpr (LPR < 64) {
             if (wave_per_row<LPR>(rp)) {
-                k_row_pass<P, VE, LPR, 64 / LPR><<<dim3(unsigned((n + 3) / 4 * ns)), dim3(256), 0, stream>>>(rp);
+                r.order_blocks = int32_t((n + 3) / 4 * ns);
+                k_row_pass<P, VE, LPR, 64 / LPR><<<dim3(unsigned(r.order_blocks)), dim3(256), 0, stream>>>(r);
                 return hipGetLastError();
             }
         }
-        k_row_pass<P, VE, LPR, 1>
-            <<<dim3(unsigned((n + kRowsPerBlock - 1) / kRowsPerBlock * ns)), dim3(256), 0, stream>>>(rp);
+        r.order_blocks = int32_t((n + kRowsPerBlock - 1) / kRowsPerBlock * ns);
+        k_row_pass<P, VE, LPR, 1><<<dim3(unsigned(r.order_blocks)), dim3(256), 0, stream>>>(r);
     } else if (lk.kind == Kind::kTail) {
         if constexpr (P::kDual && P::kExtra == kExtraSelf) {  // the reverse pass only
             if (lk.tail->n_frag <= 0) return hipSuccess;

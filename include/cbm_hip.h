@@ -444,6 +444,28 @@ int cbm_attach_uncompressed(cbm_handle *h, cbm_handle *plain, int32_t mode);
 int cbm_uncompressed_active(cbm_handle *h, int32_t dtype, int32_t n_cols);
 
 /*
+ * Gather order of the Delta pass: order [n_order], device, borrowed (it must
+ * outlive the handle's use of it), the short rows of this handle's forward
+ * row list (plan array CBM_P_FWD_PTR) in the order of cbm_gather_order
+ * (cbm_host.h), every short row exactly once.  The Delta pass over all levels
+ * at once (the split forward, cbm_delta_spmm, and the adjoint through the
+ * forward structure) then sums the short rows in that order, each XCD
+ * taking one contiguous stretch of it, so that rows sharing columns run
+ * together on one L2; each row is summed as before, so the results are the
+ * same bits.  Not used with XCD column ranges, column slabs or the
+ * one-launch forward.  mode: 1 on, 0 off, -1 automatic (the same as on).
+ * n_order = 0 detaches.  No counterpart in the reference, whose Delta SpMM
+ * is one cusparseSpMM call (cbm/cbm.cu:184-189).
+ */
+int cbm_attach_order(cbm_handle *h, int32_t n_order, const int32_t *order, int32_t mode);
+int cbm_set_order(cbm_handle *h, int32_t mode);
+
+/* Whether a forward product of n_cols columns in dtype storage runs its
+   Delta pass in the attached gather order (1) or not (0); CBM_ERR_ARG for
+   invalid arguments. */
+int cbm_order_active(cbm_handle *h, int32_t dtype, int32_t n_cols);
+
+/*
  * Tree tails (1 on, the default; 0 off; 2 and 3 on with other staging).  The
  * deepest tree levels of the forward propagation and of the adjoint's reverse
  * pass, each of which would be a short dependent launch, are finished in one
@@ -539,6 +561,25 @@ int cbm_candidates_emit(int32_t n_rows, const int64_t *ptr, const int32_t *col, 
 
 /* Message describing the last cbm_candidates_* failure on the calling thread. */
 const char *cbm_candidates_last_error(void);
+
+/*
+ * Partner scan of the gather order (cbm_gather_order in cbm_host.h; no
+ * counterpart in the reference).  For every row r of an ordered row list
+ * (ptr int32 [n_rows + 1], col int32 with bit 31 a sign, columns in
+ * [0, n_cols)) and, per column, the rows holding it (t_ptr int32
+ * [n_cols + 1], t_col int32): partner[2r], partner[2r + 1] receive the two
+ * other rows sharing the most columns with r (most first, ties to the lower
+ * row; -1 for none) and overlap[2r], overlap[2r + 1] the shared counts,
+ * counted only through columns held by at most `cap` rows.  Rows whose
+ * entries are not wanted (long rows) are simply left out of t_col.  The
+ * workspace is device memory, 256-byte aligned, of at least
+ * cbm_partners_workspace_bytes bytes (as the list path of the candidate
+ * scan).  Asynchronous on `stream`; allocates nothing.
+ */
+int64_t cbm_partners_workspace_bytes(int32_t n_rows, int32_t n_workgroups);
+int cbm_partners(int32_t n_rows, const int32_t *ptr, const int32_t *col, int32_t n_cols, const int32_t *t_ptr,
+                 const int32_t *t_col, int32_t cap, int32_t n_workgroups, int32_t *partner, int32_t *overlap,
+                 void *workspace, int64_t workspace_bytes, hipStream_t stream);
 
 /* ------------------------------------------------------------------------- */
 /* Row clustering: the data passes of the fingerprint clustering of           */

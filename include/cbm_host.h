@@ -337,6 +337,25 @@ int64_t cbm_plan_stat(const cbm_plan *p, int32_t which);
 int32_t cbm_plan_depth(const cbm_plan *p);
 void cbm_plan_free(cbm_plan *p);
 
+/* ------------------------------------------------------------------------- */
+/* Gather order of the Delta pass.                                            */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * No counterpart in the reference (its Delta SpMM is one cusparseSpMM call,
+ * cbm/cbm.cu:184-189).  An order of the short rows (at most CBM_LONG_ROW
+ * entries) of an ordered row list (ptr [n+1], as CBM_P_FWD_PTR of a plan) in
+ * which rows that share columns sit next to one another, for the row pass
+ * to take XCD by XCD (cbm_attach_order in cbm_hip.h): a depth-first walk,
+ * heaviest child first, of the maximum spanning forest of the partner edges
+ * (partner [n * n_partners], -1 for none, with their overlap counts; from
+ * cbm_partners in cbm_hip.h).  Writes the n_order short rows to order [n]
+ * (caller-provided, at least n entries).  Any partner lists give a valid
+ * order; the order never changes a result, only which rows run together.
+ */
+int cbm_gather_order(int32_t n, const int32_t *ptr, const int32_t *partner, const int32_t *overlap,
+                     int32_t n_partners, int32_t *order, int32_t *n_order);
+
 #ifdef __cplusplus
 }
 #endif
