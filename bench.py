@@ -772,7 +772,8 @@ def main():
                      "traffic_source": (f"committed {traffic.get('source')}" if traffic else None),
                      "kernel": (f"{dom} (BF16): " + ("k_fused_forward (one launch: Delta pieces + LDS propagation)"
                                                      if dom == "forward" else
-                                                     "the Delta pass (k_row_mixed: long-row chunks + short rows, or "
+                                                     "the Delta pass (k_row_mixed: long-row chunks + short rows, "
+                                                     "in the gather order where the matrix has one, or "
                                                      "k_row_pass over XCD range pieces) + k_row_reduce")),
                      "algorithmic_bytes": rec["bytes_phase_bf16"][dom],
                      "ms": rec["ms_phase_bf16"][dom],

@@ -120,15 +120,21 @@ class cbm4mm:
         storage.save(path, n=self.num_nodes, alpha=self.alpha, delta_crow=self.deltas.crow_indices(),
                      delta_col=self.deltas.col_indices(), delta_sign=torch.sign(self.deltas.values()),
                      mca_offset=self.mca_offset, mca_src_idx=self.mca_src_idx, mca_dst_idx=self.mca_dst_idx,
-                     plan=plan, build_stats=self.build_stats, saved_by=type(self).__name__)
+                     plan=plan, build_stats=self.build_stats, saved_by=type(self).__name__,
+                     gather_order=getattr(ext, "_order", None))
 
     @classmethod
-    def load(cls, path, *, device="cuda"):
+    def load(cls, path, *, device="cuda", gather_order=None):
         """A matrix of this class from a structure file (any ``cbm4*`` class's
-        file: the structure does not depend on the product)."""
+        file: the structure does not depend on the product).  The file's
+        gather order is used when it has one; otherwise ``gather_order``
+        decides as in the constructor (False: none either way)."""
         st = storage.load(path)
         obj = cls.__new__(cls)
         obj.alpha = st.alpha
+        if gather_order is not False:
+            obj._stored_order = st.gather_order
+            obj._order_wanted = st.n > ORDER_MIN_ROWS if gather_order is None else True
         obj._assemble(st.n, st.delta_crow, st.delta_col, st.delta_values(), st.mca_offset, st.mca_src_idx,
                       st.mca_dst_idx, st.build_stats, device, 0, plan=st.plan)
         return obj

@@ -5,9 +5,10 @@ a file here holds everything a ``cbm4*`` object needs, so loading skips the
 candidate scan, the arborescence and the device layout.
 
 Format: one uncompressed ``.npz`` (numpy's zip of ``.npy`` arrays, read with
-``allow_pickle=False``), version 4 (version 1 lacked the layout's 4-range
+``allow_pickle=False``), version 5 (version 1 lacked the layout's 4-range
 piece set, versions 1 and 2 its tree tails, versions 1 to 3 the units of the
-one-launch forward; such files still load, without them):
+one-launch forward, versions 1 to 4 the gather order; such files still load,
+without them):
 
   meta               uint8   JSON text: format, version, n, alpha, the class
                              that saved it, build and layout statistics, the
@@ -20,6 +21,8 @@ one-launch forward; such files still load, without them):
   mca_dst_idx        int32   [E]      tree edges, child   (init's mca_col_idx)
   plan/<name>        int32            the device layout of cbm_plan_create, one
                                       array per CBM_P_* slot (include/cbm_host.h)
+  gather_order       int32   [m]      the short rows' gather order (cbm_gather_order),
+                                      present when the saved matrix had one
 
 The structure does not depend on the product (A, A D, D A D), so a file saved
 by any ``cbm4*`` class loads into any of them.
@@ -34,7 +37,7 @@ import torch
 from . import structure
 
 FORMAT = "cbm-structure"
-VERSION = 4
+VERSION = 5
 # Layout arrays (and statistics) that older files do not have, by version.
 _NEW_IN_2 = ("fpc4_b", "fpc4_e", "fpc4_slot", "fpc4_range", "frs4_slot", "frs4_cnt", "bpc4_b", "bpc4_e", "bpc4_mid",
              "bpc4_slot", "bpc4_range", "brs4_slot", "brs4_cnt", "bpc4_row")
@@ -67,6 +70,7 @@ class Stored:
     mca_dst_idx: torch.Tensor
     plan: structure.Plan
     build_stats: dict = field(default_factory=dict)
+    gather_order: torch.Tensor = None
 
     def delta_values(self):
         """The +-1 values of Delta as FP32."""
@@ -82,14 +86,17 @@ def _np(t, dtype):
 
 
 def save(path, *, n, alpha, delta_crow, delta_col, delta_sign, mca_offset, mca_src_idx, mca_dst_idx, plan,
-         build_stats=None, saved_by=""):
+         build_stats=None, saved_by="", gather_order=None):
     """Writes a structure file.  ``plan`` is a :class:`cbm.structure.Plan`
-    of the same matrix; ``delta_sign`` holds the signs of Delta's entries."""
+    of the same matrix; ``delta_sign`` holds the signs of Delta's entries;
+    ``gather_order`` (optional) the short rows' gather order."""
     arrays = {"delta_crow": _np(delta_crow, np.int32), "delta_col": _np(delta_col, np.int32),
               "delta_sign": _np(delta_sign, np.int8), "mca_offset": _np(mca_offset, np.int32),
               "mca_src_idx": _np(mca_src_idx, np.int32), "mca_dst_idx": _np(mca_dst_idx, np.int32)}
     for name in structure.PLAN_NAMES:
         arrays["plan/" + name] = _np(plan.arrays[name], np.int32)
+    if gather_order is not None:
+        arrays["gather_order"] = _np(gather_order, np.int32)
     n = int(n)
     if arrays["delta_crow"].shape != (n + 1,) or arrays["delta_col"].shape != arrays["delta_sign"].shape:
         raise ValueError("Delta arrays do not describe an n x n CSR matrix")
@@ -120,11 +127,13 @@ def load(path):
         if meta.get("format") != FORMAT:
             raise StructureFileError(f"{path}: format {meta.get('format')!r}, expected {FORMAT!r}")
         version = meta.get("version")
-        if version not in (1, 2, 3, VERSION):
+        if version not in (1, 2, 3, 4, VERSION):
             raise StructureFileError(f"{path}: version {version}, this library reads 1 to {VERSION}")
         absent = ((set(_NEW_IN_2) if version < 2 else set()) | (set(_NEW_IN_3) if version < 3 else set())
                   | (set(_NEW_IN_4) if version < 4 else set()))
         names = list(_CORE) + ["plan/" + k for k in structure.PLAN_NAMES if k not in absent]
+        if version >= 5 and "gather_order" in meta["crc32"]:
+            names.append("gather_order")
         arrays = {"plan/" + k: np.zeros(0, np.int32) for k in absent}
         for name in names:
             if name not in z.files:
@@ -144,6 +153,12 @@ def load(path):
             or arrays["plan/lvl_node"].shape[0] != n or arrays["plan/fwd_col"].shape[0] != nnz
             or arrays["plan/bwd_row"].shape[0] != nnz or arrays["plan/ch_node"].shape[0] != e):
         raise StructureFileError(f"{path}: array sizes disagree with n = {n}")
+    if "gather_order" in arrays:
+        order = arrays["gather_order"]
+        lens = np.diff(arrays["plan/fwd_ptr"])
+        short = np.nonzero(lens <= structure.LONG_ROW)[0]
+        if order.shape[0] != short.shape[0] or not np.array_equal(np.sort(order), short):
+            raise StructureFileError(f"{path}: the gather order is not an order of the short rows")
     t = {k: torch.from_numpy(v.copy()) for k, v in arrays.items()}
     stats = dict(meta["plan_stats"])
     for k in _NEW_STATS_IN_2:
@@ -157,4 +172,4 @@ def load(path):
     return Stored(n=n, alpha=int(meta["alpha"]), saved_by=meta.get("saved_by", ""), delta_crow=t["delta_crow"],
                   delta_col=t["delta_col"], delta_sign=t["delta_sign"], mca_offset=t["mca_offset"],
                   mca_src_idx=t["mca_src_idx"], mca_dst_idx=t["mca_dst_idx"], plan=plan,
-                  build_stats=dict(meta["build_stats"]))
+                  build_stats=dict(meta["build_stats"]), gather_order=t.get("gather_order"))

@@ -203,3 +203,21 @@ def test_automatic_choice_orders_only_large_matrices(graph, cuda):
     m = cbm4mm(ei, torch.ones(ei.shape[1]), alpha=0, device=cuda)
     assert getattr(m.cusparse_extension, "_order", None) is None
     assert not m.cusparse_extension.order_active(torch.bfloat16, 128)
+
+
+def test_saved_order_loads_back(mats, cuda, tmp_path):
+    m = mats["ax"]
+    path = tmp_path / "m.cbm.npz"
+    m.save(path)
+    m2 = cbm4mm.load(path, device=cuda)
+    assert torch.equal(m2.cusparse_extension._order, m.cusparse_extension._order)
+    m2.cusparse_extension.set_wave_rows(False)
+    x = _x(N, 64, torch.bfloat16, 31, cuda)
+    y1 = torch.empty(N, 64, device=cuda)
+    y2 = torch.empty(N, 64, device=cuda)
+    m.matmul(x, y1)
+    m2.matmul(x, y2)
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
+    m3 = cbm4mm.load(path, device=cuda, gather_order=False)
+    assert getattr(m3.cusparse_extension, "_order", None) is None

@@ -127,22 +127,24 @@ def test_save_checks_its_input(built, tmp_path):
         _save(tmp_path / "y.npz", b, pl, delta_sign=torch.zeros_like(b.csr_val))
 
 
-@pytest.mark.parametrize("version", [1, 2, 3])
+@pytest.mark.parametrize("version", [1, 2, 3, 4])
 def test_older_files_load_without_the_newer_layout_parts(built, tmp_path, version):
     """A version 1 file (no 4-range piece set, no tree tails, no units), a
-    version 2 file (no tails, no units) or a version 3 file (no units) loads
-    with those parts empty -- no pieces, tails at the depth, i.e. none, no
-    units -- and the rest of the layout unchanged."""
+    version 2 file (no tails, no units), a version 3 file (no units) or a
+    version 4 file (no gather order) loads with those parts empty -- no
+    pieces, tails at the depth, i.e. none, no units, no order -- and the rest
+    of the layout unchanged."""
     b, pl = built
-    path = tmp_path / "v4.npz"
-    _save(path, b, pl)
-    gone = (set(storage._NEW_IN_4) | (set(storage._NEW_IN_3) if version <= 2 else set())
+    path = tmp_path / "v5.npz"
+    _save(path, b, pl, gather_order=_short_rows(pl))
+    gone = ((set(storage._NEW_IN_4) if version <= 3 else set()) | (set(storage._NEW_IN_3) if version <= 2 else set())
             | (set(storage._NEW_IN_2) if version == 1 else set()))
-    gone_stats = (set(storage._NEW_STATS_IN_4) | (set(storage._NEW_STATS_IN_3) if version <= 2 else set())
+    gone_stats = ((set(storage._NEW_STATS_IN_4) if version <= 3 else set())
+                  | (set(storage._NEW_STATS_IN_3) if version <= 2 else set())
                   | (set(storage._NEW_STATS_IN_2) if version == 1 else set()))
 
     def older(name, a):
-        if name.startswith("plan/") and name[5:] in gone:
+        if (name.startswith("plan/") and name[5:] in gone) or name == "gather_order":
             return None
         if name == "meta":
             m = json.loads(bytes(a).decode())
@@ -151,10 +153,12 @@ def test_older_files_load_without_the_newer_layout_parts(built, tmp_path, versio
                 m["plan_stats"].pop(k)
             for k in gone:
                 m["crc32"].pop("plan/" + k)
+            m["crc32"].pop("gather_order")
             return np.frombuffer(json.dumps(m).encode(), dtype=np.uint8)
         return a
     _rewrite(path, tmp_path / "old.npz", older)
     st = storage.load(tmp_path / "old.npz")
+    assert st.gather_order is None
     for k in gone_stats:
         assert st.plan.stats[k] == (pl.depth if k in storage._NEW_STATS_IN_3 else 0), k
     for k in structure.PLAN_NAMES:
@@ -162,3 +166,25 @@ def test_older_files_load_without_the_newer_layout_parts(built, tmp_path, versio
             assert st.plan.arrays[k].numel() == 0, k
         else:
             assert torch.equal(st.plan.arrays[k], pl.arrays[k]), k
+
+
+def _short_rows(pl):
+    return torch.nonzero(pl.arrays["fwd_ptr"].diff() <= structure.LONG_ROW).view(-1).to(torch.int32).flip(0)
+
+
+def test_gather_order_round_trip(built, tmp_path):
+    b, pl = built
+    order = _short_rows(pl)
+    _save(tmp_path / "o.npz", b, pl, gather_order=order)
+    assert torch.equal(storage.load(tmp_path / "o.npz").gather_order, order)
+    _save(tmp_path / "none.npz", b, pl)
+    assert storage.load(tmp_path / "none.npz").gather_order is None
+
+
+def test_a_gather_order_of_other_rows_is_refused(built, tmp_path):
+    b, pl = built
+    order = _short_rows(pl).clone()
+    order[0] = order[1]
+    _save(tmp_path / "bad.npz", b, pl, gather_order=order)
+    with pytest.raises(storage.StructureFileError, match="gather order"):
+        storage.load(tmp_path / "bad.npz")
