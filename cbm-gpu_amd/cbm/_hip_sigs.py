@@ -192,7 +192,8 @@ def declare(lib, sig):
     sig(lib, "cbm_candidates_emit", c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp,
         c_vp, c_i64, c_vp, c_i64, c_vp)
     sig(lib, "cbm_partners_workspace_bytes", c_i64, c_i32, c_i32)
-    sig(lib, "cbm_partners", c_i32, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_i64, c_vp)
+    sig(lib, "cbm_partners", c_i32, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i64,
+        c_vp)
 
 
 # Every symbol include/cbm_hip.h declares (checked by the CPU test suite).

@@ -295,13 +295,15 @@ LONG_ROW = 256        # CBM_LONG_ROW
 # Columns held by more rows than this are not counted by the partner scan
 # (each adds one to all of them and separates little, at a quadratic cost).
 PARTNER_CAP = 256
+# Partners per row the forest is built from.
+PARTNERS = 4
 
 
-def device_partners(fwd_ptr, fwd_col, device, cap=PARTNER_CAP, n_workgroups=0):
-    """The two best-overlapping partner rows of every short row of a forward
-    row list (``cbm_partners``, HIP): ``(partner, overlap)``, int32 ``[n, 2]``
-    on ``device``; -1 / 0 where there is none.  Long rows are left out of the
-    column lists, so they are nobody's partner."""
+def device_partners(fwd_ptr, fwd_col, device, cap=PARTNER_CAP, k=PARTNERS, n_workgroups=0):
+    """The k (1 to 4) best-overlapping partner rows of every short row of a
+    forward row list (``cbm_partners``, HIP): ``(partner, overlap)``, int32
+    ``[n, k]`` on ``device``; -1 / 0 where there is none.  Long rows are left
+    out of the column lists, so they are nobody's partner."""
     dev = torch.device(device)
     if dev.type != "cuda":
         raise RuntimeError("tensor is not a CUDA tensor")
@@ -328,11 +330,11 @@ def device_partners(fwd_ptr, fwd_col, device, cap=PARTNER_CAP, n_workgroups=0):
             raise ValueError(f"no partner workspace for {n} rows")
         ws = torch.empty(ws_bytes + 256, dtype=torch.uint8, device=dev)
         ws_ptr = (ws.data_ptr() + 255) // 256 * 256
-        partner = torch.empty((n, 2), dtype=torch.int32, device=dev)
-        overlap = torch.empty((n, 2), dtype=torch.int32, device=dev)
+        partner = torch.empty((n, int(k)), dtype=torch.int32, device=dev)
+        overlap = torch.empty((n, int(k)), dtype=torch.int32, device=dev)
         stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
         _lib.check_candidates(hip.cbm_partners(n, ptr.data_ptr(), col.data_ptr(), n, t_ptr.data_ptr(),
-                                               t_col.data_ptr(), int(cap), int(n_workgroups), partner.data_ptr(),
+                                               t_col.data_ptr(), int(cap), int(k), int(n_workgroups), partner.data_ptr(),
                                                overlap.data_ptr(), ws_ptr, ws_bytes, stream), "partners")
         torch.cuda.current_stream(dev).synchronize()
         del ws
@@ -358,8 +360,8 @@ def gather_order(fwd_ptr, partner, overlap):
     return out[:k.value].clone()
 
 
-def device_gather_order(fwd_ptr, fwd_col, device):
+def device_gather_order(fwd_ptr, fwd_col, device, cap=PARTNER_CAP, k=PARTNERS):
     """The gather order of a forward row list held on ``device``: the partner
     scan there, the forest walk on the host."""
-    partner, overlap = device_partners(fwd_ptr, fwd_col, device)
+    partner, overlap = device_partners(fwd_ptr, fwd_col, device, cap=cap, k=k)
     return gather_order(fwd_ptr.cpu(), partner.cpu(), overlap.cpu())

@@ -332,23 +332,26 @@ __global__ __launch_bounds__(kListThreads) void k_cand_lists(const CandArgs a) {
 }
 
 // Partner scan of the gather order (cbm_partners): for every row r of an
-// ordered row list, the two rows sharing the most columns with it.  The same
+// ordered row list, the k rows sharing the most columns with it.  The same
 // counting as k_cand_lists, over every other row instead of the rows above
 // r, and only through columns held by at most `cap` rows (a column held by
 // many rows adds one to each of them and tells little about which share
-// most, at a quadratic cost).  The counts are walked once for the best two
-// (most overlap, then lowest row), combined over the workgroup by a
-// butterfly and through LDS.
+// most, at a quadratic cost).  The counts are walked once for the best
+// kMaxPartners (most overlap, then lowest row), combined over the workgroup
+// by a tree in LDS.
+constexpr int kMaxPartners = 4;
+
 struct PartnerArgs {
     int32_t n = 0;
     int32_t n_cols = 0;
     int32_t cap = 0;
+    int32_t k = 0;                   // partners written per row (1 .. kMaxPartners)
     const int32_t *ptr = nullptr;    // rows: entry offsets and columns (bit 31 a sign)
     const int32_t *col = nullptr;
     const int32_t *t_ptr = nullptr;  // per column: the rows holding it
     const int32_t *t_col = nullptr;
-    int32_t *partner = nullptr;      // [2 n]
-    int32_t *overlap = nullptr;      // [2 n]
+    int32_t *partner = nullptr;      // [k n]
+    int32_t *overlap = nullptr;      // [k n]
     uint32_t *next_row = nullptr;
     uint32_t *slabs = nullptr;       // 2 n words per workgroup (counters, touched list)
 };
@@ -358,17 +361,19 @@ __device__ inline uint64_t partner_key(uint32_t ov, int32_t row) {
     return ov ? (uint64_t(ov) << 32) | uint64_t(uint32_t(INT32_MAX - row)) : 0ull;
 }
 
-// The best two of the pairs (a1 >= a2) and (b1 >= b2) of distinct rows.
-__device__ inline void top2_merge(uint64_t &a1, uint64_t &a2, uint64_t b1, uint64_t b2) {
-    const uint64_t hi = a1 > b1 ? a1 : b1, lo = a1 > b1 ? b1 : a1;
-    const uint64_t rest = a2 > b2 ? a2 : b2;
-    a1 = hi;
-    a2 = lo > rest ? lo : rest;
+// Inserts x into the descending list t (keys of distinct rows; 0 is empty).
+__device__ inline void top_insert(uint64_t (&t)[kMaxPartners], uint64_t x) {
+#pragma unroll
+    for (int i = 0; i < kMaxPartners; ++i) {
+        const uint64_t hi = t[i] > x ? t[i] : x, lo = t[i] > x ? x : t[i];
+        t[i] = hi;
+        x = lo;
+    }
 }
 
 __global__ __launch_bounds__(kListThreads) void k_partner_lists(const PartnerArgs a) {
     constexpr int kWaves = kListThreads / 64;
-    __shared__ uint64_t best[2 * kWaves];
+    __shared__ uint64_t best[kMaxPartners * kListThreads];
     __shared__ int32_t s_row, s_touched;
     uint32_t *cnt = a.slabs + size_t(blockIdx.x) * 2 * size_t(a.n);
     int32_t *touched = reinterpret_cast<int32_t *>(cnt + a.n);
@@ -393,30 +398,33 @@ __global__ __launch_bounds__(kListThreads) void k_partner_lists(const PartnerArg
         }
         __syncthreads();
         const int32_t nt = s_touched;
-        uint64_t k1 = 0, k2 = 0;
+        uint64_t t[kMaxPartners] = {};
         for (int32_t i = tid; i < nt; i += kListThreads) {
             const int32_t c = touched[i];
             const uint32_t ov = __hip_atomic_load(&cnt[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&cnt[c], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            top2_merge(k1, k2, partner_key(ov, c), 0ull);
+            top_insert(t, partner_key(ov, c));
         }
+        // Tree over the workgroup through LDS: at each step the lower half
+        // takes in the upper half's lists.
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint64_t o1 = __shfl_xor(k1, off), o2 = __shfl_xor(k2, off);
-            top2_merge(k1, k2, o1, o2);
+        for (int j = 0; j < kMaxPartners; ++j) best[kMaxPartners * tid + j] = t[j];
+        for (int stride = kListThreads / 2; stride > 0; stride >>= 1) {
+            __syncthreads();
+            if (tid < stride) {
+#pragma unroll
+                for (int j = 0; j < kMaxPartners; ++j) top_insert(t, best[kMaxPartners * (tid + stride) + j]);
+#pragma unroll
+                for (int j = 0; j < kMaxPartners; ++j) best[kMaxPartners * tid + j] = t[j];
+            }
         }
-        if (lane == 0) {
-            best[2 * wave] = k1;
-            best[2 * wave + 1] = k2;
-        }
-        __syncthreads();
         if (tid == 0) {
-            uint64_t b1 = 0, b2 = 0;
-            for (int w = 0; w < kWaves; ++w) top2_merge(b1, b2, best[2 * w], best[2 * w + 1]);
-            const uint64_t kk[2] = {b1, b2};
-            for (int j = 0; j < 2; ++j) {
-                a.partner[2 * int64_t(r) + j] = kk[j] ? INT32_MAX - int32_t(uint32_t(kk[j])) : -1;
-                a.overlap[2 * int64_t(r) + j] = int32_t(kk[j] >> 32);
+#pragma unroll
+            for (int j = 0; j < kMaxPartners; ++j) {
+                if (j < a.k) {
+                    a.partner[int64_t(a.k) * r + j] = t[j] ? INT32_MAX - int32_t(uint32_t(t[j])) : -1;
+                    a.overlap[int64_t(a.k) * r + j] = int32_t(t[j] >> 32);
+                }
             }
         }
         __syncthreads();  // s_row, s_touched and best are rewritten for the next row
@@ -529,13 +537,13 @@ extern "C" int64_t cbm_partners_workspace_bytes(int32_t n_rows, int32_t n_workgr
 }
 
 extern "C" int cbm_partners(int32_t n_rows, const int32_t *ptr, const int32_t *col, int32_t n_cols,
-                            const int32_t *t_ptr, const int32_t *t_col, int32_t cap, int32_t n_workgroups,
-                            int32_t *partner, int32_t *overlap, void *workspace, int64_t workspace_bytes,
-                            hipStream_t stream) {
+                            const int32_t *t_ptr, const int32_t *t_col, int32_t cap, int32_t k,
+                            int32_t n_workgroups, int32_t *partner, int32_t *overlap, void *workspace,
+                            int64_t workspace_bytes, hipStream_t stream) {
     Shape s;
-    if (!resolve(n_rows, CBM_CAND_LISTS, n_workgroups, s) || n_cols < 0 || cap < 0 || !ptr || !t_ptr ||
-        !partner || !overlap || !workspace) {
-        set_error("cbm_partners: invalid arguments (n_rows=%d n_cols=%d cap=%d)", n_rows, n_cols, cap);
+    if (!resolve(n_rows, CBM_CAND_LISTS, n_workgroups, s) || n_cols < 0 || cap < 0 || k < 1 ||
+        k > kMaxPartners || !ptr || !t_ptr || !partner || !overlap || !workspace) {
+        set_error("cbm_partners: invalid arguments (n_rows=%d n_cols=%d cap=%d k=%d)", n_rows, n_cols, cap, k);
         return CBM_ERR_ARG;
     }
     if ((reinterpret_cast<uintptr_t>(workspace) & 255u) != 0) {
@@ -551,6 +559,7 @@ extern "C" int cbm_partners(int32_t n_rows, const int32_t *ptr, const int32_t *c
     a.n = n_rows;
     a.n_cols = n_cols;
     a.cap = cap;
+    a.k = k;
     a.ptr = ptr;
     a.col = col;
     a.t_ptr = t_ptr;

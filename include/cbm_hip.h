@@ -567,19 +567,19 @@ const char *cbm_candidates_last_error(void);
  * counterpart in the reference).  For every row r of an ordered row list
  * (ptr int32 [n_rows + 1], col int32 with bit 31 a sign, columns in
  * [0, n_cols)) and, per column, the rows holding it (t_ptr int32
- * [n_cols + 1], t_col int32): partner[2r], partner[2r + 1] receive the two
- * other rows sharing the most columns with r (most first, ties to the lower
- * row; -1 for none) and overlap[2r], overlap[2r + 1] the shared counts,
- * counted only through columns held by at most `cap` rows.  Rows whose
- * entries are not wanted (long rows) are simply left out of t_col.  The
- * workspace is device memory, 256-byte aligned, of at least
- * cbm_partners_workspace_bytes bytes (as the list path of the candidate
- * scan).  Asynchronous on `stream`; allocates nothing.
+ * [n_cols + 1], t_col int32): partner[k r .. k r + k) receive the k (1 to
+ * 4) other rows sharing the most columns with r (most first, ties to the
+ * lower row; -1 for none) and overlap[k r ..] the shared counts, counted
+ * only through columns held by at most `cap` rows.  Rows whose entries are
+ * not wanted (long rows) are simply left out of t_col.  The workspace is
+ * device memory, 256-byte aligned, of at least cbm_partners_workspace_bytes
+ * bytes (as the list path of the candidate scan).  Asynchronous on
+ * `stream`; allocates nothing.
  */
 int64_t cbm_partners_workspace_bytes(int32_t n_rows, int32_t n_workgroups);
 int cbm_partners(int32_t n_rows, const int32_t *ptr, const int32_t *col, int32_t n_cols, const int32_t *t_ptr,
-                 const int32_t *t_col, int32_t cap, int32_t n_workgroups, int32_t *partner, int32_t *overlap,
-                 void *workspace, int64_t workspace_bytes, hipStream_t stream);
+                 const int32_t *t_col, int32_t cap, int32_t k, int32_t n_workgroups, int32_t *partner,
+                 int32_t *overlap, void *workspace, int64_t workspace_bytes, hipStream_t stream);
 
 /* ------------------------------------------------------------------------- */
 /* Row clustering: the data passes of the fingerprint clustering of           */

@@ -132,6 +132,26 @@ def _check_products(wl, mat, mode, f, dtypes=("fp32", "bf16")):
     return errs
 
 
+def _check_order_bits(wl, mat, f):
+    """The gather order changes which rows run together, never a bit: the
+    forward and the adjoint equal those in position order."""
+    h = mat.cusparse_extension
+    x = torch.from_numpy(_randn(wl.n, f, 4000 + f)).to(wl.cuda).to(torch.bfloat16)
+    outs = []
+    try:
+        for on in (True, False):
+            h.set_order(on)
+            y = torch.empty(wl.n, f, device=wl.cuda)
+            dx = torch.empty(wl.n, f, device=wl.cuda)
+            mat.matmul(x, y)
+            h.matmul_t(x, dx)
+            torch.cuda.synchronize()
+            outs.append((y, dx))
+    finally:
+        h.set_order(None)
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+
+
 def _check_integer(wl, h, f, seed):
     """Integer X and G: the plain product A (handle h) bitwise exact on the
     sampled rows, forward and adjoint."""
@@ -245,8 +265,10 @@ def test_c5_mm_full_graph(cuda, alpha):
     h = mat.cusparse_extension
     assert h.plan_stats["fwd_pieces"] == 0 and h.plan_stats["bwd_pieces"] == 0
     _say(f"c5 alpha={alpha}: nnz(Delta)={h.nnz} depth={h.depth}")
+    assert h.order_active(torch.bfloat16, 128) and h.order_active(torch.float32, 32)
     for f in (32, 128):
         _check_products(wl, mat, "ax", f)
     _check_integer(wl, h, 32, 51 + alpha)
+    _check_order_bits(wl, mat, 128)
     del mat, h
     torch.cuda.empty_cache()
