@@ -48,6 +48,51 @@ def propagate(adjacency, h):
     return csr_apply(ext, h)
 
 
+# The weight gradient x^T g of a layer sums over the N rows; a single product
+# gives a 256 x 256 output a handful of workgroups, so for tall inputs the
+# rows are cut into WGRAD_SPLITS chunks whose products run as one batched
+# product and are then added (N = 169,343, 128 x 256 output: 1.03 -> 0.28 ms;
+# 256 x 256: 1.04 -> 0.49 ms; the sum is also closer to FP64, 5e-7 against
+# 3e-6 relative).  Inputs with fewer rows than WGRAD_SPLITS * 256 use one product.
+WGRAD_SPLITS = 64
+
+
+def weight_grad(x, g):
+    """``x^T g`` for tall ``x`` (N x k) and ``g`` (N x m), FP32."""
+    n = x.size(0)
+    c = n // WGRAD_SPLITS
+    if c < 256:
+        return x.t() @ g
+    m = c * WGRAD_SPLITS
+    xt = x[:m].reshape(WGRAD_SPLITS, c, x.size(1)).transpose(1, 2).contiguous()
+    gb = g[:m].reshape(WGRAD_SPLITS, c, g.size(1)).contiguous()
+    out = torch.bmm(xt, gb).sum(0)
+    if m < n:
+        out = out + x[m:].t() @ g[m:]
+    return out
+
+
+class Linear(torch.autograd.Function):
+    """``y = x W`` with the weight gradient of :func:`weight_grad`."""
+
+    @staticmethod
+    def forward(ctx, x, w):
+        ctx.save_for_backward(x, w)
+        return x @ w
+
+    @staticmethod
+    def backward(ctx, gy):
+        x, w = ctx.saved_tensors
+        gy = gy.contiguous()
+        dx = gy @ w.t() if ctx.needs_input_grad[0] else None
+        dw = weight_grad(x, gy) if ctx.needs_input_grad[1] else None
+        return dx, dw
+
+
+def linear(x, w):
+    return Linear.apply(x.contiguous(), w)
+
+
 def _axpy(coef, x, y):
     """y += coef * x (FP32, the library's row kernel)."""
     rc = _lib.hip().cbm_op_axpy(int(y.size(0)), int(y.size(1)), float(coef), x.data_ptr(), x.stride(0),
@@ -97,7 +142,7 @@ class GCNConv(torch.nn.Module):
         self.adjacency = adjacency
 
     def forward(self, x):
-        out = propagate(self.adjacency, x @ self.weight)
+        out = propagate(self.adjacency, linear(x, self.weight))
         return out + self.bias if self.bias is not None else out
 
 
@@ -135,5 +180,5 @@ class GCN(torch.nn.Module):
         return propagate(self.adjacency, h)
 
     def forward(self, x):
-        h = torch.relu(self.propagate(x @ self.w1))
-        return self.propagate(h @ self.w2)
+        h = torch.relu(self.propagate(linear(x, self.w1)))
+        return self.propagate(linear(h, self.w2))

@@ -221,3 +221,33 @@ def test_saved_order_loads_back(mats, cuda, tmp_path):
     assert torch.equal(y1, y2)
     m3 = cbm4mm.load(path, device=cuda, gather_order=False)
     assert getattr(m3.cusparse_extension, "_order", None) is None
+
+
+def test_product_over_a_in_gather_order(graph, cuda):
+    """The handle over A itself (small and mid-size graphs) gets an order of
+    its own rows: same bits as in position order."""
+    ei, csr = graph
+    m = cbm4dad(ei, torch.ones(ei.shape[1]), alpha=0, device=cuda, uncompressed=True, gather_order=True)
+    ext = m.cusparse_extension
+    plain = ext._plain
+    assert plain is not None and getattr(plain, "_order", None) is not None
+    ext.attach_uncompressed(plain, True)
+    plain.set_wave_rows(False)
+    assert ext.order_active(torch.float32, 128)
+    x = _x(N, 128, torch.float32, 41, cuda)
+
+    def run():
+        y = torch.empty(N, 128, device=cuda)
+        dx = torch.empty(N, 128, device=cuda)
+        m.matmul(x, y)
+        m.matmul_t(x, dx)
+        torch.cuda.synchronize()
+        return torch.cat([y, dx])
+
+    plain.set_order(True)
+    a = run()
+    plain.set_order(False)
+    b = run()
+    plain.set_order(True)
+    assert torch.equal(a, b)
+    assert O.max_rel_error(a[:N].cpu().numpy(), O.forward(csr, x.cpu().numpy(), "dadx")) <= 1e-5

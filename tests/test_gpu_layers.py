@@ -102,3 +102,31 @@ def test_gin_rejects_a_csr_adjacency(graph, cuda):
     ei, _ = graph
     with pytest.raises(TypeError, match="CBM kernels"):
         GINConv(csr4mm(ei, num_nodes=N, device=cuda), device=cuda)
+
+
+@pytest.mark.parametrize("n", [1000, 64 * 256 * 2 + 37])
+def test_weight_grad_against_fp64(cuda, n):
+    """x^T g by chunks of rows in one batched product (with the rows past the
+    last whole chunk added by themselves), or by one product for short x."""
+    from cbm.gcn import weight_grad
+    gen = torch.Generator(device=cuda).manual_seed(n)
+    x = torch.randn(n, 128, device=cuda, generator=gen)
+    g = torch.randn(n, 64, device=cuda, generator=gen)
+    ref = x.double().t() @ g.double()
+    got = weight_grad(x, g)
+    assert got.shape == (128, 64) and got.dtype == torch.float32
+    assert float((got.double() - ref).abs().max() / ref.abs().max()) <= 1e-5
+
+
+def test_linear_gradients_match_autograd_of_a_plain_product(cuda):
+    from cbm.gcn import linear
+    gen = torch.Generator(device=cuda).manual_seed(3)
+    x = torch.randn(40000, 32, device=cuda, generator=gen, dtype=torch.float64).float().requires_grad_(True)
+    w = torch.randn(32, 16, device=cuda, generator=gen).requires_grad_(True)
+    gy = torch.randn(40000, 16, device=cuda, generator=gen)
+    (linear(x, w) * gy).sum().backward()
+    x2 = x.detach().double().requires_grad_(True)
+    w2 = w.detach().double().requires_grad_(True)
+    (x2 @ w2 * gy.double()).sum().backward()
+    for a, b in ((x.grad, x2.grad), (w.grad, w2.grad)):
+        assert float((a.double() - b).abs().max() / b.abs().max()) <= 1e-5
