@@ -504,7 +504,10 @@ def gather_ceiling(name, w, m, args, dev, stream):
 
 def profile_steps(w, m, args, dev, stream):
     """The CBM step alone (BF16 then FP32 storage), for kernel traces and
-    counter runs: args.steps timed steps after args.warmup, per dtype."""
+    counter runs: args.steps timed steps after args.warmup, per dtype; then,
+    per dtype, the Delta pass alone (cbm_delta_spmm over X) timed the way the
+    bench times its phases (args.steps calls after args.warmup), so that the
+    trace's launches of that loop can be set beside the bench's figure."""
     import torch
     h = m.cusparse_extension
     n, f = w["n"], w["cols"]
@@ -513,9 +516,13 @@ def profile_steps(w, m, args, dev, stream):
     g32 = torch.randn(n, f, device=dev, generator=gen)
     y, dx, z = (torch.empty(n, f, device=dev) for _ in range(3))
     h.reserve(f)
-    for tag, xx, gg in (("bf16", x32.to(torch.bfloat16), g32.to(torch.bfloat16)), ("fp32", x32, g32)):
+    inputs = (("bf16", x32.to(torch.bfloat16), g32.to(torch.bfloat16)), ("fp32", x32, g32))
+    for tag, xx, gg in inputs:
         ms = time_loop(lambda: (h.forward(xx, y), h.matmul_t(gg, dx, workspace=z)), args.steps, args.warmup, stream)
         log(f"profile {tag}: {ms:.3f} ms per step ({args.steps} steps)")
+    for tag, xx, _ in inputs:
+        ms = time_loop(lambda: h.delta_spmm(xx, y), args.steps, args.warmup, stream)
+        log(f"profile {tag}: Delta pass alone {ms:.3f} ms per call ({args.steps} calls after {args.warmup})")
 
 
 def _csr_of(ei, n):
