@@ -281,7 +281,7 @@ def measure(name, w, ei, m, args, dev, stream):
                                     "fp32": round(time_loop(step(x32, g32), K, W, stream), 4)}
         h.set_symmetric(True)
     ordered = {"bf16": h.order_active(torch.bfloat16, f), "fp32": h.order_active(torch.float32, f)}
-    if getattr(h, "_order", None) is not None:
+    if getattr(h, "_order", None) is not None or getattr(getattr(h, "_plain", None), "_order", None) is not None:
         h.set_order(False)
         forms["position_order"] = {"bf16": round(time_loop(step(x16, g16), K, W, stream), 4),
                                    "fp32": round(time_loop(step(x32, g32), K, W, stream), 4)}

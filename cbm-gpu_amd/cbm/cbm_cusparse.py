@@ -228,9 +228,12 @@ class cusparseCBM:
 
     def set_order(self, mode):
         """Switches the gather order of the Delta pass: True, False or None
-        (automatic)."""
+        (automatic); an attached uncompressed handle's own order too."""
         self._order_mode = -1 if mode is None else int(bool(mode))
         _lib.check_hip(self._lib.cbm_set_order(self._h, self._order_mode), "set_order")
+        plain = getattr(self, "_plain", None)
+        if plain is not None and getattr(plain, "_order", None) is not None:
+            plain.set_order(mode)
 
     def order_active(self, dtype, n_cols):
         """Whether a forward product with dtype storage and n_cols columns
