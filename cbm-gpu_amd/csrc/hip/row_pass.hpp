@@ -229,20 +229,39 @@ __device__ inline void gather_src_impl(const RowPass &a, const TS *__restrict__ 
                     for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[u][q], sc[u], acc[q]);
             }
         }
-        for (; k < cnt; ++k) {
-            const int32_t raw = group_bcast<LPR>(mine, k);
-            const int32_t j = raw & kIndexMask;
-            float sc;
-            if constexpr (SCALED) {
-                sc = group_bcast<LPR>(mine_scale, k);
-            } else {
-                sc = signed_scale(1.0f, raw);
+        if (k < cnt) {
+            // The last one to three entries in one step, so that their
+            // gathers are in flight together: the slots past the row load
+            // nothing and add v = 0, which leaves acc's bits unchanged (acc
+            // starts at +0 and a sum of fma results is never -0).
+            int32_t j[4];
+            float sc[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int32_t src_lane = min(k + u, LPR - 1);
+                const int32_t raw = group_bcast<LPR>(mine, src_lane);
+                j[u] = raw & kIndexMask;
+                if constexpr (SCALED) {
+                    sc[u] = group_bcast<LPR>(mine_scale, src_lane);
+                } else {
+                    sc[u] = signed_scale(1.0f, raw);
+                }
             }
             if (active) {
-                float v[VE];
-                Load<TS, VE>::run(src + int64_t(j) * ld + c0, v);
+                float v[4][VE];
 #pragma unroll
-                for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[q], sc, acc[q]);
+                for (int u = 0; u < 4; ++u) {
+                    if (k + u < cnt) {
+                        Load<TS, VE>::run(src + int64_t(j[u]) * ld + c0, v[u]);
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < VE; ++q) v[u][q] = 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int q = 0; q < VE; ++q) acc[q] = fmaf(v[u][q], sc[u], acc[q]);
             }
         }
     }
