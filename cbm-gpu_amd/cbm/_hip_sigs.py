@@ -158,7 +158,7 @@ def declare(lib, sig):
     sig(lib, "cbm_set_fused", c_i32, c_vp, c_i32)
     sig(lib, "cbm_attach_uncompressed", c_i32, c_vp, c_vp, c_i32)
     sig(lib, "cbm_uncompressed_active", c_i32, c_vp, c_i32, c_i32)
-    sig(lib, "cbm_attach_order", c_i32, c_vp, c_i32, c_vp, c_i32)
+    sig(lib, "cbm_attach_order", c_i32, c_vp, c_i32, c_vp, c_i32, c_vp)
     sig(lib, "cbm_set_order", c_i32, c_vp, c_i32)
     sig(lib, "cbm_order_active", c_i32, c_vp, c_i32, c_i32)
     sig(lib, "cbm_fused_active", c_i32, c_vp, c_i32, c_i32)

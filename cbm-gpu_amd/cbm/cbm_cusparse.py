@@ -223,8 +223,10 @@ class cusparseCBM:
         order = getattr(self, "_order", None)
         if order is None:
             return
-        _lib.check_hip(self._lib.cbm_attach_order(self._h, int(order.numel()), self._order_dev.data_ptr(),
-                                                  self._order_mode), "attach_order")
+        with torch.cuda.device(self.device):
+            _lib.check_hip(self._lib.cbm_attach_order(self._h, int(order.numel()), self._order_dev.data_ptr(),
+                                                      self._order_mode, current_stream_handle(self.device)),
+                           "attach_order")
 
     def set_order(self, mode):
         """Switches the gather order of the Delta pass: True, False or None

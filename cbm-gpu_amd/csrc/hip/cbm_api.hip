@@ -65,6 +65,7 @@ struct cbm_handle {
     uint32_t *fused_flags = nullptr; // a flag per unit and 64-column slab
     int64_t fused_flag_count = 0;
     const int32_t *order = nullptr; // gather order of the short rows, borrowed (cbm_attach_order)
+    int32_t *order_meta = nullptr;  // per ordered row (position, first entry, end, output row), owned
     int32_t n_order = 0;
     int32_t order_mode = 0;    // Delta pass in the gather order: 1 on, 0 off, -1 automatic
     int last_launches = 0;
@@ -447,6 +448,7 @@ int delta_flat(cbm_handle *h, const Bufs &b, cbmk::RowPass rp, hipStream_t strea
         // The short rows in the gather order, XCD by XCD; the long rows by
         // their chunks as ever.
         rp.order = h->order;
+        rp.order_meta = h->order_meta;
         return run_range(h, rp, 0, h->n_order, 0, ck, rd, stream, what);
     }
     return run_range(h, rp, 0, h->d.n_rows, 0, ck, rd, stream, what);
@@ -838,7 +840,8 @@ extern "C" int cbm_destroy(cbm_handle *h) {
     hipError_t e = hipSuccess;
     for (void *p : {static_cast<void *>(h->ps_pos), static_cast<void *>(h->scratch), static_cast<void *>(h->comp),
                     static_cast<void *>(h->partial), static_cast<void *>(h->fused_state),
-                    static_cast<void *>(h->fused_flags), static_cast<void *>(h->tf_order)}) {
+                    static_cast<void *>(h->fused_flags), static_cast<void *>(h->tf_order),
+                    static_cast<void *>(h->order_meta)}) {
         if (!p) continue;
         const hipError_t ei = hipFree(p);
         if (e == hipSuccess) e = ei;
@@ -943,14 +946,32 @@ extern "C" int cbm_uncompressed_active(cbm_handle *h, int32_t dtype, int32_t n_c
     return route(h, dtype, n_cols) != h ? 1 : 0;
 }
 
-extern "C" int cbm_attach_order(cbm_handle *h, int32_t n_order, const int32_t *order, int32_t mode) {
+extern "C" int cbm_attach_order(cbm_handle *h, int32_t n_order, const int32_t *order, int32_t mode,
+                                hipStream_t stream) {
     if (!h || n_order < 0 || n_order > h->d.n_rows || mode < -1 || mode > 1 || (n_order > 0 && !order)) {
         set_error("cbm_attach_order: invalid arguments (n_order=%d mode=%d)", n_order, mode);
         return CBM_ERR_ARG;
     }
-    h->order = n_order > 0 ? order : nullptr;
+    if (h->order_meta) {
+        const hipError_t e = hipFree(h->order_meta);
+        h->order_meta = nullptr;
+        if (e != hipSuccess) return device_error(e, "cbm_attach_order");
+    }
+    h->order = nullptr;
+    h->n_order = 0;
+    h->order_mode = 0;
+    if (n_order == 0) return CBM_OK;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&h->order_meta), size_t(n_order) * 4 * sizeof(int32_t));
+    if (e != hipSuccess) {
+        h->order_meta = nullptr;
+        return device_error(e, "cbm_attach_order");
+    }
+    e = cbmk::launch_order_meta(n_order, order, h->d.n_rows, h->d.fwd_ptr, h->d.lvl_node, h->order_meta, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return device_error(e, "cbm_attach_order");
+    h->order = order;
     h->n_order = n_order;
-    h->order_mode = n_order > 0 ? mode : 0;
+    h->order_mode = mode;
     return CBM_OK;
 }
 

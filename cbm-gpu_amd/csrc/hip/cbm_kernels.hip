@@ -525,6 +525,27 @@ hipError_t launch_axpy(int32_t n_rows, int32_t ncols, float coef, const float *x
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void k_order_meta(int32_t n, const int32_t *__restrict__ order, int32_t n_rows,
+                                                    const int32_t *__restrict__ ptr,
+                                                    const int32_t *__restrict__ node_of, int4 *__restrict__ meta) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t p = order[i];
+    if (p < 0 || p >= n_rows) {
+        meta[i] = make_int4(-1, 0, 0, 0);
+        return;
+    }
+    meta[i] = make_int4(p, ptr[p], ptr[p + 1], node_of ? node_of[p] : p);
+}
+
+hipError_t launch_order_meta(int32_t n, const int32_t *order, int32_t n_rows, const int32_t *ptr,
+                             const int32_t *node_of, int32_t *meta, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    k_order_meta<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream>>>(n, order, n_rows, ptr, node_of,
+                                                                            reinterpret_cast<int4 *>(meta));
+    return hipGetLastError();
+}
+
 hipError_t launch_dad_scales(int32_t n, const int32_t *node_of, const int32_t *parent_of, const float *d, float *ps,
                              hipStream_t stream) {
     if (n <= 0) return hipSuccess;

@@ -103,7 +103,17 @@ struct RowPass {
     // contiguous eighth, so each XCD walks one stretch of the order.
     const int32_t *order = nullptr;
     int32_t order_blocks = 0;
+    // With an order, optionally the rows' (position, first entry, end, output
+    // row) in order: int32 [4 x rows], 16-byte aligned, one load per row
+    // instead of the chain order -> ptr -> node_of.  A position below 0 is
+    // skipped.  Only for passes without an extra term or a dual source.
+    const int32_t *order_meta = nullptr;
 };
+
+// order_meta[i] = (order[i], ptr[order[i]], ptr[order[i] + 1], node_of[order[i]])
+// for i < n (positions outside [0, n_rows) give (-1, 0, 0, 0)).
+hipError_t launch_order_meta(int32_t n, const int32_t *order, int32_t n_rows, const int32_t *ptr,
+                             const int32_t *node_of, int32_t *meta, hipStream_t stream);
 
 // Wave slots of the device (256 CUs x 4 SIMDs x 8 waves): a pass with fewer
 // rows x lanes than 64 x this leaves slots free.

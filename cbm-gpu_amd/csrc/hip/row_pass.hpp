@@ -409,11 +409,22 @@ __device__ __forceinline__ void row_pass_block(const RowPass &a, uint32_t bid) {
                           lane / kLanesPerRow;
     const int64_t p64 = int64_t(p_begin) + group;
     if (p64 >= int64_t(p_end)) return;  // a row's lanes leave together
-    const int32_t p = a.order ? a.order[p64] : int32_t(p64);
-    if (p < 0) return;
-    const int32_t kb = a.ptr[p], ke = a.ptr_end ? a.ptr_end[p] : a.ptr[p + 1];
+    int32_t p, kb, ke, row;
+    if (a.order_meta) {
+        const int4 mt = reinterpret_cast<const int4 *>(a.order_meta)[p64];
+        p = mt.x;
+        kb = mt.y;
+        ke = mt.z;
+        row = mt.w;
+        if (p < 0) return;
+    } else {
+        p = a.order ? a.order[p64] : int32_t(p64);
+        if (p < 0) return;
+        kb = a.ptr[p];
+        ke = a.ptr_end ? a.ptr_end[p] : a.ptr[p + 1];
+        row = a.node_of ? a.node_of[p] : p;
+    }
     if (ke - kb > a.skip_above) return;  // a long row, done by the chunk kernel
-    const int32_t row = a.node_of ? a.node_of[p] : p;
     const int64_t len = int64_t(ke) - kb;
     const int32_t sb = kb + int32_t(len * sub / S), se = kb + int32_t(len * (sub + 1) / S);
     const int32_t cb = slab * a.slab_w, ce = min(cb + a.slab_w, a.ncols);

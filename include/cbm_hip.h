@@ -454,10 +454,13 @@ int cbm_uncompressed_active(cbm_handle *h, int32_t dtype, int32_t n_cols);
  * together on one L2; each row is summed as before, so the results are the
  * same bits.  Not used with XCD column ranges, column slabs or the
  * one-launch forward.  mode: 1 on, 0 off, -1 automatic (the same as on).
- * n_order = 0 detaches.  No counterpart in the reference, whose Delta SpMM
- * is one cusparseSpMM call (cbm/cbm.cu:184-189).
+ * n_order = 0 detaches.  The handle keeps, per ordered row, its position,
+ * entry range and output row in device memory it allocates here (16 bytes a
+ * row, one load per row in the pass), computed on `stream`; the call
+ * synchronises the stream.  No counterpart in the reference, whose Delta
+ * SpMM is one cusparseSpMM call (cbm/cbm.cu:184-189).
  */
-int cbm_attach_order(cbm_handle *h, int32_t n_order, const int32_t *order, int32_t mode);
+int cbm_attach_order(cbm_handle *h, int32_t n_order, const int32_t *order, int32_t mode, hipStream_t stream);
 int cbm_set_order(cbm_handle *h, int32_t mode);
 
 /* Whether a forward product of n_cols columns in dtype storage runs its
