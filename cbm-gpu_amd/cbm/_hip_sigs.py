@@ -167,6 +167,7 @@ def declare(lib, sig):
     sig(lib, "cbm_set_forward_split", c_i32, c_vp, c_i32)
     sig(lib, "cbm_set_wave_rows", c_i32, c_vp, c_i32)
     sig(lib, "cbm_set_mixed_grid", c_i32, c_vp, c_i32)
+    sig(lib, "cbm_set_chunk_finish", c_i32, c_vp, c_i32)
     sig(lib, "cbm_set_xcd_slabs", c_i32, c_vp, c_i32)
     sig(lib, "cbm_set_xcd_ranges", c_i32, c_vp, c_i32)
     sig(lib, "csr_last_error", ctypes.c_char_p)
@@ -200,7 +201,7 @@ def declare(lib, sig):
 EXPORTED = ("cbm_last_error", "cbm_desc_size", "cbm_create", "cbm_destroy", "cbm_reserve", "cbm_matmul", "cbm_matmul_noalloc", "cbm_matmul_t", "cbm_adjoint_reverse",
             "cbm_adjoint_delta_t", "cbm_delta_spmm",
             "cbm_propagate", "cbm_last_launches", "cbm_set_slab_bytes", "cbm_xcd_ranges_active", "cbm_set_xcd_phases", "cbm_set_tree_tails", "cbm_set_symmetric", "cbm_set_fused", "cbm_attach_uncompressed", "cbm_uncompressed_active", "cbm_attach_order", "cbm_set_order", "cbm_order_active", "cbm_fused_active", "cbm_fused_errors", "cbm_set_kahan_depth",
-            "cbm_set_forward_split", "cbm_set_wave_rows", "cbm_set_mixed_grid", "cbm_set_xcd_slabs", "cbm_set_xcd_ranges", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
+            "cbm_set_forward_split", "cbm_set_wave_rows", "cbm_set_mixed_grid", "cbm_set_chunk_finish", "cbm_set_xcd_slabs", "cbm_set_xcd_ranges", "csr_create", "csr_destroy", "csr_matmul", "csr_set_algorithm",
             "csr_last_error", "cbm_op_spmm", "cbm_op_spmm_t_workspace_bytes", "cbm_op_spmm_t", "cbm_op_update", "cbm_op_axpy", "cbm_op_last_error",
             "cbm_candidates_workspace_bytes", "cbm_candidates_count", "cbm_candidates_emit",
             "cbm_partners_workspace_bytes", "cbm_partners",

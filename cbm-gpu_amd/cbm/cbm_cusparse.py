@@ -481,6 +481,13 @@ class cusparseCBM:
         m = -1 if mode is None else int(bool(mode))
         _lib.check_hip(self._lib.cbm_set_mixed_grid(self._h, m), "set_mixed_grid")
 
+    def set_chunk_finish(self, mode):
+        """Rows of several long-row chunks: True finished by their last chunk
+        inside the gather pass, False by a reduction launch after it, None by
+        matrix size (the default).  Same results."""
+        m = -1 if mode is None else int(bool(mode))
+        _lib.check_hip(self._lib.cbm_set_chunk_finish(self._h, m), "set_chunk_finish")
+
     def set_xcd_ranges(self, mode):
         """XCD column ranges for the gather passes: None automatic (by size),
         False off, True on (when the layout has range pieces; the piece set

@@ -68,6 +68,9 @@ struct cbm_handle {
     int32_t *order_meta = nullptr;  // per ordered row (position, first entry, end, output row), owned
     int32_t n_order = 0;
     int32_t order_mode = 0;    // Delta pass in the gather order: 1 on, 0 off, -1 automatic
+    uint32_t *arrive = nullptr; // a counter per long-row scratch slot, zero between launches (cbm_reserve)
+    int64_t arrive_count = 0;
+    int32_t chunk_finish = -1; // multi-chunk rows finished by their last chunk: 1 on, 0 off, -1 by size
     int last_launches = 0;
 };
 
@@ -257,12 +260,36 @@ int ensure_fused(cbm_handle *h, int32_t n_cols) {
     return CBM_OK;
 }
 
+// The arrival counters of the long-row chunks (one per scratch slot, zero
+// when allocated; each launch leaves them zero).
+int ensure_arrive(cbm_handle *h, int64_t slots) {
+    if (slots <= h->arrive_count) return CBM_OK;
+    if (h->arrive) {
+        const hipError_t e = hipFree(h->arrive);
+        h->arrive = nullptr;
+        h->arrive_count = 0;
+        if (e != hipSuccess) return device_error(e, "cbm chunk counters");
+    }
+    hipError_t e = hipMalloc(&h->arrive, sizeof(uint32_t) * size_t(slots));
+    if (e == hipSuccess) e = hipMemset(h->arrive, 0, sizeof(uint32_t) * size_t(slots));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        if (h->arrive) (void)hipFree(h->arrive);
+        h->arrive = nullptr;
+        return device_error(e, "cbm chunk counters");
+    }
+    h->arrive_count = slots;
+    return CBM_OK;
+}
+
 int grow_buffers(cbm_handle *h, int32_t n_cols) {
     int rc = ensure_fused(h, n_cols);
     if (rc == CBM_OK) rc = ensure_comp(h, n_cols);
     if (rc == CBM_OK) rc = ensure_partial(h, n_cols);
     if (rc != CBM_OK) return rc;
     const int64_t slots = std::max<int64_t>({h->d.fwd_slots, h->d.bwd_slots, h->d.fwd_gslots, h->d.rev_slots});
+    rc = ensure_arrive(h, slots);
+    if (rc != CBM_OK) return rc;
     const int64_t need = slots * int64_t(n_cols);
     if (need <= h->scratch_floats) return CBM_OK;
     if (h->scratch) {
@@ -337,13 +364,34 @@ int noalloc_scratch_address(float **out) {
     return CBM_OK;
 }
 
+// By default a pass finishes its multi-chunk rows in the chunk kernel when
+// the matrix has at most kChunkFinishMaxRows rows and the pass at most
+// kChunkFinishMaxLong such rows: there the reduction's launch is a large part
+// of a short pass (N = 19,717, F = 128: 22 -> 19 us per product).  Each
+// arrival releases its wave's stores at device scope and each last chunk
+// acquires, which invalidates its XCD's L2; with many multi-chunk rows that
+// costs the L2 hits of the gathers (N = 132,534 with 80M entries, F = 64:
+// 0.61 -> 1.11 ms per product), so larger passes keep the reduction launch.
+constexpr int32_t kChunkFinishMaxRows = 1 << 18;
+constexpr int32_t kChunkFinishMaxLong = 64;
+
+// Whether a range's n_multi multi-chunk rows are finished by their last
+// chunk (ChunkArgs::arrive) instead of a reduction launch.
+bool chunk_finish_on(const cbm_handle *h, const cbmk::RowPass &rp, int64_t n_multi) {
+    if (!h->arrive || rp.xcd_slabs != 1 || rp.ranges || h->chunk_finish == 0) return false;
+    return h->chunk_finish == 1 || (h->d.n_rows <= kChunkFinishMaxRows && n_multi <= kChunkFinishMaxLong);
+}
+
 // One ordered row range whose first n_long rows are long: the chunk kernel
-// and the short-row pass (one grid), then the reduction of multi-chunk rows.
+// and the short-row pass (one grid), then the reduction of multi-chunk rows
+// (or none: their last chunks finish them).
 int run_range(cbm_handle *h, cbmk::RowPass rp, int32_t p_begin, int32_t p_end, int32_t n_long,
-              const cbmk::ChunkArgs &ck, const cbmk::ReduceArgs &rd, hipStream_t stream, const char *what) {
+              cbmk::ChunkArgs ck, const cbmk::ReduceArgs &rd, hipStream_t stream, const char *what) {
     hipError_t e = hipSuccess;
     rp.p_begin = p_begin + n_long;
     rp.p_end = p_end;
+    const bool finish = rd.count > 0 && ck.count > 0 && chunk_finish_on(h, rp, rd.count);
+    ck.arrive = finish ? h->arrive : nullptr;
     if (ck.count > 0) {
         int launched = 0;
         e = cbmk::launch_row_mixed(rp, ck, &launched, stream);
@@ -354,7 +402,7 @@ int run_range(cbm_handle *h, cbmk::RowPass rp, int32_t p_begin, int32_t p_end, i
         if (e != hipSuccess) return device_error(e, what);
         ++h->last_launches;
     }
-    if (rd.count > 0) {
+    if (rd.count > 0 && !finish) {
         e = cbmk::launch_row_reduce(rp, rd, stream);
         if (e != hipSuccess) return device_error(e, what);
         ++h->last_launches;
@@ -841,7 +889,7 @@ extern "C" int cbm_destroy(cbm_handle *h) {
     for (void *p : {static_cast<void *>(h->ps_pos), static_cast<void *>(h->scratch), static_cast<void *>(h->comp),
                     static_cast<void *>(h->partial), static_cast<void *>(h->fused_state),
                     static_cast<void *>(h->fused_flags), static_cast<void *>(h->tf_order),
-                    static_cast<void *>(h->order_meta)}) {
+                    static_cast<void *>(h->order_meta), static_cast<void *>(h->arrive)}) {
         if (!p) continue;
         const hipError_t ei = hipFree(p);
         if (e == hipSuccess) e = ei;
@@ -870,11 +918,15 @@ namespace {
 
 // Launches of one forward product over this handle's layout (split form):
 // the Delta pass (one grid, two where chunks and short rows stay apart), the
-// reduction of multi-chunk rows, and the propagation (per-level launches down
+// reduction of multi-chunk rows unless their last chunks finish them, and the
+// propagation (per-level launches down
 // to the tree tail, then the tail).
 int64_t forward_launches(const cbm_handle *h) {
     const int32_t L = h->d.n_levels;
-    int64_t n = 1 + (h->fred_ptr[size_t(L)] > 0 ? 1 : 0);
+    const int64_t n_multi = h->fred_ptr[size_t(L)];
+    const bool finish = h->arrive && h->xcd_slabs <= 1 && h->chunk_finish != 0 &&
+                        (h->chunk_finish == 1 || (h->d.n_rows <= kChunkFinishMaxRows && n_multi <= kChunkFinishMaxLong));
+    int64_t n = 1 + (n_multi > 0 && !finish ? 1 : 0);
     if (L > 1) {
         const bool tail = tails_on(h) && h->d.tf_level < L;
         n += (tail ? h->d.tf_level - 1 : L - 1) + (tail ? tail_launches(forward_tail(h)) : 0);
@@ -1395,6 +1447,15 @@ extern "C" int cbm_set_mixed_grid(cbm_handle *h, int32_t mode) {
         return CBM_ERR_ARG;
     }
     h->mixed_grid = mode;
+    return CBM_OK;
+}
+
+extern "C" int cbm_set_chunk_finish(cbm_handle *h, int32_t mode) {
+    if (!h || mode < -1 || mode > 1) {
+        set_error("cbm_set_chunk_finish: invalid arguments");
+        return CBM_ERR_ARG;
+    }
+    h->chunk_finish = mode;
     return CBM_OK;
 }
 

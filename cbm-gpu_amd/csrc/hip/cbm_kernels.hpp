@@ -125,12 +125,17 @@ hipError_t launch_row_pass(const RowPass &rp, hipStream_t stream);
 // Chunks [first, first + count) of the long rows of a row pass: chunk i covers
 // part meta & 0xffff of meta >> 16 equal parts of row pos[i]'s entries; its sum
 // finishes the row (slot[i] < 0) or goes to scratch row slot[i] (ld = ncols).
+// With arrive (one zeroed counter per scratch slot, xcd_slabs 1 only), the
+// last chunk of a multi-chunk row to finish adds the row's scratch rows in
+// slot order and finishes the row, as launch_row_reduce would (same bits),
+// and sets the counter back to zero; no reduction launch follows.
 struct ChunkArgs {
     const int32_t *pos = nullptr;
     const int32_t *meta = nullptr;
     const int32_t *slot = nullptr;
     int32_t first = 0, count = 0;
     float *scratch = nullptr;
+    uint32_t *arrive = nullptr;
 };
 hipError_t launch_row_chunks(const RowPass &rp, const ChunkArgs &c, hipStream_t stream);
 

@@ -502,6 +502,42 @@ __global__ __launch_bounds__(256) void k_row_tail(const RowPass a, const TailArg
     }
 }
 
+// Row p (output row `row`) is the sum of scratch rows s0 .. s0 + ns - 1 (ld =
+// ncols), added in slot order (0 when ns is 0), then finished; by the LPR
+// lanes of one group, every column.
+template <class P, int VE, int LPR>
+__device__ inline void sum_slots_finish(const RowPass &a, const float *__restrict__ scratch, int32_t p, int32_t row,
+                                        int32_t s0, int32_t ns, int l) {
+    for (int32_t c0 = l * VE; c0 < a.ncols; c0 += LPR * VE) {
+        float acc[VE];
+        if (ns > 0) {
+            Load<float, VE>::run(scratch + int64_t(s0) * a.ncols + c0, acc);
+        } else {
+#pragma unroll
+            for (int q = 0; q < VE; ++q) acc[q] = 0.0f;
+        }
+        int32_t s = 1;
+        // Four slot loads in flight, added in slot order (the same bits as one
+        // at a time).
+        for (; s + 4 <= ns; s += 4) {
+            float v[4][VE];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) Load<float, VE>::run(scratch + int64_t(s0 + s + u) * a.ncols + c0, v[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int q = 0; q < VE; ++q) acc[q] += v[u][q];
+        }
+        for (; s < ns; ++s) {
+            float v[VE];
+            Load<float, VE>::run(scratch + int64_t(s0 + s) * a.ncols + c0, v);
+#pragma unroll
+            for (int q = 0; q < VE; ++q) acc[q] += v[q];
+        }
+        finish_row<P, VE>(a, p, row, c0, acc);
+    }
+}
+
 // Long rows: one workgroup of BS threads per chunk of a row.  The chunk's
 // entries are split into contiguous slices, one per group of LPR lanes; the
 // groups' partial sums are added in group order through LDS.  A one-chunk
@@ -548,6 +584,24 @@ __device__ __forceinline__ void row_chunk_block(const RowPass &a, const ChunkArg
         }
         __syncthreads();
     }
+    if (c.arrive != nullptr && slot >= 0 && threadIdx.x < 64) {
+        // Wave 0 holds group 0, which stored this chunk's sums: it releases
+        // them at device scope and counts the chunk in; the row's last chunk
+        // to arrive acquires the others' sums, adds all of them in slot order
+        // and finishes the row, then clears the counter for the next launch
+        // (no other chunk of this launch touches it after its own arrival).
+        const int32_t first = slot - int32_t(idx);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        uint32_t before = 0;
+        if (threadIdx.x == 0)
+            before = __hip_atomic_fetch_add(c.arrive + first, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        before = uint32_t(__builtin_amdgcn_readfirstlane(int32_t(before)));
+        if (before == uint32_t(cnt - 1)) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (threadIdx.x == 0) __hip_atomic_store(c.arrive + first, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (g == 0) sum_slots_finish<P, VE, LPR>(a, c.scratch, p, row, first, int32_t(cnt), l);
+        }
+    }
 }
 
 template <class P, int VE, int LPR, int BS>
@@ -582,35 +636,11 @@ __global__ __launch_bounds__(256) void k_row_reduce(const RowPass a, const Reduc
     const int32_t i = r.first + int32_t(i64);
     const int32_t p = r.pos ? r.pos[i] : i, s0 = r.slot[i], ns = r.cnt[i];
     const int32_t row = a.node_of ? a.node_of[p] : p;
-    for (int32_t c0 = l * VE; c0 < a.ncols; c0 += LPR * VE) {
-        float acc[VE];
-        if (ns > 0) {
-            Load<float, VE>::run(r.scratch + int64_t(s0) * a.ncols + c0, acc);
-        } else {
-#pragma unroll
-            for (int q = 0; q < VE; ++q) acc[q] = 0.0f;
-        }
-        int32_t s = 1;
-        // Four slot loads in flight, added in slot order (the same bits as one
-        // at a time).
-        for (; s + 4 <= ns; s += 4) {
-            float v[4][VE];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) Load<float, VE>::run(r.scratch + int64_t(s0 + s + u) * a.ncols + c0, v[u]);
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int q = 0; q < VE; ++q) acc[q] += v[u][q];
-        }
-        for (; s < ns; ++s) {
-            float v[VE];
-            Load<float, VE>::run(r.scratch + int64_t(s0 + s) * a.ncols + c0, v);
-#pragma unroll
-            for (int q = 0; q < VE; ++q) acc[q] += v[q];
-        }
-        finish_row<P, VE>(a, p, row, c0, acc);
-    }
+    sum_slots_finish<P, VE, LPR>(a, r.scratch, p, row, s0, ns, l);
 }
+
+// Matrices of up to this many rows take the other lane width (dispatch_ve).
+constexpr int32_t kAltLaneMaxRows = 65536;
 
 inline int lanes_per_row(int32_t ncols, int ve) {
     const int32_t need = (ncols + ve - 1) / ve;
@@ -782,7 +812,23 @@ hipError_t dispatch_ve(const RowPass &rp, const Launch &lk, hipStream_t stream) 
     if (P::kExtra != kExtraNone) vec = vec && aligned16(rp.e, rp.lde, int(sizeof(typename P::TE)));
     if (P::kDual) vec = vec && aligned16(rp.g2, rp.ldg2, int(sizeof(typename P::TG2)));
     if (P::kExtra == kExtraParentKahan) vec = vec && aligned16(rp.comp, rp.ldc, 4);
-    const int ve = vec ? kVe : 1;
+    // The other lane width of a single-source pass over a small matrix
+    // (kAlt columns per lane, 0 for none), where the pass is bound by its
+    // rows' chains of dependent gathers rather than by bandwidth: FP32 rows
+    // of 64 or more columns take 32 bytes per lane, so a row has half the
+    // lanes and a wave twice the rows (with a wavefront per row, each row's
+    // entries split over twice the sub-groups: chains half as long); BF16
+    // rows narrower than 64 bytes take 8 bytes per lane, so that such a row
+    // keeps four lanes.  N = 19,717: FP32 F = 128 step 0.074 -> 0.043 ms,
+    // BF16 F = 16 0.034 -> 0.025 ms.  On matrices that fill the device the
+    // usual widths measured faster (N = 169,343, FP32 F = 128: 0.268 against
+    // 0.291 ms).
+    constexpr int kAlt = P::kDual ? 0 : (sizeof(TG) == 4 ? 8 : 4);
+    const bool small = rp.rows_hint > 0 && rp.rows_hint <= kAltLaneMaxRows;
+    bool alt = false;
+    if constexpr (kAlt == 8) alt = small && vec && rp.ncols % 8 == 0 && rp.ncols >= 64;
+    if constexpr (kAlt == 4) alt = small && vec && rp.ncols < 32;
+    const int ve = vec ? (alt ? kAlt : kVe) : 1;
     // Slab width: a multiple of the vector width and at least 16 vectors, so
     // that every slab keeps 16 or more lanes per row.  Rows that wide (and
     // the unslabbed rows, which are wider) give the chunk kernel 16 groups
@@ -803,6 +849,9 @@ hipError_t dispatch_ve(const RowPass &rp, const Launch &lk, hipStream_t stream) 
         r.xcd_slabs = (rp.ncols + w - 1) / w;
     }
     r.slab_w = w;
+    if constexpr (kAlt > 0) {
+        if (alt) return dispatch_lpr<P, kAlt>(r, lk, lanes_per_row(w, kAlt), stream);
+    }
     if (vec) return dispatch_lpr<P, kVe>(r, lk, lanes_per_row(w, kVe), stream);
     return dispatch_lpr<P, 1>(r, lk, lanes_per_row(w, 1), stream);
 }

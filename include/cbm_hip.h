@@ -343,6 +343,20 @@ int cbm_set_wave_rows(cbm_handle *h, int32_t mode);
 int cbm_set_mixed_grid(cbm_handle *h, int32_t mode);
 
 /*
+ * Long rows of more than one chunk: 1 has the last chunk of each such row to
+ * finish (an arrival counter per scratch slot, allocated by cbm_reserve) add
+ * the row's chunk sums and finish it inside the gather pass, 0 adds them in
+ * a reduction launch after the pass, -1 (the default) the first for passes
+ * with at most 64 such rows, over matrices of up to 262,144 rows (the
+ * last chunk's device-scope acquire invalidates its XCD's L2, which costs a
+ * pass with many such rows its cache hits).  Column slabs and XCD ranges
+ * always use the reduction launch.  Both forms add the chunk sums in chunk
+ * order: the same bits.  No counterpart in
+ * the reference (its Delta SpMM is one cusparseSpMM call, cbm/cbm.cu:184-189).
+ */
+int cbm_set_chunk_finish(cbm_handle *h, int32_t mode);
+
+/*
  * XCD feature slabs (1..64, default 1 = off): the columns of the gathered
  * matrix are cut into up to `slabs` slabs and workgroup b of a gather pass
  * works on slab b % slabs.  MI355X dispatches workgroups round-robin over

@@ -150,6 +150,46 @@ def test_one_grid_and_two_launches_are_bit_identical(hub, cuda, mode, dtype, f):
 
 
 @pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("f", [8, 64, 256, 300])
+def test_chunk_finish_in_pass_and_by_reduction_are_bit_identical(hub, cuda, mode, dtype, f):
+    """Rows of several long-row chunks are finished either by their last
+    chunk inside the gather pass (arrival counters) or by a reduction launch
+    after it; both add the chunk sums in chunk order, so the forward, the
+    Delta half, the adjoint's reverse and Delta^T passes give the same bits,
+    call after call (the counters are left at zero by every launch), and
+    the results match the oracle.  F = 300 takes two column passes per row."""
+    a, mats = hub
+    h = mats[mode].cusparse_extension
+    xt = torch.from_numpy(_x((HUB_N, f), 41 + f)).to(cuda).to(dtype)
+    res = {}
+    for finish in (True, False, True):
+        h.set_chunk_finish(finish)
+        out = []
+        for _ in range(2):
+            y = torch.full((HUB_N, f), float("nan"), device=cuda)
+            yd = torch.full((HUB_N, f), float("nan"), device=cuda)
+            dx = torch.full((HUB_N, f), float("nan"), device=cuda)
+            h.forward(xt, y)
+            h.delta_spmm(xt, yd)
+            h.matmul_t(xt, dx)
+            out.append((y, yd, dx))
+        for k in range(3):
+            assert torch.equal(out[0][k], out[1][k]), (mode, f, finish, k)
+        if finish in res:
+            for k in range(3):
+                assert torch.equal(res[finish][k], out[0][k]), (mode, f, finish, k)
+        res[finish] = out[0]
+    for k in range(3):
+        assert torch.equal(res[True][k], res[False][k]), (mode, f, k)
+    xf = xt.float().cpu().numpy()
+    assert O.max_rel_error(res[True][0].cpu().numpy(), O.forward(a, xf, mode)) <= 1e-5
+    assert O.max_rel_error(res[True][2].cpu().numpy(), O.adjoint(a, xf, mode)) <= 1e-5
+    assert h._lib.cbm_set_chunk_finish(h._h, 2) != 0  # out of range: rejected
+    h.set_chunk_finish(None)
+
+
+@pytest.mark.parametrize("mode", ["ax", "adx", "dadx"])
 @pytest.mark.parametrize("dtype,f", [(torch.float32, 1), (torch.float32, 3), (torch.float32, 4), (torch.float32, 8),
                                      (torch.float32, 64), (torch.bfloat16, 16), (torch.bfloat16, 128)])
 def test_split_and_fused_forward_are_bit_identical(hub, cuda, mode, dtype, f):
