@@ -536,3 +536,48 @@ def test_tree_tails(hub, cuda, dtype, f):
     dx = torch.empty(HUB_N, 24, device=cuda)
     h.matmul_t(xt, dx)
     assert np.array_equal(dx.cpu().numpy().astype(np.float64), O.adjoint(a, xi, "ax"))
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_chunk_finish_replays_from_a_graph(hub, cuda, dtype):
+    """The in-pass finish of multi-chunk rows leaves its arrival counters at
+    zero after every launch, so a HIP graph of the forward and the adjoint
+    replays correctly again and again (no memset node in the graph): each
+    replay, with eager calls and synchronisations between replays, gives the
+    eager bits."""
+    a, mats = hub
+    h = mats["dadx"].cusparse_extension
+    h.set_chunk_finish(True)
+    f = 64
+    h.reserve(f)
+    x = torch.from_numpy(_x((HUB_N, f), 91)).to(cuda).to(dtype)
+    y = torch.empty(HUB_N, f, device=cuda)
+    dx = torch.empty(HUB_N, f, device=cuda)
+    z = torch.empty(HUB_N, f, device=cuda)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        h.forward(x, y)
+        h.matmul_t(x, dx, workspace=z)
+    torch.cuda.current_stream().wait_stream(s)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        h.forward(x, y)
+        h.matmul_t(x, dx, workspace=z)
+    for it in range(4):
+        x.copy_(torch.from_numpy(_x((HUB_N, f), 92 + it)).to(cuda).to(dtype))
+        y_ref = torch.empty_like(y)
+        dx_ref = torch.empty_like(dx)
+        h.forward(x, y_ref)
+        h.matmul_t(x, dx_ref)
+        torch.cuda.synchronize()
+        y.fill_(float("nan"))
+        dx.fill_(float("nan"))
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y, y_ref), it
+        assert torch.equal(dx, dx_ref), it
+    xf = x.float().cpu().numpy()
+    assert O.max_rel_error(y.cpu().numpy(), O.forward(a, xf, "dadx")) <= 1e-5
+    assert O.max_rel_error(dx.cpu().numpy(), O.adjoint(a, xf, "dadx")) <= 1e-5
+    h.set_chunk_finish(None)
