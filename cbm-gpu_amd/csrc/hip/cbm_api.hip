@@ -71,6 +71,15 @@ struct cbm_handle {
     uint32_t *arrive = nullptr; // a counter per long-row scratch slot, zero between launches (cbm_reserve)
     int64_t arrive_count = 0;
     int32_t chunk_finish = -1; // multi-chunk rows finished by their last chunk: 1 on, 0 off, -1 by size
+    // Small matrices: the forward Delta pass's own long-row tables (rows of
+    // more than kSmallLongRow entries in chunks of about kSmallChunk), by
+    // position, with slots numbered over all levels; one owned allocation.
+    bool sc_on = false;
+    int32_t *sc_buf = nullptr;
+    const int32_t *sc_ck_pos = nullptr, *sc_ck_meta = nullptr, *sc_ck_slot = nullptr;
+    const int32_t *sc_red_pos = nullptr, *sc_red_slot = nullptr, *sc_red_cnt = nullptr;
+    std::vector<int32_t> sc_ck_lvl, sc_red_lvl;  // first chunk / reduction of each level (L + 1)
+    int32_t sc_slots = 0;
     int last_launches = 0;
 };
 
@@ -287,7 +296,8 @@ int grow_buffers(cbm_handle *h, int32_t n_cols) {
     if (rc == CBM_OK) rc = ensure_comp(h, n_cols);
     if (rc == CBM_OK) rc = ensure_partial(h, n_cols);
     if (rc != CBM_OK) return rc;
-    const int64_t slots = std::max<int64_t>({h->d.fwd_slots, h->d.bwd_slots, h->d.fwd_gslots, h->d.rev_slots});
+    const int64_t slots =
+        std::max<int64_t>({h->d.fwd_slots, h->d.bwd_slots, h->d.fwd_gslots, h->d.rev_slots, h->sc_slots});
     rc = ensure_arrive(h, slots);
     if (rc != CBM_OK) return rc;
     const int64_t need = slots * int64_t(n_cols);
@@ -361,6 +371,78 @@ int noalloc_scratch_address(float **out) {
         g_scratch_addr[dev] = static_cast<float *>(p);
     }
     *out = g_scratch_addr[dev];
+    return CBM_OK;
+}
+
+// Matrices of up to kSmallChunkRows rows run their forward Delta pass with
+// long-row tables of their own: rows of more than kSmallLongRow entries (the
+// plan's limit is CBM_LONG_ROW) in chunks of about kSmallChunk (CBM_CHUNK).
+// Such a pass does not fill the device, and its time is its longest chain of
+// dependent gathers: a row of 256 entries on one wavefront, or a 1024-entry
+// chunk, is 16 steps of a lane group, a 256-entry chunk 4.  N = 19,717,
+// F = 128: step 0.0377 -> 0.0306 ms BF16.  The adjoint's reverse and Delta^T
+// passes keep the plan's tables.
+constexpr int32_t kSmallChunkRows = 1 << 16;
+constexpr int32_t kSmallLongRow = 64;
+constexpr int32_t kSmallChunk = 256;
+
+// Builds the small-matrix tables from the plan's forward row offsets.  Runs
+// once, at the first reservation (it reads the offsets back from the device,
+// so it waits for the device); until then the plan's tables serve.
+int build_small_chunks(cbm_handle *h) {
+    const int32_t n = h->d.n_rows, L = h->d.n_levels;
+    std::vector<int32_t> ptr(size_t(n) + 1);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(ptr.data(), h->d.fwd_ptr, sizeof(int32_t) * ptr.size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return device_error(e, "cbm_reserve (small-matrix chunks)");
+    std::vector<int32_t> ck_pos, ck_meta, ck_slot, red_pos, red_slot, red_cnt;
+    h->sc_ck_lvl.assign(size_t(L) + 1, 0);
+    h->sc_red_lvl.assign(size_t(L) + 1, 0);
+    int32_t slots = 0;
+    for (int32_t l = 0; l < L; ++l) {
+        h->sc_ck_lvl[size_t(l)] = int32_t(ck_pos.size());
+        h->sc_red_lvl[size_t(l)] = int32_t(red_pos.size());
+        for (int32_t p = h->lvl_ptr[size_t(l)]; p < h->lvl_ptr[size_t(l) + 1]; ++p) {
+            const int64_t len = int64_t(ptr[size_t(p) + 1]) - ptr[size_t(p)];
+            if (len <= kSmallLongRow) continue;
+            const int64_t cnt = std::min<int64_t>((len + kSmallChunk - 1) / kSmallChunk, 0xFFFF);
+            for (int64_t c = 0; c < cnt; ++c) {
+                ck_pos.push_back(p);
+                ck_meta.push_back(int32_t(uint32_t(c) | (uint32_t(cnt) << 16)));
+                ck_slot.push_back(cnt > 1 ? slots + int32_t(c) : -1);
+            }
+            if (cnt > 1) {
+                red_pos.push_back(p);
+                red_slot.push_back(slots);
+                red_cnt.push_back(int32_t(cnt));
+                slots += int32_t(cnt);
+            }
+        }
+    }
+    h->sc_ck_lvl[size_t(L)] = int32_t(ck_pos.size());
+    h->sc_red_lvl[size_t(L)] = int32_t(red_pos.size());
+    const size_t nc = ck_pos.size(), nr = red_pos.size();
+    if (nc + nr > 0) {
+        std::vector<int32_t> all;
+        all.reserve(3 * (nc + nr));
+        for (const auto *v : {&ck_pos, &ck_meta, &ck_slot, &red_pos, &red_slot, &red_cnt})
+            all.insert(all.end(), v->begin(), v->end());
+        e = hipMalloc(&h->sc_buf, sizeof(int32_t) * all.size());
+        if (e == hipSuccess) e = hipMemcpy(h->sc_buf, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (h->sc_buf) (void)hipFree(h->sc_buf);
+            h->sc_buf = nullptr;
+            return device_error(e, "cbm_reserve (small-matrix chunks)");
+        }
+        h->sc_ck_pos = h->sc_buf;
+        h->sc_ck_meta = h->sc_buf + nc;
+        h->sc_ck_slot = h->sc_buf + 2 * nc;
+        h->sc_red_pos = h->sc_buf + 3 * nc;
+        h->sc_red_slot = h->sc_buf + 3 * nc + nr;
+        h->sc_red_cnt = h->sc_buf + 3 * nc + 2 * nr;
+    }
+    h->sc_slots = slots;
+    h->sc_on = true;
     return CBM_OK;
 }
 
@@ -450,15 +532,28 @@ int forward_levels(cbm_handle *h, const Bufs &b, cbmk::RowPass rp, bool with_par
     rp.comp = b.comp;
     rp.comp_base = comp_base(h);
     rp.ldc = rp.ncols;
+    if (h->sc_on) {
+        // The small-matrix tables: a level's long rows are not at its front,
+        // so its short-row pass covers the whole level and skips them.
+        ck.pos = h->sc_ck_pos;
+        ck.meta = h->sc_ck_meta;
+        ck.slot = h->sc_ck_slot;
+        rd.pos = h->sc_red_pos;
+        rd.slot = h->sc_red_slot;
+        rd.cnt = h->sc_red_cnt;
+        rp.skip_above = kSmallLongRow;
+    }
     for (int32_t l = 0; l < h->d.n_levels; ++l) {
         rp.extra = (with_parent && l > 0) ? (kahan_at(h, l) ? cbmk::kExtraParentKahan : cbmk::kExtraParent)
                                           : cbmk::kExtraNone;
-        ck.first = h->fck_ptr[size_t(l)];
-        ck.count = h->fck_ptr[size_t(l) + 1] - ck.first;
-        rd.first = h->fred_ptr[size_t(l)];
-        rd.count = h->fred_ptr[size_t(l) + 1] - rd.first;
-        const int rc = run_range(h, rp, h->lvl_ptr[size_t(l)], h->lvl_ptr[size_t(l) + 1], h->lvl_nlong[size_t(l)], ck,
-                                 rd, stream, what);
+        const std::vector<int32_t> &cl = h->sc_on ? h->sc_ck_lvl : h->fck_ptr;
+        const std::vector<int32_t> &rl = h->sc_on ? h->sc_red_lvl : h->fred_ptr;
+        ck.first = cl[size_t(l)];
+        ck.count = cl[size_t(l) + 1] - ck.first;
+        rd.first = rl[size_t(l)];
+        rd.count = rl[size_t(l) + 1] - rd.first;
+        const int rc = run_range(h, rp, h->lvl_ptr[size_t(l)], h->lvl_ptr[size_t(l) + 1],
+                                 h->sc_on ? 0 : h->lvl_nlong[size_t(l)], ck, rd, stream, what);
         if (rc != CBM_OK) return rc;
     }
     return CBM_OK;
@@ -492,6 +587,17 @@ int delta_flat(cbm_handle *h, const Bufs &b, cbmk::RowPass rp, hipStream_t strea
     rd.scratch = b.scratch;
     rp.extra = cbmk::kExtraNone;
     rp.skip_above = CBM_LONG_ROW;
+    if (h->sc_on) {
+        ck.pos = h->sc_ck_pos;
+        ck.meta = h->sc_ck_meta;
+        ck.slot = h->sc_ck_slot;
+        ck.count = h->sc_ck_lvl[size_t(L)];
+        rd.pos = h->sc_red_pos;
+        rd.slot = h->sc_red_slot;
+        rd.cnt = h->sc_red_cnt;
+        rd.count = h->sc_red_lvl[size_t(L)];
+        rp.skip_above = kSmallLongRow;
+    }
     if (order_on(h)) {
         // The short rows in the gather order, XCD by XCD; the long rows by
         // their chunks as ever.
@@ -889,7 +995,8 @@ extern "C" int cbm_destroy(cbm_handle *h) {
     for (void *p : {static_cast<void *>(h->ps_pos), static_cast<void *>(h->scratch), static_cast<void *>(h->comp),
                     static_cast<void *>(h->partial), static_cast<void *>(h->fused_state),
                     static_cast<void *>(h->fused_flags), static_cast<void *>(h->tf_order),
-                    static_cast<void *>(h->order_meta), static_cast<void *>(h->arrive)}) {
+                    static_cast<void *>(h->order_meta), static_cast<void *>(h->arrive),
+                    static_cast<void *>(h->sc_buf)}) {
         if (!p) continue;
         const hipError_t ei = hipFree(p);
         if (e == hipSuccess) e = ei;
@@ -906,6 +1013,7 @@ extern "C" int cbm_reserve(cbm_handle *h, int32_t n_cols) {
     }
     int rc = CBM_OK;
     if (!h->tf_order && h->d.tf_level < h->d.n_levels) rc = stage_forward_tail(h);
+    if (rc == CBM_OK && !h->sc_on && h->d.n_rows <= kSmallChunkRows) rc = build_small_chunks(h);
     if (rc != CBM_OK) return rc;
     const int32_t cols = std::max(h->reserved_cols, n_cols);
     rc = grow_buffers(h, cols);
@@ -923,7 +1031,7 @@ namespace {
 // to the tree tail, then the tail).
 int64_t forward_launches(const cbm_handle *h) {
     const int32_t L = h->d.n_levels;
-    const int64_t n_multi = h->fred_ptr[size_t(L)];
+    const int64_t n_multi = h->sc_on ? h->sc_red_lvl[size_t(L)] : h->fred_ptr[size_t(L)];
     const bool finish = h->arrive && h->xcd_slabs <= 1 && h->chunk_finish != 0 &&
                         (h->chunk_finish == 1 || (h->d.n_rows <= kChunkFinishMaxRows && n_multi <= kChunkFinishMaxLong));
     int64_t n = 1 + (n_multi > 0 && !finish ? 1 : 0);
@@ -1396,7 +1504,7 @@ extern "C" int cbm_matmul_noalloc(cbm_handle *h, int32_t dtype, const void *x, i
     if (rc != CBM_OK) return rc;
     // The long-row partial sums of the forward form in use, then the
     // compensation rows, each start 256-byte aligned.
-    const int64_t slots = use_split(h) ? h->d.fwd_gslots : h->d.fwd_slots;
+    const int64_t slots = h->sc_on ? h->sc_slots : use_split(h) ? h->d.fwd_gslots : h->d.fwd_slots;
     const int64_t part = (slots * n_cols + 63) / 64 * 64;
     const int64_t comp = int64_t(h->d.n_rows - comp_base(h)) * n_cols;
     const int64_t need = (part + comp) * int64_t(sizeof(float));
