@@ -374,17 +374,26 @@ int noalloc_scratch_address(float **out) {
     return CBM_OK;
 }
 
-// Matrices of up to kSmallChunkRows rows run their forward Delta pass with
-// long-row tables of their own: rows of more than kSmallLongRow entries (the
-// plan's limit is CBM_LONG_ROW) in chunks of about kSmallChunk (CBM_CHUNK).
-// Such a pass does not fill the device, and its time is its longest chain of
-// dependent gathers: a row of 256 entries on one wavefront, or a 1024-entry
-// chunk, is 16 steps of a lane group, a 256-entry chunk 4.  N = 19,717,
-// F = 128: step 0.0377 -> 0.0306 ms BF16.  The adjoint's reverse and Delta^T
-// passes keep the plan's tables.
-constexpr int32_t kSmallChunkRows = 1 << 16;
+// Small or sparse matrices run their forward Delta pass with long-row tables
+// of their own: rows of more than kSmallLongRow entries (the plan's limit is
+// CBM_LONG_ROW) in chunks of about kSmallChunk (CBM_CHUNK).  Such a pass
+// does not fill the device, or fills it with short rows, and its time is its
+// longest chains of dependent gathers: a row of 256 entries on one wavefront,
+// or a 1024-entry chunk, is 16 steps of a lane group, a 256-entry chunk 4.
+// N = 19,717, F = 128: step 0.0377 -> 0.0295 ms BF16; N = 169,343 with 14
+// entries per row: 0.209 -> 0.176 ms.  Matrices of long rows (N = 132,534
+// with 300 entries per row: 1.23 -> 1.52 ms at F = 64) keep the plan's
+// tables, and so do the adjoint's reverse and Delta^T passes.
+constexpr int32_t kSmallChunkRows = 1 << 16;  // any matrix up to this size
+constexpr int32_t kSparseChunkRows = 1 << 18; // up to this size with at most
+constexpr int64_t kSparseChunkAvg = 32;       //   this many entries per row
 constexpr int32_t kSmallLongRow = 64;
 constexpr int32_t kSmallChunk = 256;
+
+bool small_tables_wanted(const cbm_handle *h) {
+    const int64_t n = h->d.n_rows;
+    return n <= kSmallChunkRows || (n <= kSparseChunkRows && h->d.nnz <= kSparseChunkAvg * n);
+}
 
 // Builds the small-matrix tables from the plan's forward row offsets.  Runs
 // once, at the first reservation (it reads the offsets back from the device,
@@ -1013,7 +1022,7 @@ extern "C" int cbm_reserve(cbm_handle *h, int32_t n_cols) {
     }
     int rc = CBM_OK;
     if (!h->tf_order && h->d.tf_level < h->d.n_levels) rc = stage_forward_tail(h);
-    if (rc == CBM_OK && !h->sc_on && h->d.n_rows <= kSmallChunkRows) rc = build_small_chunks(h);
+    if (rc == CBM_OK && !h->sc_on && small_tables_wanted(h)) rc = build_small_chunks(h);
     if (rc != CBM_OK) return rc;
     const int32_t cols = std::max(h->reserved_cols, n_cols);
     rc = grow_buffers(h, cols);
