@@ -71,9 +71,10 @@ struct cbm_handle {
     uint32_t *arrive = nullptr; // a counter per long-row scratch slot, zero between launches (cbm_reserve)
     int64_t arrive_count = 0;
     int32_t chunk_finish = -1; // multi-chunk rows finished by their last chunk: 1 on, 0 off, -1 by size
-    // Small matrices: the forward Delta pass's own long-row tables (rows of
-    // more than kSmallLongRow entries in chunks of about kSmallChunk), by
-    // position, with slots numbered over all levels; one owned allocation.
+    // Small or sparse matrices: the forward Delta pass's own long-row tables
+    // (rows of more than kSmallLongRow entries in chunks of about
+    // kSmallChunk), by position, with slots numbered over all levels; one
+    // owned allocation, built at the first cbm_reserve.
     bool sc_on = false;
     int32_t *sc_buf = nullptr;
     const int32_t *sc_ck_pos = nullptr, *sc_ck_meta = nullptr, *sc_ck_slot = nullptr;
