@@ -4,6 +4,8 @@ their in-order reduction, forward and adjoint, on a dense graph with hubs
 These tests exercise the multi-launch forms, so the handles here run with
 the one-launch forward off and the adjoint through the Delta^T pass
 (test_gpu_fused.py covers those two)."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -581,3 +583,43 @@ def test_chunk_finish_replays_from_a_graph(hub, cuda, dtype):
     assert O.max_rel_error(y.cpu().numpy(), O.forward(a, xf, "dadx")) <= 1e-5
     assert O.max_rel_error(dx.cpu().numpy(), O.adjoint(a, xf, "dadx")) <= 1e-5
     h.set_chunk_finish(None)
+
+
+def test_small_matrix_tables_and_plan_tables_agree(cuda):
+    """Small matrices run the forward Delta pass with long-row tables of their
+    own (rows above 64 entries in 256-entry chunks), built at the first
+    reservation; before it, the allocation-free entry point runs the plan's
+    tables (rows above 256 entries in 1024-entry chunks).  Integer inputs give
+    the exact product either way; random inputs agree to FP32 rounding."""
+    ei = hub_graph()
+    a = O.csr_of(ei.numpy(), HUB_N)
+    h = _multi_launch(cbm4mm(ei, torch.ones(ei.shape[1]), alpha=0, device=cuda)).cusparse_extension
+    lens = np.diff(h._dev["fwd_ptr"].cpu().numpy())
+    assert ((lens > 64) & (lens <= 256)).any() and (lens > 256).any()
+    assert h._reserved == 0  # nothing has reserved this handle yet: the plan's tables serve
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def noalloc(x, y):
+        code = 1 if x.dtype == torch.bfloat16 else 0
+        return h._lib.cbm_matmul_noalloc(h._h, code, x.data_ptr(), x.stride(0), y.data_ptr(), y.stride(0),
+                                         int(x.size(1)), st)
+
+    f = 16
+    xi = np.random.default_rng(5).integers(-3, 4, size=(HUB_N, f)).astype(np.float32)
+    xr = _x((HUB_N, f), 6)
+    plan_out = []
+    for x in (xi, xr):
+        y = torch.full((HUB_N, f), float("nan"), device=cuda)
+        assert noalloc(torch.from_numpy(x).to(cuda), y) == 0, h._lib.cbm_last_error()
+        plan_out.append(y)
+    torch.cuda.synchronize()
+    h.reserve(f)  # builds the small-matrix tables
+    for x, y_plan in zip((xi, xr), plan_out):
+        y = torch.full((HUB_N, f), float("nan"), device=cuda)
+        h.forward(torch.from_numpy(x).to(cuda), y)
+        if x is xi:
+            assert torch.equal(y, y_plan)
+            assert np.array_equal(y.cpu().numpy().astype(np.float64), O.forward(a, x, "ax"))
+        else:
+            assert O.max_rel_error(y.cpu().numpy(), y_plan.cpu().numpy().astype(np.float64)) <= 1e-6
+            assert O.max_rel_error(y.cpu().numpy(), O.forward(a, x, "ax")) <= 1e-5
