@@ -289,6 +289,33 @@ class cusparseCBM:
         if y.stride(1) != 1 or y.stride(0) < n_cols:
             raise ValueError(f"{name} must have unit column stride")
 
+    def _operands(self, x, y, name):
+        """(x, dtype code, F, ld of x, ld of y) after the checks of _check_in
+        and _check_out, read in one pass over the tensors' attributes when
+        they hold (a product call on a small graph takes a few microseconds,
+        so its host-side checks are a large part of it); otherwise the full
+        checks, which raise or make x contiguous."""
+        xs, ys = x.shape, y.shape
+        xst, yst = x.stride(), y.stride()
+        dt = x.dtype
+        idx = self.device.index
+        if (x.is_cuda and y.is_cuda and len(xs) == 2 and xs[0] == self.num_rows and ys == xs
+                and (dt is torch.float32 or dt is torch.bfloat16) and y.dtype is torch.float32
+                and xst[1] == 1 and xst[0] >= xs[1] and yst[1] == 1 and yst[0] >= ys[1]
+                and x.get_device() == idx and y.get_device() == idx):
+            return x, (F32 if dt is torch.float32 else BF16), xs[1], xst[0], yst[0]
+        x, code = self._check_in(x)
+        f = int(x.size(1))
+        self._check_out(y, f, name)
+        return x, code, f, x.stride(0), y.stride(0)
+
+    def _stream(self):
+        """torch's current stream on this matrix's device, as a HIP stream
+        handle (an int, which ctypes passes as a pointer)."""
+        if _raw_stream is not None:
+            return _raw_stream(self.device.index)
+        return current_stream_handle(self.device)
+
     def _ensure_width(self, n_cols):
         """Reserves the native scratch for n_cols columns if the handle serves
         fewer (the native product calls never allocate).  Under HIP-graph
@@ -303,15 +330,14 @@ class cusparseCBM:
     # --------------------------------------------------------------- products
     def forward(self, rhs, res):
         """res = M rhs for this handle's product M (A, A D or D A D)."""
-        x, code = self._check_in(rhs)
-        f = int(x.size(1))
-        self._check_out(res, f)
+        x, code, f, ldx, ldy = self._operands(rhs, res, "res")
         if f == 0:
             return 0
-        self._ensure_width(f)
-        rc = self._lib.cbm_matmul(self._h, code, x.data_ptr(), x.stride(0), res.data_ptr(), res.stride(0), f,
-                                  current_stream_handle(self.device))
-        _lib.check_hip(rc, "matmul")
+        if f > self._reserved:
+            self._ensure_width(f)
+        rc = self._lib.cbm_matmul(self._h, code, x.data_ptr(), ldx, res.data_ptr(), ldy, f, self._stream())
+        if rc != _lib.CBM_OK:
+            _lib.check_hip(rc, "matmul")
         return 0
 
     def matmul_AX_ADX(self, rhs, res):
@@ -332,20 +358,20 @@ class cusparseCBM:
 
     def matmul_t(self, grad, out, workspace=None):
         """out = M^T grad.  ``workspace`` (FP32, N x F) is allocated if not given."""
-        g, code = self._check_in(grad)
-        f = int(g.size(1))
-        self._check_out(out, f, "out")
+        g, code, f, ldg, ldo = self._operands(grad, out, "out")
         if f == 0:
             return 0
         if workspace is not None:
             self._check_out(workspace, f, "workspace")
         elif not self._symmetric:  # the forward-structure adjoint needs none
             workspace = torch.empty((self.num_rows, f), dtype=torch.float32, device=self.device)
-        self._ensure_width(f)
+        if f > self._reserved:
+            self._ensure_width(f)
         ws_ptr, ws_ld = (workspace.data_ptr(), workspace.stride(0)) if workspace is not None else (None, f)
-        rc = self._lib.cbm_matmul_t(self._h, code, g.data_ptr(), g.stride(0), out.data_ptr(), out.stride(0),
-                                    ws_ptr, ws_ld, f, current_stream_handle(self.device))
-        _lib.check_hip(rc, "matmul_t")
+        rc = self._lib.cbm_matmul_t(self._h, code, g.data_ptr(), ldg, out.data_ptr(), ldo, ws_ptr, ws_ld, f,
+                                    self._stream())
+        if rc != _lib.CBM_OK:
+            _lib.check_hip(rc, "matmul_t")
         return 0
 
     def adjoint_halves(self, grad, out, workspace):
