@@ -497,8 +497,9 @@ class cusparseCBM:
 
     def set_wave_rows(self, mode):
         """Row mapping of the short-row passes: True a whole wavefront per row,
-        False rows packed per wavefront, None by matrix size (the default)."""
-        m = -1 if mode is None else int(bool(mode))
+        16 sixteen lanes per row (for rows narrower than that), False rows
+        packed per wavefront, None by matrix size (the default)."""
+        m = -1 if mode is None else 2 if (mode == 16 and not isinstance(mode, bool)) else int(bool(mode))
         _lib.check_hip(self._lib.cbm_set_wave_rows(self._h, m), "set_wave_rows")
 
     def set_mixed_grid(self, mode):

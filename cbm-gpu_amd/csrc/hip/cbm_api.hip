@@ -1551,7 +1551,7 @@ extern "C" int cbm_set_forward_split(cbm_handle *h, int32_t mode) {
 }
 
 extern "C" int cbm_set_wave_rows(cbm_handle *h, int32_t mode) {
-    if (!h || mode < -1 || mode > 1) {
+    if (!h || mode < -1 || mode > 2) {
         set_error("cbm_set_wave_rows: invalid arguments");
         return CBM_ERR_ARG;
     }

@@ -66,8 +66,11 @@ struct RowPass {
     // fill the device's wave slots, the short-row pass gives each row a whole
     // wavefront: 64 / LPR sub-groups sum slices of its entries, which are
     // then added across sub-groups.  This trades occupancy that would idle
-    // anyway for fewer dependent gather rounds per row.  wave_rows: -1 by
-    // that rule, 0 never, 1 always.
+    // anyway for fewer dependent gather rounds per row.  Where more rows than
+    // wave slots would take several rounds of wavefronts, rows narrower than
+    // 16 lanes get 16 lanes (16 / LPR sub-groups, four rows per wavefront)
+    // instead.  wave_rows: -1 by that rule, 0 never, 1 always a wavefront,
+    // 2 always 16 lanes.
     int32_t rows_hint = 0;
     int32_t wave_rows = -1;
     // Long-row chunks and short rows of one range (launch_row_mixed): one

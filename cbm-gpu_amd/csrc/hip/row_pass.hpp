@@ -663,14 +663,24 @@ struct Launch {
     int *launched = nullptr;  // kMixed: kernels launched
 };
 
-// Whether a short-row pass gives each row a whole wavefront (see RowPass).
+// The lanes a short-row pass gives each row (see RowPass::wave_rows): 64 (a
+// wavefront per row, its entries in 64 / LPR slices), 16 (four rows per
+// wavefront, 16 / LPR slices each; rows of fewer than 16 lanes only) or LPR
+// (rows packed).  By rule: a wavefront per row while the rows cannot fill the
+// device's wave slots packed and fit them in one round as wavefronts; 16
+// lanes per row where more rows than slots would need several rounds of
+// wavefronts; packed where packed rows fill the slots.
 template <int LPR>
-bool wave_per_row(const RowPass &rp) {
+int row_lanes(const RowPass &rp) {
     // XCD slabs are for matrices that fill the chip; with them, rows stay packed.
-    if (LPR >= 64 || rp.wave_rows == 0 || rp.xcd_slabs > 1) return false;
-    if (rp.wave_rows > 0) return true;
-    return rp.rows_hint > 0 && int64_t(rp.rows_hint) * LPR < 64 * kWaveSlots;
+    if (LPR >= 64 || rp.wave_rows == 0 || rp.xcd_slabs > 1) return LPR;
+    if (rp.wave_rows == 1) return 64;
+    if (rp.wave_rows == 2) return LPR < 16 ? 16 : LPR;
+    if (rp.rows_hint <= 0 || int64_t(rp.rows_hint) * LPR >= 64 * kWaveSlots) return LPR;
+    if (LPR < 16 && int64_t(rp.rows_hint) > kWaveSlots) return 16;
+    return 64;
 }
+
 
 // Whether one grid of k_row_mixed keeps as many waves per CU as the chunk
 // kernel and the short-row pass it stands for (it needs the registers of the
@@ -714,10 +724,13 @@ hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
         constexpr int kBs = (16 * LPR > 256) ? 16 * LPR : 256;
         constexpr int64_t kWaves = kBs / 64;
         constexpr int kWaveS = LPR < 64 ? 64 / LPR : 1;
+        constexpr int kMidS = LPR < 16 ? 16 / LPR : 1;
         constexpr int64_t kRows = kWaves * (64 / LPR);
-        bool wave = false;
-        if constexpr (LPR < 64) wave = wave_per_row<LPR>(rp);
-        const int64_t pass_blocks = (wave ? (n + kWaves - 1) / kWaves : (n + kRows - 1) / kRows) * ns;
+        const int lanes = row_lanes<LPR>(rp);
+        const bool wave = lanes == 64;
+        const bool mid = lanes == 16 && LPR < 16;
+        const int64_t per_block = wave ? kWaves : mid ? kWaves * 4 : kRows;
+        const int64_t pass_blocks = (n + per_block - 1) / per_block * ns;
         *lk.launched = 0;
         if (chunk_blocks + pass_blocks == 0) return hipSuccess;
         // One grid gains the wait between the two launches and lets the
@@ -729,8 +742,9 @@ hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
         const bool one =
             rp.mixed_grid > 0 ||
             (rp.mixed_grid < 0 && (small || chunk_blocks <= pass_blocks ||
-                                   (wave ? mixed_keeps_occupancy<P, VE, LPR, kWaveS, kBs>()
-                                         : mixed_keeps_occupancy<P, VE, LPR, 1, kBs>())));
+                                   (wave  ? mixed_keeps_occupancy<P, VE, LPR, kWaveS, kBs>()
+                                    : mid ? mixed_keeps_occupancy<P, VE, LPR, kMidS, kBs>()
+                                          : mixed_keeps_occupancy<P, VE, LPR, 1, kBs>())));
         if (!one) {
             const hipError_t e = launch_one<P, VE, LPR>(rp, Launch{Kind::kChunk, lk.chunk, nullptr}, stream);
             if (e != hipSuccess) return e;
@@ -747,6 +761,13 @@ hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
                 return hipGetLastError();
             }
         }
+        if constexpr (LPR < 16) {
+            if (mid) {
+                k_row_mixed<P, VE, LPR, kMidS, kBs><<<dim3(unsigned(chunk_blocks + pass_blocks)), dim3(kBs), 0,
+                                                      stream>>>(r, *lk.chunk, uint32_t(chunk_blocks));
+                return hipGetLastError();
+            }
+        }
         k_row_mixed<P, VE, LPR, 1, kBs><<<dim3(unsigned(chunk_blocks + pass_blocks)), dim3(kBs), 0, stream>>>(
             r, *lk.chunk, uint32_t(chunk_blocks));
     } else if (lk.kind == Kind::kPass) {
@@ -754,10 +775,18 @@ hipError_t launch_one(const RowPass &rp, const Launch &lk, hipStream_t stream) {
         if (n <= 0) return hipSuccess;
         const int64_t ns = rp.xcd_slabs;
         RowPass r = rp;
+        const int lanes = row_lanes<LPR>(rp);
         if constexpr (LPR < 64) {
-            if (wave_per_row<LPR>(rp)) {
+            if (lanes == 64) {
                 r.order_blocks = int32_t((n + 3) / 4 * ns);
                 k_row_pass<P, VE, LPR, 64 / LPR><<<dim3(unsigned(r.order_blocks)), dim3(256), 0, stream>>>(r);
+                return hipGetLastError();
+            }
+        }
+        if constexpr (LPR < 16) {
+            if (lanes == 16) {
+                r.order_blocks = int32_t((n + 15) / 16 * ns);
+                k_row_pass<P, VE, LPR, 16 / LPR><<<dim3(unsigned(r.order_blocks)), dim3(256), 0, stream>>>(r);
                 return hipGetLastError();
             }
         }

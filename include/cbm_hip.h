@@ -325,11 +325,14 @@ int cbm_set_forward_split(cbm_handle *h, int32_t mode);
 /*
  * Row mapping of the short-row passes: 1 gives every row a whole wavefront
  * (its entries split into slices summed by sub-groups of lanes, then added
- * across them), 0 packs 64 / lanes-per-row rows into a wavefront, -1 (the
- * default) chooses the first when N rows cannot fill the device's wave slots
- * otherwise (small matrices, where per-row gather latency, not bandwidth,
- * sets the time).  The two mappings add a row's entries in a different
- * order; each is deterministic.
+ * across them), 2 gives rows narrower than 16 lanes 16 lanes (four rows per
+ * wavefront, in slices the same way), 0 packs 64 / lanes-per-row rows into a
+ * wavefront, -1 (the default) chooses the first when N rows cannot fill the
+ * device's wave slots packed and fit them in one round as wavefronts, the
+ * second when they cannot fill them packed but need several rounds as
+ * wavefronts (small matrices, where per-row gather latency, not bandwidth,
+ * sets the time).  The mappings add a row's entries in different orders;
+ * each is deterministic.
  */
 int cbm_set_wave_rows(cbm_handle *h, int32_t mode);
 

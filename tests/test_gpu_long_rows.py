@@ -123,12 +123,13 @@ def test_one_grid_and_two_launches_are_bit_identical(hub, cuda, mode, dtype, f):
     """The long-row chunks and the short rows of a pass run as one grid or as
     two launches (chunks first); every workgroup does the same work either
     way, so forward, Delta half and adjoint give the same bits in both forms,
-    for both row mappings and both forward forms.  At F = 256 the rows take
+    for every row mapping (a wavefront, 16 lanes or the row's own lanes per
+    row) and both forward forms.  At F = 256 the rows take
     64 lanes and 1024-thread workgroups."""
     a, mats = hub
     h = mats[mode].cusparse_extension
     xt = torch.from_numpy(_x((HUB_N, f), 31 + f)).to(cuda).to(dtype)
-    for wave, split in ((None, None), (True, True), (False, True), (None, False)):
+    for wave, split in ((None, None), (True, True), (16, True), (False, True), (None, False)):
         h.set_wave_rows(wave)
         h.set_forward_split(split)
         res = {}
