@@ -220,9 +220,15 @@ int cbm_destroy(cbm_handle *h);
  * that change what a width needs (cbm_set_kahan_depth, cbm_set_xcd_ranges)
  * grow the buffers for the reserved width themselves.  The first reservation
  * also orders the forward tail's fragments for staging in LDS (it reads the
- * tail tables back, waiting for the device; see cbm_set_tree_tails).  The
- * reference allocates a cuSPARSE workspace on every call instead
- * (cbm/cbm.cu:181,203).
+ * tail tables back, waiting for the device; see cbm_set_tree_tails), builds
+ * the forward Delta pass's long-row tables of a small or sparse matrix (rows
+ * of more than 64 entries in chunks of about 256, for up to 65,536 rows, or
+ * up to 262,144 with at most 32 entries per row; from the plan's row offsets,
+ * read back the same way), and allocates one arrival counter per scratch
+ * slot (cbm_set_chunk_finish).  The scratch and the counters serve one call
+ * at a time: product calls on one handle that overlap on the device (other
+ * streams, other threads) race.  The reference allocates a cuSPARSE
+ * workspace on every call instead (cbm/cbm.cu:181,203).
  */
 int cbm_reserve(cbm_handle *h, int32_t n_cols);
 
