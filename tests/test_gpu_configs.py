@@ -184,6 +184,32 @@ def test_c2_dad_full_graph(cuda, alpha):
         _check_products(wl, mat, "dadx", f)
 
 
+# --------------------------------------------------------------------- C3
+
+@pytest.mark.parametrize("over_a", [False, True])
+def test_c3_dad_full_graph(cuda, over_a):
+    """cbm4dad, N = 169,343, alpha = 16 (about 14 entries per row: the
+    forward Delta pass takes the small-matrix long-row tables, rows above 64
+    entries in 256-entry chunks, and the gather order), F in {128, 256},
+    compressed and over A (the form the cost model picks at this size):
+    forward and adjoint against FP64 on sampled rows, integer X exact over
+    A, and the gather order changes no bit."""
+    wl = workload("c3", cuda)
+    mat = CLASSES["dad"][0](wl.ei, torch.ones(wl.ei.shape[1]), alpha=16, device=cuda, uncompressed=over_a)
+    h = mat.cusparse_extension
+    if over_a:
+        h.attach_uncompressed(h._plain, True)
+        assert h.uncompressed_active(torch.float32, 128) == 1
+    _say(f"c3 over_a={over_a}: nnz(Delta)={h.nnz} depth={h.depth}")
+    for f in (128, 256):
+        _check_products(wl, mat, "dadx", f)
+    _check_order_bits(wl, mat, 128)
+    if over_a:
+        plain = h._plain.with_scaling(MODE_AX)
+        plain.reserve(128)
+        _check_integer(wl, plain, 128, 43)
+
+
 # --------------------------------------------------------------------- C4
 
 @pytest.fixture(scope="module")
