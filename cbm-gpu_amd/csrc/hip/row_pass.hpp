@@ -681,7 +681,6 @@ int row_lanes(const RowPass &rp) {
     return 64;
 }
 
-
 // Whether one grid of k_row_mixed keeps as many waves per CU as the chunk
 // kernel and the short-row pass it stands for (it needs the registers of the
 // larger part), asked of the runtime once per instantiation.
